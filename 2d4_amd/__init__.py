@@ -1,0 +1,267 @@
+"""2d4_amd -- MI355X (gfx950) implementation of 2D4's periodic pair-interaction path.
+
+Python mirror of the C ABI in include/d4.h (ctypes over 2d4_amd/lib/lib2d4.so).
+The host-side semantics follow the reference's Field / Simulation interface:
+
+  field_eval          sdddstCore::Field::xy / xy_diff_x       (Field.cpp:295-346)
+  speed_pass          Simulation::calculateSpeedsAtTime       (simulation.cpp:287-333)
+  jacobian_pass       Simulation::calcJacobianAndSpeedsAtTimes (simulation.cpp:345-527)
+  jacobian_rescale    Simulation::calcJacobianAndSpeedsFromPrev (simulation.cpp:546-608)
+  jacobian_dense      the Jacobian pass with an unlimited cutoff, dense output
+  vjp                 adjoint of the speed pass (no reference counterpart)
+
+Every call runs on the GPU through lib2d4; there is no CPU fallback: if the
+library is missing or no GPU is present, calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "lib2d4.so")
+
+D4_OK = 0
+D4_ERR_INVALID = -1
+D4_ERR_HIP = -2
+D4_ERR_CAPACITY = -3
+D4_ERR_NO_DEVICE = -4
+
+_P = C.POINTER
+_d = C.c_double
+_pd = _P(C.c_double)
+_pi = _P(C.c_int)
+_vp = C.c_void_p
+
+
+class D4Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lib2d4 error {code}: {msg}")
+        self.code = code
+
+
+class JacParams(C.Structure):
+    """d4_jac_params (include/d4.h); fields as SimulationData names them (simulation_data.h:181-197)."""
+    _fields_ = [("h", _d), ("tau_a", _d), ("tau_b", _d), ("cutoff", _d), ("cut_sq", _d),
+                ("one_per_cut_sq", _d), ("heaviside", C.c_int), ("wfunc", C.c_char), ("tol_cap2", _d)]
+
+
+# symbol -> (restype, argtypes); the set include/d4.h declares
+SIGNATURES = {
+    "d4_ctx_create": (C.c_int, [C.c_int, _P(_vp)]),
+    "d4_ctx_destroy": (None, [_vp]),
+    "d4_last_error": (C.c_char_p, [_vp]),
+    "d4_ctx_set_stream": (C.c_int, [_vp, _vp]),
+    "d4_ctx_synchronize": (C.c_int, [_vp]),
+    "d4_dev_alloc": (C.c_int, [_vp, C.c_int64, _P(_vp)]),
+    "d4_dev_free": (C.c_int, [_vp, _vp]),
+    "d4_memcpy_h2d": (C.c_int, [_vp, _vp, _vp, C.c_int64]),
+    "d4_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_int64]),
+    "d4_field_eval": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
+    "d4_field_eval_dev": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
+    "d4_speed_pass": (C.c_int, [_vp, C.c_int, _vp, _d, _vp, _vp, _d]),
+    "d4_speed_pass_dev": (C.c_int, [_vp, C.c_int, _vp, _d, _vp, _vp, _d]),
+    "d4_jacobian_pass": (C.c_int, [_vp, C.c_int, _vp, _P(JacParams), _vp, _vp, _vp, _vp, _vp, C.c_int,
+                                   _pi, _vp, _vp, _vp]),
+    "d4_jacobian_rescale": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_char, _d, _d, _vp]),
+    "d4_jacobian_dense_dev": (C.c_int, [_vp, C.c_int, _vp, _P(JacParams), _vp, _vp, _vp, C.c_int64, _vp, _vp]),
+    "d4_jacobian_dense_scale_dev": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, _vp]),
+    "d4_jacobian_dense": (C.c_int, [_vp, C.c_int, _vp, _P(JacParams), _vp, _vp, _vp, _vp, _vp, C.c_int]),
+    "d4_vjp_dev": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _pd, C.c_int]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads lib2d4.so (built by __graft_entry__.build()); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        from . import _bind_extra
+        _bind_extra.bind(L)
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """Owns a d4_ctx (device, stream, device buffers)."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.d4_ctx_create(device, C.byref(h))
+        if rc != D4_OK:
+            raise D4Error(rc, "d4_ctx_create failed (no usable GPU?)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.d4_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != D4_OK:
+            raise D4Error(rc, self.L.d4_last_error(self.h).decode(errors="replace"))
+        return rc
+
+    def set_stream(self, stream_handle):
+        self.check(self.L.d4_ctx_set_stream(self.h, C.c_void_p(stream_handle) if stream_handle else None))
+
+    def use_torch_stream(self):
+        import torch
+        self.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    def synchronize(self):
+        self.check(self.L.d4_ctx_synchronize(self.h))
+
+    # ---------------------------------------------------------- numpy API
+    def field_eval(self, dx, dy, e2pix=None, c2piy=None, exact=False, want_sigma=True, want_dsigma=True):
+        """Field::xy / xy_diff_x per pair (Field.cpp:295-346); returns (sxy, dsxy)."""
+        dx, dy = _f64(dx), _f64(dy)
+        n = dx.size
+        e = _f64(e2pix) if e2pix is not None else None
+        c = _f64(c2piy) if c2piy is not None else None
+        s = np.empty(n) if want_sigma else None
+        ds = np.empty(n) if want_dsigma else None
+        self.check(self.L.d4_field_eval(self.h, n, _ptr(dx), _ptr(dy), _ptr(e), _ptr(c), _ptr(s), _ptr(ds),
+                                        1 if exact else 0))
+        return s, ds
+
+    def speed_pass(self, xy, tau=0.0, tol=None, tol_cap2=0.0):
+        """calculateSpeedsAtTime (simulation.cpp:287-333); xy is (n, 2) sorted; tol updated in place."""
+        xy = _f64(xy).reshape(-1, 2)
+        n = xy.shape[0]
+        v = np.empty(n)
+        if tol is not None:
+            assert tol.dtype == np.float64 and tol.flags.c_contiguous and tol.size == n
+        self.check(self.L.d4_speed_pass(self.h, n, _ptr(xy), float(tau), _ptr(v), _ptr(tol), float(tol_cap2)))
+        return v
+
+    def jacobian_pass(self, xy, params, tol=None, cap=None):
+        """calcJacobianAndSpeedsAtTimes (simulation.cpp:345-527) -> dict with vA, vB, Ap, Ai, Ax, indexes, dvec."""
+        xy = _f64(xy).reshape(-1, 2)
+        n = xy.shape[0]
+        vA, vB, dvec = np.empty(n), np.empty(n), np.empty(n)
+        Ap = np.empty(n + 1, dtype=np.int32)
+        idx = np.empty(n, dtype=np.int32)
+        nnz = C.c_int(0)
+        cap = int(cap) if cap is not None else 0
+        for _ in range(2):
+            Ai = np.empty(max(cap, 1), dtype=np.int32)
+            Ax = np.empty(max(cap, 1))
+            rc = self.L.d4_jacobian_pass(self.h, n, _ptr(xy), C.byref(params), _ptr(vA), _ptr(vB), _ptr(Ap),
+                                         _ptr(Ai), _ptr(Ax), cap, C.byref(nnz), _ptr(idx), _ptr(dvec), _ptr(tol))
+            if rc == D4_ERR_CAPACITY:
+                cap = nnz.value
+                continue
+            self.check(rc)
+            break
+        k = nnz.value
+        return dict(vA=vA, vB=vB, Ap=Ap, Ai=Ai[:k].copy(), Ax=Ax[:k].copy(), indexes=idx, dvec=dvec, nnz=k)
+
+    def jacobian_rescale(self, Ap, Ai, Ax, dvec, wfunc, factor, dtau, speed=None):
+        """calcJacobianAndSpeedsFromPrev (simulation.cpp:546-608), arrays updated in place."""
+        n = dvec.size
+        for a in (Ax, dvec) + ((speed,) if speed is not None else ()):
+            assert a.dtype == np.float64 and a.flags.c_contiguous
+        Ap = np.ascontiguousarray(Ap, dtype=np.int32)
+        Ai = np.ascontiguousarray(Ai, dtype=np.int32)
+        self.check(self.L.d4_jacobian_rescale(self.h, n, _ptr(Ap), _ptr(Ai), _ptr(Ax), _ptr(dvec),
+                                              wfunc.encode() if isinstance(wfunc, str) else wfunc,
+                                              float(factor), float(dtau), _ptr(speed)))
+
+    def jacobian_dense(self, xy, params, tol=None, scale=False):
+        """Dense Jacobian pass: (vA, vB, M, dvec); M column-major as (col, row) = M[i][j] -> A[j, i]."""
+        xy = _f64(xy).reshape(-1, 2)
+        n = xy.shape[0]
+        vA, vB, dvec = np.empty(n), np.empty(n), np.empty(n)
+        M = np.empty((n, n))
+        self.check(self.L.d4_jacobian_dense(self.h, n, _ptr(xy), C.byref(params), _ptr(vA), _ptr(vB), _ptr(M),
+                                            _ptr(dvec), _ptr(tol), 1 if scale else 0))
+        return vA, vB, M.T.copy(), dvec  # returned in (row, col) orientation
+
+    # ---------------------------------------------------------- torch API (device tensors)
+    def speed_pass_t(self, xy, tau=0.0, tol=None, tol_cap2=0.0, out=None):
+        import torch
+        n = xy.shape[0]
+        v = out if out is not None else torch.empty(n, dtype=torch.float64, device=xy.device)
+        self.use_torch_stream()
+        self.check(self.L.d4_speed_pass_dev(self.h, n, xy.data_ptr(), float(tau), v.data_ptr(),
+                                            tol.data_ptr() if tol is not None else None, float(tol_cap2)))
+        return v
+
+    def jacobian_dense_t(self, xy, params, M=None, tol=None, scale=False):
+        import torch
+        n = xy.shape[0]
+        dev = xy.device
+        vA = torch.empty(n, dtype=torch.float64, device=dev)
+        vB = torch.empty(n, dtype=torch.float64, device=dev)
+        d = torch.empty(n, dtype=torch.float64, device=dev)
+        if M is None:
+            M = torch.empty((n, n), dtype=torch.float64, device=dev)  # M[col, row]
+        self.use_torch_stream()
+        self.check(self.L.d4_jacobian_dense_dev(self.h, n, xy.data_ptr(), C.byref(params), vA.data_ptr(),
+                                                vB.data_ptr(), M.data_ptr(), n, d.data_ptr(),
+                                                tol.data_ptr() if tol is not None else None))
+        if scale:
+            self.check(self.L.d4_jacobian_dense_scale_dev(self.h, n, M.data_ptr(), n, d.data_ptr()))
+        return vA, vB, M, d
+
+    def vjp_t(self, xy, w, exact=True, want_dy=True):
+        """(gx, gy, gtau) of L = sum_i w_i v_i(x, y, tau)."""
+        import torch
+        n = xy.shape[0]
+        gx = torch.empty(n, dtype=torch.float64, device=xy.device)
+        gy = torch.empty(n, dtype=torch.float64, device=xy.device) if want_dy else None
+        gt = C.c_double(0.0)
+        self.use_torch_stream()
+        self.check(self.L.d4_vjp_dev(self.h, n, xy.data_ptr(), w.data_ptr(), gx.data_ptr(),
+                                     gy.data_ptr() if gy is not None else None, C.byref(gt), 1 if exact else 0))
+        return gx, gy, gt.value
+
+
+def jac_params(h=0.0, tau_a=0.0, tau_b=0.0, n=None, cutoff_mult=None, heaviside=False, wfunc="c", tol_cap2=0.0,
+               cutoff=None):
+    """Builds d4_jac_params; cutoff from the multiplier as SimulationData::updateCutOff (simulation_data.cpp:253-262)."""
+    if cutoff is None:
+        if cutoff_mult is None:
+            cutoff = 1e20 / np.sqrt(n if n else 1)
+        else:
+            m = cutoff_mult
+            if not heaviside and cutoff_mult >= np.sqrt(2 * n) / 12:
+                m = 1e20
+            cutoff = m / np.sqrt(n)
+    cut_sq = cutoff * cutoff
+    return JacParams(h=h, tau_a=tau_a, tau_b=tau_b, cutoff=cutoff, cut_sq=cut_sq, one_per_cut_sq=1.0 / cut_sq,
+                     heaviside=1 if heaviside else 0, wfunc=wfunc.encode(), tol_cap2=tol_cap2)
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx

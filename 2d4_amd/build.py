@@ -1,0 +1,67 @@
+"""Builds lib2d4.so (gfx950 kernels + C ABI + host Simulation) in-tree with hipcc.
+
+The library goes to 2d4_amd/lib/lib2d4.so; nothing is installed elsewhere.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIBDIR, "lib2d4.so")
+
+ARCH = "gfx950"
+# -packed-fp32-ops: keep hipcc from forming packed FP32 instructions
+HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wall", "-Wno-unused-command-line-argument"]
+
+SOURCES = ["d4_kernels.hip", "d4_capi.hip", "d4_batch.hip", "d4_sim.hip", "d4_solver.cpp"]
+
+
+def _hipcc():
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed ({r.returncode}): {' '.join(cmd[:3])} ...")
+    return r
+
+
+def build_lib(verbose=False):
+    """Compiles every source under csrc/ into lib2d4.so (always recompiles)."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    tmp = LIB + ".tmp"
+    cmd = [_hipcc()] + HIPFLAGS + ["-shared", "-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
+    _run(cmd, verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_test_helpers(verbose=False):
+    """Test-only helpers under tests/csrc (the kernel math compiled for the host)."""
+    src = os.path.join(ROOT, "tests", "csrc", "hostmath.hip")
+    if not os.path.exists(src):
+        return None
+    out = os.path.join(ROOT, "tests", "csrc", "libhostmath.so")
+    cmd = [_hipcc(), "-O2", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
+           "-Wno-unused-command-line-argument", "-o", out, src]
+    _run(cmd, verbose)
+    return out
+
+
+if __name__ == "__main__":
+    print(build_lib(verbose=True))
+    print(build_test_helpers(verbose=True))

@@ -1,0 +1,505 @@
+// d4_capi.hip -- the C ABI of include/d4.h over the gfx950 kernels.
+#include "../../include/d4.h"
+#include "d4_ctx.h"
+#include "d4_field.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace d4;
+
+d4_ctx::~d4_ctx()
+{
+    for (auto* b : {&part, &tiles, &xy, &v0, &v1, &tol, &coo, &coo_count, &cnt, &cursor, &Ap, &Ai, &Ax, &idx, &dvec,
+                    &nw, &aux0, &aux1, &aux2})
+        b->release();
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+}
+
+int d4_ctx::fail(int code, const std::string& msg)
+{
+    err = msg;
+    return code;
+}
+
+int d4_ctx::hip_fail(hipError_t e, const char* where)
+{
+    err = std::string(where) + ": " + hipGetErrorString(e);
+    return D4_ERR_HIP;
+}
+
+int d4_ctx::ensure(DevBuf& b, size_t bytes, const char* what)
+{
+    if (b.bytes >= bytes) return D4_OK;
+    b.release();
+    size_t cap = std::max(bytes, (size_t)256);
+    hipError_t e = hipMalloc(&b.p, cap);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return hip_fail(e, what);
+    }
+    b.bytes = cap;
+    return D4_OK;
+}
+
+// Upper-triangle tile table of one realization of n (row-major, slot = index).
+int d4_ctx::single_tiles(int n, int* ntiles)
+{
+    const int nb = (n + kTile - 1) / kTile;
+    const int nt = nb * (nb + 1) / 2;
+    *ntiles = nt;
+    if (tiles_n == n) return D4_OK;
+    std::vector<TileDesc> t;
+    t.reserve(nt);
+    int slot = 0;
+    for (int I = 0; I < nb; ++I)
+        for (int J = I; J < nb; ++J) t.push_back(TileDesc{0, I * kTile, J * kTile, slot++});
+    int rc = ensure(tiles, sizeof(TileDesc) * (size_t)nt, "tile table");
+    if (rc) return rc;
+    hipError_t e = hipMemcpy(tiles.p, t.data(), sizeof(TileDesc) * (size_t)nt, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "tile table upload");
+    tiles_n = n;
+    return D4_OK;
+}
+
+static RealParams single_params(int n, const d4_jac_params* p, double tau)
+{
+    RealParams r{};
+    r.nb = (n + kTile - 1) / kTile;
+    r.slot0 = 0;
+    if (p) {
+        r.h = p->h;
+        r.tauA = p->tau_a;
+        r.tauB = p->tau_b;
+        r.cut_sq = p->cut_sq;
+        r.cutoff = p->cutoff;
+        r.one_per = p->one_per_cut_sq;
+        r.range_sq = (p->heaviside ? 1.0 : 49.0) * p->cut_sq;  // simulation.cpp:461-465
+        r.heaviside = p->heaviside;
+        r.wfunc = p->wfunc;
+        r.tol_cap2 = p->tol_cap2;
+    } else {
+        r.tauA = r.tauB = tau;
+        r.wfunc = 'c';
+    }
+    return r;
+}
+
+// Runs one pair pass over a single realization whose positions are already on
+// the device, plus the reduction.  Outputs are device pointers.
+int d4_ctx::single_pass(int mode, int n, const double* d_xy, const RealParams& rp, double* d_tol, double* d_vA,
+                        double* d_vB, double* d_q1, double* d_dvec, double* d_M, int64_t ld, const double* d_w,
+                        int exact, int want_dy, int add_tau)
+{
+    int ntiles = 0;
+    int rc = single_tiles(n, &ntiles);
+    if (rc) return rc;
+    const int nq = pass_nq(mode);
+    rc = ensure(part, sizeof(double) * 128 * (size_t)nq * (size_t)ntiles, "partials");
+    if (rc) return rc;
+    PassArgs a{};
+    a.xy = reinterpret_cast<const double2*>(d_xy);
+    a.off = nullptr;
+    a.n0 = n;
+    a.tiles = static_cast<const TileDesc*>(tiles.p);
+    a.ntiles = ntiles;
+    a.rp = nullptr;
+    a.rp0 = rp;
+    a.part = static_cast<double*>(part.p);
+    a.tol = d_tol;
+    a.M = d_M;
+    a.ld = ld;
+    a.coo = static_cast<CooEntry*>(coo.p);
+    a.coo_count = static_cast<int*>(coo_count.p);
+    a.coo_cap = (int)(coo.bytes / sizeof(CooEntry));
+    a.w = d_w;
+    a.exact = exact;
+    a.want_dy = want_dy;
+    hipError_t e = launch_pair_pass(mode, a, stream);
+    if (e != hipSuccess) return hip_fail(e, "pair pass launch");
+    ReduceArgs r{};
+    r.part = a.part;
+    r.nq = nq;
+    r.off = nullptr;
+    r.rp = nullptr;
+    r.rp0 = rp;
+    r.R = 1;
+    r.total = n;
+    r.elem_real = nullptr;
+    r.vA = d_vA;
+    r.vB = d_vB;
+    r.q1 = d_q1;
+    r.dvec = d_dvec;
+    r.M = d_M;
+    r.ld = ld;
+    r.add_tau = add_tau;
+    e = launch_reduce(r, stream);
+    if (e != hipSuccess) return hip_fail(e, "reduce launch");
+    return D4_OK;
+}
+
+int d4_ctx::sync(const char* where)
+{
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return hip_fail(e, where);
+    return D4_OK;
+}
+
+int d4_ctx::h2d(void* dst, const void* src, size_t bytes, const char* where)
+{
+    if (!bytes) return D4_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return hip_fail(e, where);
+    return D4_OK;
+}
+
+int d4_ctx::d2h(void* dst, const void* src, size_t bytes, const char* where)
+{
+    if (!bytes) return D4_OK;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream);
+    if (e != hipSuccess) return hip_fail(e, where);
+    return D4_OK;
+}
+
+#define D4_TRY(x)                 \
+    do {                          \
+        int rc_ = (x);            \
+        if (rc_) return rc_;      \
+    } while (0)
+
+extern "C" {
+
+int d4_ctx_create(int device, d4_ctx** out)
+{
+    if (!out) return D4_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return D4_ERR_NO_DEVICE;
+    if (device < 0 || device >= count) return D4_ERR_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return D4_ERR_HIP;
+    d4_ctx* c = new d4_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return D4_ERR_HIP;
+    }
+    c->stream = c->own_stream;
+    if (c->ensure(c->coo_count, sizeof(int) * 4, "coo counter")) {
+        delete c;
+        return D4_ERR_HIP;
+    }
+    *out = c;
+    return D4_OK;
+}
+
+void d4_ctx_destroy(d4_ctx* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    delete ctx;
+}
+
+const char* d4_last_error(const d4_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int d4_ctx_set_stream(d4_ctx* ctx, void* stream)
+{
+    if (!ctx) return D4_ERR_INVALID;
+    ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+    return D4_OK;
+}
+
+int d4_ctx_synchronize(d4_ctx* ctx)
+{
+    if (!ctx) return D4_ERR_INVALID;
+    return ctx->sync("synchronize");
+}
+
+int d4_dev_alloc(d4_ctx* ctx, int64_t bytes, void** out)
+{
+    if (!ctx || !out || bytes < 0) return D4_ERR_INVALID;
+    hipError_t e = hipMalloc(out, (size_t)std::max<int64_t>(bytes, 8));
+    if (e != hipSuccess) return ctx->hip_fail(e, "d4_dev_alloc");
+    return D4_OK;
+}
+
+int d4_dev_free(d4_ctx* ctx, void* p)
+{
+    if (!ctx) return D4_ERR_INVALID;
+    hipError_t e = hipFree(p);
+    if (e != hipSuccess) return ctx->hip_fail(e, "d4_dev_free");
+    return D4_OK;
+}
+
+int d4_memcpy_h2d(d4_ctx* ctx, void* dst, const void* src, int64_t bytes)
+{
+    if (!ctx || bytes < 0) return D4_ERR_INVALID;
+    D4_TRY(ctx->h2d(dst, src, (size_t)bytes, "d4_memcpy_h2d"));
+    return ctx->sync("d4_memcpy_h2d");
+}
+
+int d4_memcpy_d2h(d4_ctx* ctx, void* dst, const void* src, int64_t bytes)
+{
+    if (!ctx || bytes < 0) return D4_ERR_INVALID;
+    D4_TRY(ctx->d2h(dst, src, (size_t)bytes, "d4_memcpy_d2h"));
+    return ctx->sync("d4_memcpy_d2h");
+}
+
+int d4_field_eval_dev(d4_ctx* ctx, int64_t n, const double* dx, const double* dy, const double* e2pix,
+                      const double* c2piy, double* sxy, double* dsxy, int flags)
+{
+    if (!ctx || n < 0 || (n > 0 && (!dx || !dy))) return D4_ERR_INVALID;
+    if ((e2pix == nullptr) != (c2piy == nullptr)) return ctx->fail(D4_ERR_INVALID, "give both e2pix and c2piy or neither");
+    hipError_t e = launch_field_eval(n, dx, dy, e2pix, c2piy, sxy, dsxy, flags & 1, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "field eval launch");
+    return D4_OK;
+}
+
+int d4_field_eval(d4_ctx* ctx, int64_t n, const double* dx, const double* dy, const double* e2pix,
+                  const double* c2piy, double* sxy, double* dsxy, int flags)
+{
+    if (!ctx || n < 0 || (n > 0 && (!dx || !dy))) return D4_ERR_INVALID;
+    if ((e2pix == nullptr) != (c2piy == nullptr)) return ctx->fail(D4_ERR_INVALID, "give both e2pix and c2piy or neither");
+    if (n == 0) return D4_OK;
+    const size_t b = sizeof(double) * (size_t)n;
+    D4_TRY(ctx->ensure(ctx->aux0, 6 * b, "field eval buffers"));
+    double* d = static_cast<double*>(ctx->aux0.p);
+    double *ddx = d, *ddy = d + n, *de = d + 2 * n, *dc = d + 3 * n, *ds = d + 4 * n, *dd = d + 5 * n;
+    D4_TRY(ctx->h2d(ddx, dx, b, "field eval upload"));
+    D4_TRY(ctx->h2d(ddy, dy, b, "field eval upload"));
+    if (e2pix) {
+        D4_TRY(ctx->h2d(de, e2pix, b, "field eval upload"));
+        D4_TRY(ctx->h2d(dc, c2piy, b, "field eval upload"));
+    }
+    D4_TRY(d4_field_eval_dev(ctx, n, ddx, ddy, e2pix ? de : nullptr, e2pix ? dc : nullptr, sxy ? ds : nullptr,
+                             dsxy ? dd : nullptr, flags));
+    if (sxy) D4_TRY(ctx->d2h(sxy, ds, b, "field eval download"));
+    if (dsxy) D4_TRY(ctx->d2h(dsxy, dd, b, "field eval download"));
+    return ctx->sync("field eval");
+}
+
+int d4_speed_pass_dev(d4_ctx* ctx, int n, const double* xy, double tau, double* v, double* tol, double tol_cap2)
+{
+    if (!ctx || n < 0 || (n > 0 && (!xy || !v))) return D4_ERR_INVALID;
+    if (n == 0) return D4_OK;
+    RealParams rp = single_params(n, nullptr, tau);
+    rp.tol_cap2 = tol ? tol_cap2 : 0.0;
+    return ctx->single_pass(kModeSpeed, n, xy, rp, tol, v, nullptr, nullptr, nullptr, nullptr, 0, nullptr, 0, 0, 1);
+}
+
+int d4_speed_pass(d4_ctx* ctx, int n, const double* xy, double tau, double* v, double* tol, double tol_cap2)
+{
+    if (!ctx || n < 0 || (n > 0 && (!xy || !v))) return D4_ERR_INVALID;
+    if (n == 0) return D4_OK;
+    const size_t b = sizeof(double) * (size_t)n;
+    D4_TRY(ctx->ensure(ctx->xy, 2 * b, "positions"));
+    D4_TRY(ctx->ensure(ctx->v0, b, "speeds"));
+    double* dtol = nullptr;
+    if (tol && tol_cap2 > 0) {
+        D4_TRY(ctx->ensure(ctx->tol, b, "tolerance"));
+        dtol = static_cast<double*>(ctx->tol.p);
+        D4_TRY(ctx->h2d(dtol, tol, b, "tolerance upload"));
+    }
+    D4_TRY(ctx->h2d(ctx->xy.p, xy, 2 * b, "position upload"));
+    D4_TRY(d4_speed_pass_dev(ctx, n, static_cast<double*>(ctx->xy.p), tau, static_cast<double*>(ctx->v0.p), dtol,
+                             tol_cap2));
+    D4_TRY(ctx->d2h(v, ctx->v0.p, b, "speed download"));
+    if (dtol) D4_TRY(ctx->d2h(tol, dtol, b, "tolerance download"));
+    return ctx->sync("speed pass");
+}
+
+int d4_jacobian_dense_dev(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, double* vA, double* vB,
+                          double* M, int64_t ld, double* dvec, double* tol)
+{
+    if (!ctx || !p || n < 0 || (n > 0 && (!xy || !vA || !M || ld < n))) return D4_ERR_INVALID;
+    if (n == 0) return D4_OK;
+    RealParams rp = single_params(n, p, 0.0);
+    if (!tol) rp.tol_cap2 = 0.0;
+    return ctx->single_pass(kModeJacDense, n, xy, rp, tol, vA, vB, nullptr, dvec, M, ld, nullptr, 0, 0, 1);
+}
+
+int d4_jacobian_dense_scale_dev(d4_ctx* ctx, int n, double* M, int64_t ld, const double* dvec)
+{
+    if (!ctx || n < 0 || (n > 0 && (!M || !dvec || ld < n))) return D4_ERR_INVALID;
+    hipError_t e = launch_dense_scale(M, ld, n, dvec, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "dense scale launch");
+    return D4_OK;
+}
+
+int d4_jacobian_dense(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, double* vA, double* vB,
+                      double* M, double* dvec, double* tol, int scale)
+{
+    if (!ctx || !p || n < 0 || (n > 0 && (!xy || !vA || !M))) return D4_ERR_INVALID;
+    if (n == 0) return D4_OK;
+    const size_t b = sizeof(double) * (size_t)n;
+    D4_TRY(ctx->ensure(ctx->xy, 2 * b, "positions"));
+    D4_TRY(ctx->ensure(ctx->v0, b, "speeds A"));
+    D4_TRY(ctx->ensure(ctx->v1, b, "speeds B"));
+    D4_TRY(ctx->ensure(ctx->dvec, b, "weights"));
+    D4_TRY(ctx->ensure(ctx->aux1, b * (size_t)n, "dense matrix"));
+    double* dtol = nullptr;
+    if (tol && p->tol_cap2 > 0) {
+        D4_TRY(ctx->ensure(ctx->tol, b, "tolerance"));
+        dtol = static_cast<double*>(ctx->tol.p);
+        D4_TRY(ctx->h2d(dtol, tol, b, "tolerance upload"));
+    }
+    D4_TRY(ctx->h2d(ctx->xy.p, xy, 2 * b, "position upload"));
+    double* dM = static_cast<double*>(ctx->aux1.p);
+    double* dd = static_cast<double*>(ctx->dvec.p);
+    D4_TRY(d4_jacobian_dense_dev(ctx, n, static_cast<double*>(ctx->xy.p), p, static_cast<double*>(ctx->v0.p),
+                                 static_cast<double*>(ctx->v1.p), dM, n, dd, dtol));
+    if (scale) D4_TRY(d4_jacobian_dense_scale_dev(ctx, n, dM, n, dd));
+    D4_TRY(ctx->d2h(vA, ctx->v0.p, b, "speed download"));
+    if (vB) D4_TRY(ctx->d2h(vB, ctx->v1.p, b, "speed download"));
+    D4_TRY(ctx->d2h(M, dM, b * (size_t)n, "matrix download"));
+    if (dvec) D4_TRY(ctx->d2h(dvec, dd, b, "weight download"));
+    if (dtol) D4_TRY(ctx->d2h(tol, dtol, b, "tolerance download"));
+    return ctx->sync("dense jacobian pass");
+}
+
+int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, double* vA, double* vB,
+                     int* Ap, int* Ai, double* Ax, int cap, int* nnz, int* indexes, double* dvec, double* tol)
+{
+    if (!ctx || !p || n < 0 || !nnz) return D4_ERR_INVALID;
+    if (n > 0 && (!xy || !vA || !vB || !Ap || !indexes || !dvec)) return D4_ERR_INVALID;
+    if (p->wfunc != 'c' && p->wfunc != 'p' && p->wfunc != 'm') return ctx->fail(D4_ERR_INVALID, "wfunc must be c, p or m");
+    *nnz = 0;
+    if (n == 0) return D4_OK;
+    const size_t b = sizeof(double) * (size_t)n;
+    D4_TRY(ctx->ensure(ctx->xy, 2 * b, "positions"));
+    D4_TRY(ctx->ensure(ctx->v0, b, "speeds A"));
+    D4_TRY(ctx->ensure(ctx->v1, b, "speeds B"));
+    D4_TRY(ctx->ensure(ctx->dvec, b, "weights"));
+    D4_TRY(ctx->ensure(ctx->idx, sizeof(int) * (size_t)n, "diagonal indexes"));
+    D4_TRY(ctx->ensure(ctx->cnt, sizeof(int) * (size_t)n, "column counts"));
+    D4_TRY(ctx->ensure(ctx->cursor, sizeof(int) * (size_t)n, "column cursors"));
+    D4_TRY(ctx->ensure(ctx->Ap, sizeof(int) * ((size_t)n + 1), "Ap"));
+    // COO capacity: start from the caller's CSC capacity, grow on overflow
+    size_t want = std::max<size_t>((size_t)std::max(cap, n) / 2 + 1024, ctx->coo.bytes / sizeof(CooEntry));
+    D4_TRY(ctx->ensure(ctx->coo, sizeof(CooEntry) * want, "pair list"));
+    double* dtol = nullptr;
+    if (tol && p->tol_cap2 > 0) {
+        D4_TRY(ctx->ensure(ctx->tol, b, "tolerance"));
+        dtol = static_cast<double*>(ctx->tol.p);
+        D4_TRY(ctx->h2d(dtol, tol, b, "tolerance upload"));
+    }
+    D4_TRY(ctx->h2d(ctx->xy.p, xy, 2 * b, "position upload"));
+    RealParams rp = single_params(n, p, 0.0);
+    if (!dtol) rp.tol_cap2 = 0.0;
+    int ncoo = 0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        hipError_t e = hipMemsetAsync(ctx->coo_count.p, 0, sizeof(int), ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "counter reset");
+        if (attempt == 1 && dtol) D4_TRY(ctx->h2d(dtol, tol, b, "tolerance upload"));
+        D4_TRY(ctx->single_pass(kModeJacCoo, n, static_cast<double*>(ctx->xy.p), rp, dtol,
+                                static_cast<double*>(ctx->v0.p), static_cast<double*>(ctx->v1.p), nullptr, nullptr,
+                                nullptr, 0, nullptr, 0, 0, 1));
+        D4_TRY(ctx->d2h(&ncoo, ctx->coo_count.p, sizeof(int), "pair count"));
+        D4_TRY(ctx->sync("jacobian pass"));
+        if ((size_t)ncoo <= ctx->coo.bytes / sizeof(CooEntry)) break;
+        if (attempt == 1) return ctx->fail(D4_ERR_HIP, "pair list overflow after resize");
+        D4_TRY(ctx->ensure(ctx->coo, sizeof(CooEntry) * ((size_t)ncoo + 1024), "pair list"));
+    }
+    // CSC assembly: counts (diagonal + both copies), scan, fill, sort/finish, scale
+    std::vector<int> ones(n, 1);
+    D4_TRY(ctx->h2d(ctx->cnt.p, ones.data(), sizeof(int) * (size_t)n, "column count init"));
+    int* dcnt = static_cast<int*>(ctx->cnt.p);
+    int* dAp = static_cast<int*>(ctx->Ap.p);
+    const CooEntry* dcoo = static_cast<const CooEntry*>(ctx->coo.p);
+    hipError_t e = launch_csc_count(dcoo, ncoo, dcnt, n, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "csc count");
+    e = launch_scan(dcnt, dAp, n, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "csc scan");
+    int total_nnz = 0;
+    D4_TRY(ctx->d2h(&total_nnz, dAp + n, sizeof(int), "nnz"));
+    D4_TRY(ctx->sync("csc scan"));
+    *nnz = total_nnz;
+    if (total_nnz > cap || !Ai || !Ax) {
+        ctx->err = "CSC capacity too small";
+        return D4_ERR_CAPACITY;
+    }
+    D4_TRY(ctx->ensure(ctx->Ai, sizeof(int) * (size_t)total_nnz, "Ai"));
+    D4_TRY(ctx->ensure(ctx->Ax, sizeof(double) * (size_t)total_nnz, "Ax"));
+    int* dAi = static_cast<int*>(ctx->Ai.p);
+    double* dAx = static_cast<double*>(ctx->Ax.p);
+    int* didx = static_cast<int*>(ctx->idx.p);
+    double* dd = static_cast<double*>(ctx->dvec.p);
+    e = launch_csc_fill(dcoo, ncoo, dAp, static_cast<int*>(ctx->cursor.p), dAi, dAx, n, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "csc fill");
+    e = launch_csc_finish(dAp, dAi, dAx, didx, dd, nullptr, nullptr, p->wfunc, n, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "csc finish");
+    e = launch_csc_scale(dAp, dAi, dAx, didx, dd, n, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "csc scale");
+    D4_TRY(ctx->d2h(vA, ctx->v0.p, b, "speed download"));
+    D4_TRY(ctx->d2h(vB, ctx->v1.p, b, "speed download"));
+    D4_TRY(ctx->d2h(Ap, dAp, sizeof(int) * ((size_t)n + 1), "Ap download"));
+    D4_TRY(ctx->d2h(Ai, dAi, sizeof(int) * (size_t)total_nnz, "Ai download"));
+    D4_TRY(ctx->d2h(Ax, dAx, sizeof(double) * (size_t)total_nnz, "Ax download"));
+    D4_TRY(ctx->d2h(indexes, didx, sizeof(int) * (size_t)n, "indexes download"));
+    D4_TRY(ctx->d2h(dvec, dd, b, "weights download"));
+    if (dtol) D4_TRY(ctx->d2h(tol, dtol, b, "tolerance download"));
+    return ctx->sync("jacobian pass");
+}
+
+int d4_jacobian_rescale(d4_ctx* ctx, int n, const int* Ap, const int* Ai, double* Ax, double* dvec, char wfunc,
+                        double factor, double dtau, double* speed)
+{
+    if (!ctx || n < 0 || (n > 0 && (!Ap || !Ai || !Ax || !dvec))) return D4_ERR_INVALID;
+    if (n == 0) return D4_OK;
+    const int nnz = Ap[n];
+    const size_t b = sizeof(double) * (size_t)n;
+    D4_TRY(ctx->ensure(ctx->Ap, sizeof(int) * ((size_t)n + 1), "Ap"));
+    D4_TRY(ctx->ensure(ctx->Ai, sizeof(int) * (size_t)std::max(nnz, 1), "Ai"));
+    D4_TRY(ctx->ensure(ctx->Ax, sizeof(double) * (size_t)std::max(nnz, 1), "Ax"));
+    D4_TRY(ctx->ensure(ctx->dvec, b, "weights"));
+    D4_TRY(ctx->ensure(ctx->nw, b, "new weights"));
+    D4_TRY(ctx->ensure(ctx->v0, b, "speeds"));
+    D4_TRY(ctx->h2d(ctx->Ap.p, Ap, sizeof(int) * ((size_t)n + 1), "Ap upload"));
+    D4_TRY(ctx->h2d(ctx->Ai.p, Ai, sizeof(int) * (size_t)nnz, "Ai upload"));
+    D4_TRY(ctx->h2d(ctx->Ax.p, Ax, sizeof(double) * (size_t)nnz, "Ax upload"));
+    D4_TRY(ctx->h2d(ctx->dvec.p, dvec, b, "weights upload"));
+    if (speed) D4_TRY(ctx->h2d(ctx->v0.p, speed, b, "speed upload"));
+    RescaleArgs a{};
+    a.Ap = static_cast<const int*>(ctx->Ap.p);
+    a.Ai = static_cast<const int*>(ctx->Ai.p);
+    a.Ax = static_cast<double*>(ctx->Ax.p);
+    a.dvec = static_cast<double*>(ctx->dvec.p);
+    a.nw = static_cast<double*>(ctx->nw.p);
+    a.speed = speed ? static_cast<double*>(ctx->v0.p) : nullptr;
+    a.factor0 = factor;
+    a.dtau0 = dtau;
+    a.wf0 = wfunc;
+    a.total = n;
+    hipError_t e = launch_rescale(a, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "rescale launch");
+    D4_TRY(ctx->d2h(Ax, ctx->Ax.p, sizeof(double) * (size_t)nnz, "Ax download"));
+    D4_TRY(ctx->d2h(dvec, ctx->dvec.p, b, "weights download"));
+    if (speed) D4_TRY(ctx->d2h(speed, ctx->v0.p, b, "speed download"));
+    return ctx->sync("rescale");
+}
+
+int d4_vjp_dev(d4_ctx* ctx, int n, const double* xy, const double* w, double* gx, double* gy, double* gtau, int flags)
+{
+    if (!ctx || n < 0 || (n > 0 && (!xy || !w || !gx))) return D4_ERR_INVALID;
+    if (n == 0) {
+        if (gtau) *gtau = 0;
+        return D4_OK;
+    }
+    RealParams rp = single_params(n, nullptr, 0.0);
+    D4_TRY(ctx->single_pass(kModeVjp, n, xy, rp, nullptr, gx, nullptr, gy, nullptr, nullptr, 0, w, flags & 1,
+                            gy != nullptr, 0));
+    if (gtau) {  // dL/dtau = sum_i b_i w_i: host-side sum over a downloaded w
+        std::vector<double> hw(n);
+        D4_TRY(ctx->d2h(hw.data(), w, sizeof(double) * (size_t)n, "w download"));
+        D4_TRY(ctx->sync("vjp"));
+        double s = 0;
+        for (int i = 0; i < n; ++i) s += (i < n / 2 ? 1.0 : -1.0) * hw[i];
+        *gtau = s;
+    }
+    return D4_OK;
+}
+
+}  // extern "C"

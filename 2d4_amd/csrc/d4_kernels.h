@@ -1,0 +1,136 @@
+// d4_kernels.h -- launch interface of the gfx950 pair kernels (internal to
+// lib2d4; the public boundary is include/d4.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace d4 {
+
+constexpr int kTile = 64;  // one wavefront per 64 x 64 block of unordered pairs
+
+// One tile of a (possibly ragged, batched) pass: realization r, block rows
+// [I0, I0+64) x block columns [J0, J0+64) of that realization, I0 <= J0.
+// slot = canonical index of the tile's partial sums.
+struct TileDesc {
+    int r;
+    int I0;
+    int J0;
+    int slot;
+};
+
+// Per-realization scalars of a pass (h = 0 for speed-only passes).
+struct RealParams {
+    double h;
+    double tauA, tauB;
+    double range_sq;    // Jacobian storage radius^2: rc^2 (heaviside) or 49 rc^2
+    double cut_sq;      // rc^2
+    double cutoff;      // rc
+    double one_per;     // 1 / rc^2
+    double tol_cap2;    // <= 0: no tolerance tracking
+    int heaviside;
+    int wfunc;          // 'c', 'p', 'm'
+    int slot0;          // first partial slot of this realization
+    int nb;             // blocks of this realization
+};
+
+struct CooEntry {
+    int i;  // lower index (column of the primary copy)
+    int j;  // higher index
+    double v;
+};
+
+enum PassMode : int {
+    kModeSpeed = 0,      // sigma only: speeds, tolerance
+    kModeJacDense = 1,   // sigma + d sigma: speeds, V dense column-major, row sums
+    kModeJacCoo = 2,     // sigma all pairs, d sigma inside the storage radius -> COO list
+    kModeVjp = 3,        // d sigma/dx (+ d sigma/dy): w-weighted pair sums
+};
+
+struct PassArgs {
+    const double2* xy;        // concatenated sorted positions
+    const int* off;           // R+1 realization offsets (null: one realization of n0)
+    int n0;
+    const TileDesc* tiles;
+    int ntiles;
+    const RealParams* rp;     // R (null: rp0)
+    RealParams rp0;
+    double* part;             // partial sums [slot][NQ][128]
+    double* tol;              // per element (may be null)
+    // dense Jacobian output (mode 1, single realization)
+    double* M;
+    int64_t ld;
+    // COO output (mode 2)
+    CooEntry* coo;
+    int* coo_count;
+    int coo_cap;
+    // VJP input (mode 3)
+    const double* w;
+    int exact;                // mode 1/2/3: 1 = true derivative instead of the reference's
+    int want_dy;              // mode 3: also d/dy
+};
+
+// number of accumulated quantities per mode
+inline int pass_nq(int mode) { return mode == kModeSpeed || mode == kModeJacCoo ? 1 : 2; }
+
+hipError_t launch_pair_pass(int mode, const PassArgs& a, hipStream_t s);
+
+// Field values for n independent pairs (E, c may be null -> computed on device).
+hipError_t launch_field_eval(int64_t n, const double* dx, const double* dy, const double* E, const double* c,
+                             double* sxy, double* dsxy, int exact, hipStream_t s);
+
+// Deterministic reduction of tile partials into per-element sums.
+// out0[e] = sum of quantity 0 (+ tau terms for speeds), out1 = quantity 1 (may be null).
+struct ReduceArgs {
+    const double* part;
+    int nq;
+    const int* off;          // null: one realization of `total` elements
+    const RealParams* rp;    // null: rp0
+    RealParams rp0;
+    int R;
+    int total;               // sum of n_r
+    const int* elem_real;    // element -> realization (null when R == 1)
+    double* vA;              // quantity 0 + tauA b
+    double* vB;              // quantity 0 + tauB b (may be null)
+    double* q1;              // quantity 1 raw (may be null)
+    double* dvec;            // weight(q1) (may be null)
+    double* M;               // dense diagonal <- q1 (may be null)
+    int64_t ld;
+    int add_tau;             // 1: add tau b to vA/vB
+};
+hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
+
+// CSC assembly from a COO list over a block-diagonal batch (global indices).
+hipError_t launch_csc_count(const CooEntry* coo, int ncoo, int* cnt, int total, hipStream_t s);
+hipError_t launch_scan(const int* cnt, int* Ap, int total, hipStream_t s);  // Ap[0..total]
+hipError_t launch_csc_fill(const CooEntry* coo, int ncoo, const int* Ap, int* cursor, int* Ai, double* Ax,
+                           int total, hipStream_t s);
+// wfunc per realization from wf_arr (null: wf0 for all)
+hipError_t launch_csc_finish(const int* Ap, int* Ai, double* Ax, int* indexes, double* dvec, const int* elem_real,
+                             const int* wf_arr, int wf0, int total, hipStream_t s);
+hipError_t launch_csc_scale(const int* Ap, const int* Ai, double* Ax, const int* indexes, const double* dvec,
+                            int total, hipStream_t s);
+// stage-II rescale (simulation.cpp:546-608)
+// per-realization factor / dtau / wfunc from the arrays, or the scalars when the arrays are null
+struct RescaleArgs {
+    const int* Ap;
+    const int* Ai;
+    double* Ax;
+    double* dvec;
+    double* nw;              // scratch, total
+    double* speed;           // may be null
+    const int* elem_real;    // null: one realization
+    const int* off;          // null: one realization of `total`
+    const double* factor_arr;
+    const double* dtau_arr;
+    const int* wf_arr;
+    double factor0, dtau0;
+    int wf0;
+    int total;
+};
+hipError_t launch_rescale(const RescaleArgs& a, hipStream_t s);
+// dense final scaling: M[j,i] *= (1+d_j)/2, diagonal += 1
+hipError_t launch_dense_scale(double* M, int64_t ld, int n, const double* dvec, hipStream_t s);
+
+}  // namespace d4

@@ -1,0 +1,487 @@
+// d4_kernels.hip -- gfx950 kernels of the periodic pair-interaction path.
+//
+// Pair passes (simulation.cpp:287-333 speed pass, simulation.cpp:345-527
+// Jacobian pass) are evaluated per unordered pair (i < j, sorted index order,
+// dx = x_i - x_j as simulation.cpp:446-450 orients them) in 64 x 64 tiles,
+// one wavefront per tile:
+//   * lane l owns target i = I0 + l; at step s it meets source j = J0 + (l ^ s),
+//     so within a step every lane touches a different source (no LDS conflicts);
+//   * target-side sums stay in registers, source-side sums go to LDS with
+//     ds_add_f64, and each tile writes both as partials to its own slot;
+//     launch_reduce adds the partials in a fixed order (deterministic, no
+//     floating-point atomics in global memory);
+//   * in the dense Jacobian mode each 16-step phase is staged in LDS and written
+//     twice, V_ij to (row j, column i) and its mirror to (row i, column j), both
+//     as 128-byte contiguous column segments.
+#include "d4_field.h"
+#include "d4_kernels.h"
+
+namespace d4 {
+
+// minimum of a non-negative double via its bit pattern (ordering of
+// non-negative doubles equals that of their unsigned 64-bit patterns)
+__device__ __forceinline__ void atomic_min_nonneg(double* p, double v)
+{
+    atomicMin(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(__double_as_longlong(v)));
+}
+
+template <int MODE, bool EXACT>
+__global__ __launch_bounds__(64) void pair_pass_kernel(PassArgs a)
+{
+    constexpr bool kDense = MODE == kModeJacDense;
+    constexpr bool kCoo = MODE == kModeJacCoo;
+    constexpr bool kVjp = MODE == kModeVjp;
+    constexpr int NQ = (MODE == kModeSpeed || MODE == kModeJacCoo) ? 1 : 2;
+
+    __shared__ double sx[kTile], sy[kTile], sw[kVjp ? kTile : 1];
+    __shared__ double jq0[kTile], jq1[NQ == 2 ? kTile : 1];
+    __shared__ double stage[kDense ? 4 * 16 * 17 : 1];
+
+    const int lane = threadIdx.x;
+    const TileDesc td = a.tiles[blockIdx.x];
+    const int base = a.off ? a.off[td.r] : 0;
+    const int n = a.off ? a.off[td.r + 1] - base : a.n0;
+    const RealParams rp = a.rp ? a.rp[td.r] : a.rp0;
+    const bool diag = td.I0 == td.J0;
+    const int half = n >> 1;
+
+    const int i = td.I0 + lane;
+    const bool vi = i < n;
+    const double2 pi = vi ? a.xy[base + i] : make_double2(0.0, 0.0);
+    const double bi = i < half ? 1.0 : -1.0;
+    {
+        const int jj = td.J0 + lane;
+        const double2 pj = jj < n ? a.xy[base + jj] : make_double2(0.0, 0.0);
+        sx[lane] = pj.x;
+        sy[lane] = pj.y;
+        if (kVjp) sw[lane] = jj < n ? a.w[base + jj] : 0.0;
+        jq0[lane] = 0.0;
+        if (NQ == 2) jq1[lane] = 0.0;
+    }
+    const double wi = (kVjp && vi) ? a.w[base + i] : 0.0;
+    __syncthreads();
+
+    double q0 = 0.0, q1 = 0.0;
+    const bool track_tol = a.tol != nullptr && rp.tol_cap2 > 0.0;
+
+    for (int ph = 0; ph < 4; ++ph) {
+        for (int tt = 0; tt < 16; ++tt) {
+            const int jl = lane ^ (ph * 16 + tt);
+            const int j = td.J0 + jl;
+            const bool act = vi && j < n && (!diag || lane < jl);
+            double V = 0.0;
+            bool inr = false;  // COO mode: pair inside the storage radius
+            if (act) {
+                const double dx = wrap(pi.x - sx[jl]);
+                const double dy = wrap(pi.y - sy[jl]);
+                const double d2 = dist2(dx, dy);
+                if (track_tol && d2 < rp.tol_cap2) {  // precision_handler.cpp:34-38
+                    atomic_min_nonneg(a.tol + base + i, d2);
+                    atomic_min_nonneg(a.tol + base + j, d2);
+                }
+                const double bj = j < half ? 1.0 : -1.0;
+                double E, c;
+                field_inputs(dx, dy, E, c);
+                double sig = 0.0, dsig = 0.0;
+                if (MODE == kModeSpeed) {
+                    pair_field<false, false>(dx, dy, E, c, sig, dsig);
+                    const double f = sig * bi * bj;
+                    q0 += f;
+                    atomicAdd(&jq0[jl], -f);
+                } else if (kDense) {
+                    pair_field<true, EXACT>(dx, dy, E, c, sig, dsig);
+                    const double f = sig * bi * bj;
+                    V = rp.h * bj * bi * dsig;
+                    q0 += f;
+                    atomicAdd(&jq0[jl], -f);
+                    q1 -= V;
+                    atomicAdd(&jq1[jl], -V);
+                } else if (kCoo) {
+                    inr = d2 < rp.range_sq;
+                    if (inr) {
+                        pair_field<true, EXACT>(dx, dy, E, c, sig, dsig);
+                        V = rp.h * bj * bi * dsig;
+                        if (!rp.heaviside && d2 > rp.cut_sq) {  // simulation.cpp:469-470
+                            const double e = sqrt(d2) - rp.cutoff;
+                            V *= exp(-(e * e) * rp.one_per);
+                        }
+                    } else {
+                        pair_field<false, false>(dx, dy, E, c, sig, dsig);
+                    }
+                    const double f = sig * bi * bj;
+                    q0 += f;
+                    atomicAdd(&jq0[jl], -f);
+                } else {  // kVjp
+                    pair_field<true, EXACT>(dx, dy, E, c, sig, dsig);
+                    const double bb = bi * bj;
+                    const double coef = (wi - sw[jl]) * bb;
+                    const double gx = coef * dsig;
+                    q0 += gx;
+                    atomicAdd(&jq0[jl], -gx);
+                    if (a.want_dy) {
+                        const double gy = coef * pair_field_dy(dx, E, c, sin(kTwoPi * dy));
+                        q1 += gy;
+                        atomicAdd(&jq1[jl], -gy);
+                    }
+                }
+            }
+            if (kCoo) {
+                // wave-aggregated append: one atomic per wave, lane offsets from the ballot prefix
+                const unsigned long long mask = __ballot(inr);
+                if (mask) {
+                    const int cnt = __popcll(mask);
+                    const int leader = __ffsll((long long)mask) - 1;
+                    int b0 = 0;
+                    if (lane == leader) b0 = atomicAdd(a.coo_count, cnt);
+                    b0 = __shfl(b0, leader);
+                    const int pos = b0 + __popcll(mask & ((1ull << lane) - 1ull));
+                    if (inr && pos < a.coo_cap) {
+                        CooEntry e;
+                        e.i = base + i;
+                        e.j = base + j;
+                        e.v = V;
+                        a.coo[pos] = e;
+                    }
+                }
+            }
+            if (kDense) stage[((lane >> 4) * 16 + (lane & 15)) * 17 + ((lane & 15) ^ tt)] = V;
+        }
+        if (kDense) {
+            __syncthreads();
+            const int g = lane >> 4, q = lane & 15;
+            const int ib = td.I0 + 16 * g;
+            const int jb = td.J0 + 16 * (g ^ ph);
+            for (int k = 0; k < 16; ++k) {
+                const int i1 = ib + k, j1 = jb + q;  // V(i1, j1) -> column i1, row j1
+                if (i1 < n && j1 < n && (!diag || i1 < j1)) a.M[(int64_t)i1 * a.ld + j1] = stage[(g * 16 + k) * 17 + q];
+                const int i2 = ib + q, j2 = jb + k;  // V(i2, j2) -> column j2, row i2
+                if (i2 < n && j2 < n && (!diag || i2 < j2)) a.M[(int64_t)j2 * a.ld + i2] = stage[(g * 16 + q) * 17 + k];
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    double* P = a.part + (size_t)td.slot * NQ * 128;
+    P[lane] = q0;
+    P[64 + lane] = jq0[lane];
+    if (NQ == 2) {
+        P[128 + lane] = q1;
+        P[192 + lane] = jq1[lane];
+    }
+}
+
+template <int MODE>
+static hipError_t launch_mode(const PassArgs& a, hipStream_t s)
+{
+    if (a.ntiles <= 0) return hipSuccess;
+    if (a.exact)
+        hipLaunchKernelGGL((pair_pass_kernel<MODE, true>), dim3(a.ntiles), dim3(kTile), 0, s, a);
+    else
+        hipLaunchKernelGGL((pair_pass_kernel<MODE, false>), dim3(a.ntiles), dim3(kTile), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_pass(int mode, const PassArgs& a, hipStream_t s)
+{
+    switch (mode) {
+    case kModeSpeed: return launch_mode<kModeSpeed>(a, s);
+    case kModeJacDense: return launch_mode<kModeJacDense>(a, s);
+    case kModeJacCoo: return launch_mode<kModeJacCoo>(a, s);
+    case kModeVjp: return launch_mode<kModeVjp>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ------------------------------------------------------------ field eval
+
+__global__ void field_eval_kernel(int64_t n, const double* dx, const double* dy, const double* E, const double* c,
+                                  double* sxy, double* dsxy, int exact)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double x = dx[k], y = dy[k];
+    double e, cc;
+    if (E && c) { e = E[k]; cc = c[k]; }
+    else field_inputs(x, y, e, cc);
+    double s = 0.0, ds = 0.0;
+    if (dsxy) {
+        if (exact) pair_field<true, true>(x, y, e, cc, s, ds);
+        else pair_field<true, false>(x, y, e, cc, s, ds);
+        dsxy[k] = ds;
+    } else {
+        pair_field<false, false>(x, y, e, cc, s, ds);
+    }
+    if (sxy) sxy[k] = s;
+}
+
+hipError_t launch_field_eval(int64_t n, const double* dx, const double* dy, const double* E, const double* c,
+                             double* sxy, double* dsxy, int exact, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const int bs = 256;
+    hipLaunchKernelGGL(field_eval_kernel, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0, s, n, dx, dy, E, c,
+                       sxy, dsxy, exact);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------- reduction
+
+__global__ void reduce_kernel(ReduceArgs a)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.total) return;
+    const int r = a.elem_real ? a.elem_real[g] : 0;
+    const int base = a.off ? a.off[r] : 0;
+    const int n = a.off ? a.off[r + 1] - base : a.total;
+    const int k = g - base;
+    const int b = k / kTile, l = k % kTile;
+    const RealParams rp = a.rp ? a.rp[r] : a.rp0;
+    const int nb = rp.nb;
+    double sum[2] = {0.0, 0.0};
+    for (int q = 0; q < a.nq; ++q) {
+        double acc = 0.0;
+        // target side: tiles (b, J), J >= b
+        const int row0 = rp.slot0 + b * nb - b * (b - 1) / 2;
+        for (int J = b; J < nb; ++J) acc += a.part[((size_t)(row0 + J - b) * a.nq + q) * 128 + l];
+        // source side: tiles (I, b), I <= b
+        for (int I = 0; I <= b; ++I) {
+            const int slot = rp.slot0 + I * nb - I * (I - 1) / 2 + (b - I);
+            acc += a.part[((size_t)slot * a.nq + q) * 128 + 64 + l];
+        }
+        sum[q] = acc;
+    }
+    const double bsg = k < (n >> 1) ? 1.0 : -1.0;
+    if (a.vA) a.vA[g] = a.add_tau ? sum[0] + rp.tauA * bsg : sum[0];
+    if (a.vB) a.vB[g] = a.add_tau ? sum[0] + rp.tauB * bsg : sum[0];
+    if (a.q1) a.q1[g] = sum[1];
+    if (a.dvec) a.dvec[g] = weight(sum[1], (char)rp.wfunc);
+    if (a.M) a.M[(int64_t)k * a.ld + k] = sum[1];
+}
+
+hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s)
+{
+    if (a.total <= 0) return hipSuccess;
+    const int bs = 256;
+    hipLaunchKernelGGL(reduce_kernel, dim3((a.total + bs - 1) / bs), dim3(bs), 0, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ CSC
+
+__global__ void csc_count_kernel(const CooEntry* coo, int ncoo, int* cnt)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncoo) return;
+    const CooEntry e = coo[k];
+    atomicAdd(&cnt[e.i], 1);
+    if (e.v != 0.0) atomicAdd(&cnt[e.j], 1);  // simulation.cpp:365: exact zeros are not mirrored
+}
+
+hipError_t launch_csc_count(const CooEntry* coo, int ncoo, int* cnt, int total, hipStream_t s)
+{
+    (void)total;
+    if (ncoo <= 0) return hipSuccess;
+    hipLaunchKernelGGL(csc_count_kernel, dim3((ncoo + 255) / 256), dim3(256), 0, s, coo, ncoo, cnt);
+    return hipGetLastError();
+}
+
+// single-workgroup exclusive scan: Ap[0] = 0, Ap[k+1] = Ap[k] + cnt[k]
+__global__ __launch_bounds__(1024) void scan_kernel(const int* cnt, int* Ap, int total)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int chunk = (total + 1023) / 1024;
+    const int lo = min(total, t * chunk), hi = min(total, lo + chunk);
+    int s = 0;
+    for (int k = lo; k < hi; ++k) s += cnt[k];
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        int v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;  // exclusive
+    if (t == 0) Ap[0] = 0;
+    for (int k = lo; k < hi; ++k) {
+        run += cnt[k];
+        Ap[k + 1] = run;
+    }
+}
+
+hipError_t launch_scan(const int* cnt, int* Ap, int total, hipStream_t s)
+{
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, Ap, total);
+    return hipGetLastError();
+}
+
+__global__ void csc_diag_kernel(const int* Ap, int* cursor, int* Ai, double* Ax, int total)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const int p = Ap[c];
+    Ai[p] = c;
+    Ax[p] = 0.0;
+    cursor[c] = p + 1;
+}
+
+__global__ void csc_fill_kernel(const CooEntry* coo, int ncoo, int* cursor, int* Ai, double* Ax)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ncoo) return;
+    const CooEntry e = coo[k];
+    int p = atomicAdd(&cursor[e.i], 1);
+    Ai[p] = e.j;
+    Ax[p] = e.v;
+    if (e.v != 0.0) {
+        p = atomicAdd(&cursor[e.j], 1);
+        Ai[p] = e.i;
+        Ax[p] = e.v;
+    }
+}
+
+hipError_t launch_csc_fill(const CooEntry* coo, int ncoo, const int* Ap, int* cursor, int* Ai, double* Ax,
+                           int total, hipStream_t s)
+{
+    hipLaunchKernelGGL(csc_diag_kernel, dim3((total + 255) / 256), dim3(256), 0, s, Ap, cursor, Ai, Ax, total);
+    if (ncoo > 0)
+        hipLaunchKernelGGL(csc_fill_kernel, dim3((ncoo + 255) / 256), dim3(256), 0, s, coo, ncoo, cursor, Ai, Ax);
+    return hipGetLastError();
+}
+
+// Per column: sort rows ascending (the order getElement + the j > i loop
+// produce, simulation.cpp:362-374, 444-478), then the diagonal
+// s = -sum of the column in that order (simulation.cpp:370, 475, 512) and
+// d = weight(s) (simulation.cpp:513).
+__global__ void csc_finish_kernel(const int* Ap, int* Ai, double* Ax, int* indexes, double* dvec,
+                                  const int* elem_real, const int* wf_arr, int wf0, int total)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    const int lo = Ap[c], hi = Ap[c + 1], len = hi - lo;
+    // shell sort with a fixed gap sequence ending in an insertion pass
+    const int gaps[8] = {701, 301, 132, 57, 23, 10, 4, 1};
+    for (int gi = 0; gi < 8; ++gi) {
+        const int gap = gaps[gi];
+        if (gap >= len) continue;
+        for (int k = lo + gap; k < hi; ++k) {
+            const int key = Ai[k];
+            const double val = Ax[k];
+            int m = k;
+            while (m - gap >= lo && Ai[m - gap] > key) {
+                Ai[m] = Ai[m - gap];
+                Ax[m] = Ax[m - gap];
+                m -= gap;
+            }
+            Ai[m] = key;
+            Ax[m] = val;
+        }
+    }
+    double sub = 0.0;
+    int dpos = lo;
+    for (int k = lo; k < hi; ++k) {
+        if (Ai[k] == c) dpos = k;
+        else sub -= Ax[k];
+    }
+    Ax[dpos] = sub;
+    indexes[c] = dpos;
+    const int wf = wf_arr ? wf_arr[elem_real ? elem_real[c] : 0] : wf0;
+    dvec[c] = weight(sub, (char)wf);
+}
+
+hipError_t launch_csc_finish(const int* Ap, int* Ai, double* Ax, int* indexes, double* dvec, const int* elem_real,
+                             const int* wf_arr, int wf0, int total, hipStream_t s)
+{
+    hipLaunchKernelGGL(csc_finish_kernel, dim3((total + 255) / 256), dim3(256), 0, s, Ap, Ai, Ax, indexes, dvec,
+                       elem_real, wf_arr, wf0, total);
+    return hipGetLastError();
+}
+
+// simulation.cpp:516-522
+__global__ void csc_scale_kernel(const int* Ap, const int* Ai, double* Ax, const int* indexes, const double* dvec,
+                                 int total)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= total) return;
+    for (int k = Ap[c]; k < Ap[c + 1]; ++k) Ax[k] *= (1 + dvec[Ai[k]]) / 2;
+    Ax[indexes[c]] += 1;
+}
+
+hipError_t launch_csc_scale(const int* Ap, const int* Ai, double* Ax, const int* indexes, const double* dvec,
+                            int total, hipStream_t s)
+{
+    hipLaunchKernelGGL(csc_scale_kernel, dim3((total + 255) / 256), dim3(256), 0, s, Ap, Ai, Ax, indexes, dvec,
+                       total);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ stage-II rescale
+
+__global__ void rescale_weights_kernel(RescaleArgs a)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.total) return;
+    const int r = a.elem_real ? a.elem_real[g] : 0;
+    const double ow = a.dvec[g];
+    const char T = (char)(a.wf_arr ? a.wf_arr[r] : a.wf0);
+    const double factor = a.factor_arr ? a.factor_arr[r] : a.factor0;
+    a.nw[g] = ow == 0 ? 0.0 : weight(weight_inv(ow, T) * factor, T);  // simulation.cpp:559-572
+}
+
+__global__ void rescale_entries_kernel(RescaleArgs a)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.total) return;
+    for (int k = a.Ap[c]; k < a.Ap[c + 1]; ++k) {  // simulation.cpp:575-594
+        const int row = a.Ai[k];
+        const double f = (1 + a.nw[row]) / (1 + a.dvec[row]);
+        if (row == c) a.Ax[k] = 1 + (a.Ax[k] - 1) / 2 * f;
+        else a.Ax[k] = a.Ax[k] / 2 * f;
+    }
+}
+
+__global__ void rescale_commit_kernel(RescaleArgs a)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= a.total) return;
+    const int r = a.elem_real ? a.elem_real[g] : 0;
+    const int base = a.off ? a.off[r] : 0;
+    const int n = a.off ? a.off[r + 1] - base : a.total;
+    const int k = g - base;
+    a.dvec[g] = a.nw[g];
+    const double dtau = a.dtau_arr ? a.dtau_arr[r] : a.dtau0;
+    if (a.speed) a.speed[g] += dtau * (k < (n >> 1) ? 1.0 : -1.0);  // simulation.cpp:602-607
+}
+
+hipError_t launch_rescale(const RescaleArgs& a, hipStream_t s)
+{
+    if (a.total <= 0) return hipSuccess;
+    const dim3 g((a.total + 255) / 256), b(256);
+    hipLaunchKernelGGL(rescale_weights_kernel, g, b, 0, s, a);
+    hipLaunchKernelGGL(rescale_entries_kernel, g, b, 0, s, a);
+    hipLaunchKernelGGL(rescale_commit_kernel, g, b, 0, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ dense final scale
+
+__global__ void dense_scale_kernel(double* M, int64_t ld, int n, const double* dvec)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    const int col = blockIdx.y;
+    if (row >= n) return;
+    double* p = M + (int64_t)col * ld + row;
+    double v = *p * ((1 + dvec[row]) / 2);
+    if (row == col) v += 1;
+    *p = v;
+}
+
+hipError_t launch_dense_scale(double* M, int64_t ld, int n, const double* dvec, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(dense_scale_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, M, ld, n, dvec);
+    return hipGetLastError();
+}
+
+}  // namespace d4

@@ -1,0 +1,129 @@
+/*
+ * d4.h -- C ABI of lib2d4, the MI355X (gfx950) implementation of 2D4's
+ * periodic pair-interaction path.
+ *
+ * Plain C types only (pointers, sizes, scalars); no HIP or torch types cross
+ * this boundary.  Every call returns 0 on success or a negative D4_ERR_* code;
+ * d4_last_error() gives the message.  Calls are synchronous with respect to
+ * the host unless the name ends in _dev, in which case all array arguments are
+ * device pointers and the work is ordered on the context's stream
+ * (d4_ctx_set_stream); the call returns once the work is enqueued.
+ *
+ * Positions follow the reference's DislwoB layout (include/dislocation.h:23-34):
+ * n records of {double x, double y}, sorted as SimulationData sorts them
+ * (simulation_data.cpp:116), the Burgers vector implied by the index: +1 for
+ * the first n/2, -1 for the rest (simulation_data.cpp:265-270).
+ *
+ * Tolerance arrays implement PrecisionHandler::updateTolerance
+ * (precision_handler.cpp:34-38): for every pair, tol[i] = min(tol[i], d^2) and
+ * tol[j] likewise when d^2 < tol_cap2; tol_cap2 <= 0 (or tol == NULL) skips it.
+ */
+#ifndef D4_H
+#define D4_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define D4_OK 0
+#define D4_ERR_INVALID (-1)
+#define D4_ERR_HIP (-2)
+#define D4_ERR_CAPACITY (-3)
+#define D4_ERR_NO_DEVICE (-4)
+
+typedef struct d4_ctx d4_ctx;
+
+/* Context: device, stream and the device buffers it owns (no reference
+ * counterpart: the reference is single-threaded host code; one context per
+ * host thread). */
+int d4_ctx_create(int device, d4_ctx** out);
+void d4_ctx_destroy(d4_ctx* ctx);
+const char* d4_last_error(const d4_ctx* ctx);
+/* hipStream_t passed as void*; NULL restores the context's own stream */
+int d4_ctx_set_stream(d4_ctx* ctx, void* stream);
+int d4_ctx_synchronize(d4_ctx* ctx);
+/* device memory helpers for callers without a HIP toolchain (ctypes, tests) */
+int d4_dev_alloc(d4_ctx* ctx, int64_t bytes, void** out);
+int d4_dev_free(d4_ctx* ctx, void* p);
+int d4_memcpy_h2d(d4_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int d4_memcpy_d2h(d4_ctx* ctx, void* dst, const void* src, int64_t bytes);
+
+/* Per-pair field, replaces sdddstCore::Field::xy / xy_diff_x
+ * (include/Fields/Field.h:40-49; src/Fields/Field.cpp:295-346).
+ * For each k: sxy[k] = xy(dx[k], dy[k]) and dsxy[k] = xy_diff_x(dx[k], dy[k]).
+ * When e2pix and c2piy are given they are used as the 4-arg overloads' inputs;
+ * when NULL the device computes exp2(PI2LN2INV*dx) and cos(2*pi*dy)
+ * (Field.cpp:335-336).  sxy or dsxy may be NULL.  flags bit 0: true derivative
+ * instead of the reference's f_dx_h substitution (Field.cpp:269-283). */
+int d4_field_eval(d4_ctx* ctx, int64_t n, const double* dx, const double* dy, const double* e2pix,
+                  const double* c2piy, double* sxy, double* dsxy, int flags);
+int d4_field_eval_dev(d4_ctx* ctx, int64_t n, const double* dx, const double* dy, const double* e2pix,
+                      const double* c2piy, double* sxy, double* dsxy, int flags);
+
+/* Speed pass, replaces Simulation::calculateSpeedsAtTime
+ * (include/simulation.h:138; src/simulation.cpp:287-333):
+ * v[i] = sum_{j != i} b_i b_j sigma(x_lo - x_hi, y_lo - y_hi) (+ for the lower
+ * index, - for the higher) + tau b_i. */
+int d4_speed_pass(d4_ctx* ctx, int n, const double* xy, double tau, double* v, double* tol, double tol_cap2);
+int d4_speed_pass_dev(d4_ctx* ctx, int n, const double* xy, double tau, double* v, double* tol, double tol_cap2);
+
+/* Parameters of a Jacobian pass (simulation.cpp:345-527 reads them from
+ * SimulationData, simulation_data.h:181-197, and the stress protocol). */
+typedef struct {
+    double h;              /* stepsize */
+    double tau_a, tau_b;   /* extStress(simTime_A), extStress(simTime_B) */
+    double cutoff;         /* cutOff */
+    double cut_sq;         /* cutOffSqr */
+    double one_per_cut_sq; /* onePerCutOffSqr */
+    int heaviside;         /* heavisideCutoff */
+    char wfunc;            /* weightFunc: 'c', 'p' or 'm' */
+    double tol_cap2;       /* PrecisionHandler minPrecisitySq if the dipole criterion is on, else 0 */
+} d4_jac_params;
+
+/* Jacobian pass, replaces Simulation::calcJacobianAndSpeedsAtTimes
+ * (include/simulation.h:150; src/simulation.cpp:345-527) up to the UMFPACK
+ * call: speeds vA (at tau_a) and vB (at tau_b), the CSC matrix
+ * A = I + diag((1+d)/2) (V + diag(s)) with ascending rows per column and the
+ * diagonal included (Ap n+1, Ai/Ax nnz), indexes[i] = position of A_ii in Ax,
+ * dvec = weight(s).  Pairs are stored iff d^2 < (heaviside ? 1 : 49) cut_sq,
+ * tapered by exp(-(d - rc)^2/rc^2) beyond rc when not heaviside.
+ * If cap < nnz nothing is written to Ai/Ax, *nnz gets the required size and
+ * D4_ERR_CAPACITY is returned. */
+int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, double* vA, double* vB,
+                     int* Ap, int* Ai, double* Ax, int cap, int* nnz, int* indexes, double* dvec, double* tol);
+
+/* Stage-II rescale, replaces Simulation::calcJacobianAndSpeedsFromPrev
+ * (include/simulation.h:166; src/simulation.cpp:546-608) on a CSC matrix from
+ * d4_jacobian_pass: factor = (new_t - base_t)/(old_t - base_t),
+ * dtau = extStress(new_t) - extStress(old_t); Ax, dvec and speed in place. */
+int d4_jacobian_rescale(d4_ctx* ctx, int n, const int* Ap, const int* Ai, double* Ax, double* dvec, char wfunc,
+                        double factor, double dtau, double* speed);
+
+/* Dense Jacobian pass (the reference's pass with an unlimited cutoff,
+ * simulation.cpp:345-515 with cutOffSqr = infinity), device pointers:
+ * M (column-major, leading dimension ld >= n) receives V + diag(s), i.e. the
+ * reference's Ax before its (1+d)/2 row scaling and +1 (simulation.cpp:516-522);
+ * M = -h dv/dx.  vA, vB, dvec, tol as d4_jacobian_pass. */
+int d4_jacobian_dense_dev(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, double* vA, double* vB,
+                          double* M, int64_t ld, double* dvec, double* tol);
+/* simulation.cpp:516-522 on a dense M from d4_jacobian_dense_dev:
+ * M[j,i] *= (1 + d_j)/2, M[i,i] += 1 -> the reference's final A. */
+int d4_jacobian_dense_scale_dev(d4_ctx* ctx, int n, double* M, int64_t ld, const double* dvec);
+/* host-pointer convenience form of the two calls above (scale != 0 applies the scaling) */
+int d4_jacobian_dense(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, double* vA, double* vB,
+                      double* M, double* dvec, double* tol, int scale);
+
+/* Vector-Jacobian product of the pair-speed map (no reference counterpart; the
+ * adjoint of calculateSpeedsAtTime, simulation.cpp:287-333), device pointers:
+ * for L = sum_i w_i v_i:  gx = dL/dx, gy = dL/dy (gy may be NULL), *gtau = dL/dtau = sum b_i w_i.
+ * flags bit 0: true derivative (else the reference's d sigma of Field.cpp:314-331,
+ * whose transpose this then is). */
+int d4_vjp_dev(d4_ctx* ctx, int n, const double* xy, const double* w, double* gx, double* gy, double* gtau,
+               int flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

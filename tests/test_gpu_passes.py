@@ -1,0 +1,257 @@
+"""GPU parity of lib2d4 (through the C ABI) against the oracle.
+
+Bounds (helpers.py): (A) per pair 1e-12 * max(|ref|, S) on shared inputs;
+(B) with the device's own exp2/cos, (A) + 4x the reference's one-ulp input
+sensitivity.  Pass-level results are bounded by the sum of their pairs'
+bounds plus 1e-13 of the absolute sum (different, fixed summation order).
+Integer results (CSC structure, counts, tolerance minima) are bit-exact.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import TOL, all_pairs, bsign, pair_allowance, random_config, scale_dsigma, scale_sigma, wrap
+
+pytestmark = pytest.mark.gpu
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _dconf64():
+    return np.load(os.path.join(GOLDEN_DIR, "dconf_golden.npz"))["xy"]
+
+
+def _row_sum(n, i, j, vi, vj):
+    out = np.zeros(n)
+    np.add.at(out, i, vi)
+    np.add.at(out, j, vj)
+    return out
+
+
+def speed_allowance(orc, xy):
+    """per-row bound for a speed pass (sum of per-pair bounds (B) + reordering)."""
+    n = xy.shape[0]
+    i, j, dx, dy = all_pairs(xy)
+    e, c = orc.inputs(dx, dy)
+    a, s = pair_allowance(orc, dx, dy, e, c)
+    return _row_sum(n, i, j, a, a) + 1e-13 * _row_sum(n, i, j, np.abs(s), np.abs(s))
+
+
+# ------------------------------------------------------------------ field
+
+def test_field_eval_same_inputs(ctx, orc, golden):
+    """(A): device field from the reference's own E, c (golden fixture inputs)."""
+    g = golden["field"]
+    ok = np.hypot(g["dx"], g["dy"]) > 1e-200
+    dx, dy, e, c = g["dx"][ok], g["dy"][ok], g["e"][ok], g["c"][ok]
+    print(f"[field A] n={dx.size}", flush=True)
+    for exact in (False, True):
+        s, d = ctx.field_eval(dx, dy, e, c, exact=exact)
+        s0, d0 = orc.field(dx, dy, e, c, exact=exact)
+        assert np.all(np.abs(s - s0) <= TOL * np.maximum(np.abs(s0), scale_sigma(dx, dy, e, c)))
+        assert np.all(np.abs(d - d0) <= TOL * np.maximum(np.abs(d0), scale_dsigma(dx, dy, e, c)))
+    # against the reference's compiled Field directly (golden values)
+    s, d = ctx.field_eval(dx, dy, e, c)
+    assert np.all(np.abs(s - g["xy4"][ok]) <= TOL * np.maximum(np.abs(s), scale_sigma(dx, dy, e, c)))
+    assert np.all(np.abs(d - g["dxy4"][ok]) <= TOL * np.maximum(np.abs(d), scale_dsigma(dx, dy, e, c)))
+
+
+def test_field_eval_device_inputs(ctx, orc):
+    """(B): the device evaluates exp2 and cos itself; report the pure-relative pass fraction."""
+    rng = np.random.default_rng(31)
+    dx = np.concatenate([rng.uniform(-0.5, 0.5, 200000), rng.uniform(-2e-3, 2e-3, 4000)])
+    dy = np.concatenate([rng.uniform(-0.5, 0.5, 200000), rng.uniform(-2e-3, 2e-3, 4000)])
+    print(f"[field B] n={dx.size}", flush=True)
+    s, d = ctx.field_eval(dx, dy)
+    e, c = orc.inputs(dx, dy)
+    a, s0 = pair_allowance(orc, dx, dy, e, c)
+    ad, d0 = pair_allowance(orc, dx, dy, e, c, deriv=True)
+    assert np.all(np.abs(s - s0) <= a)
+    assert np.all(np.abs(d - d0) <= ad)
+    rel = np.abs(s - s0) / np.abs(s0)
+    print(f"[field B] sigma pure-relative <= 1e-12 for {np.mean(rel <= 1e-12):.5f} of pairs, max {rel.max():.2e}")
+
+
+# ------------------------------------------------------------------ speed pass
+
+@pytest.mark.parametrize("n", [2, 3, 64, 100, 1000, 1024])
+def test_speed_pass_vs_oracle(ctx, orc, n):
+    xy = _dconf64() if n == 64 else random_config(n, 100 + n)
+    for tau in (0.0, 0.1):
+        print(f"[speed] n={n} tau={tau}", flush=True)
+        v = ctx.speed_pass(xy, tau)
+        v0 = orc.speed_pass(xy, tau)
+        assert np.all(np.abs(v - v0) <= speed_allowance(orc, xy) + 1e-15 * abs(tau))
+
+
+def test_speed_pass_tolerance_exact(ctx, orc):
+    """PrecisionHandler::updateTolerance minima are bit-exact (same d^2 rounding)."""
+    xy = random_config(1000, 7)
+    cap2 = (1e-5 / 0.05) ** 2 * 400
+    t_gpu = np.full(1000, cap2)
+    t_orc = t_gpu.copy()
+    ctx.speed_pass(xy, 0.0, t_gpu, cap2)
+    orc.speed_pass(xy, 0.0, t_orc, cap2)
+    assert (t_orc < cap2).sum() > 0
+    assert np.array_equal(t_gpu, t_orc)
+
+
+def test_speed_pass_deterministic(ctx):
+    xy = random_config(3000, 8)
+    a = ctx.speed_pass(xy, 0.05)
+    b = ctx.speed_pass(xy, 0.05)
+    assert np.array_equal(a, b)
+
+
+# ------------------------------------------------------------------ Jacobian (CSC)
+
+def _jac_case(name):
+    if name == "dense64":
+        xy = _dconf64()
+        return xy, dict(h=1e-3, ta=0.0, tb=0.0, mult=1e20, hv=False, wf="c")
+    if name == "heaviside1024":
+        xy = random_config(1024, 9)
+        return xy, dict(h=1e-4, ta=0.1, tb=0.1, mult=0.05 * 32, hv=True, wf="c")
+    if name == "taper300":
+        xy = random_config(300, 10)
+        return xy, dict(h=2e-4, ta=0.02, tb=0.05, mult=0.05 * np.sqrt(300), hv=False, wf="m")
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("case", ["dense64", "heaviside1024", "taper300"])
+def test_jacobian_pass_vs_oracle(pkg, ctx, orc, case):
+    xy, k = _jac_case(case)
+    n = xy.shape[0]
+    print(f"[jacobian] {case} n={n}", flush=True)
+    cutoff, cut_sq, one_per = orc.cutoff(n, k["mult"], k["hv"])
+    p = pkg.jac_params(h=k["h"], tau_a=k["ta"], tau_b=k["tb"], cutoff=cutoff, heaviside=k["hv"], wfunc=k["wf"])
+    assert p.cut_sq == cut_sq
+    G = ctx.jacobian_pass(xy, p)
+    R = orc.jacobian_pass(xy, k["h"], k["ta"], k["tb"], cutoff, cut_sq, one_per, k["hv"], k["wf"])
+    # structure: bit-exact
+    assert G["nnz"] == R["nnz"]
+    assert np.array_equal(G["Ap"], R["Ap"]) and np.array_equal(G["Ai"], R["Ai"])
+    assert np.array_equal(G["indexes"], R["indexes"])
+    # values: per-entry bound from the pair bounds on d sigma
+    i, j, dx, dy = all_pairs(xy)
+    e, c = orc.inputs(dx, dy)
+    ad, _ = pair_allowance(orc, dx, dy, e, c, deriv=True)
+    aV = np.zeros((n, n))
+    aV[i, j] = aV[j, i] = k["h"] * ad
+    col_allow = aV.sum(axis=0)
+    d = R["dvec"]
+    rows = R["Ai"]
+    cols = np.repeat(np.arange(n), np.diff(R["Ap"]))
+    Vabs = np.abs(R["Ax"])
+    bound = (1 + d[rows]) / 2 * aV[rows, cols] + Vabs * 0.5 * col_allow[rows] + 1e-13 * Vabs
+    diag = rows == cols
+    bound[diag] = col_allow[cols[diag]] + np.abs(R["Ax"][diag]) * (1e-13 + 0.5 * col_allow[cols[diag]])
+    assert np.all(np.abs(G["Ax"] - R["Ax"]) <= bound)
+    assert np.all(np.abs(G["dvec"] - d) <= 0.5 * col_allow + 1e-13)
+    allow = speed_allowance(orc, xy)
+    assert np.all(np.abs(G["vA"] - R["vA"]) <= allow + 1e-15)
+    assert np.all(np.abs(G["vB"] - R["vB"]) <= allow + 1e-15)
+
+
+def test_jacobian_pass_capacity_and_tolerance(pkg, ctx, orc):
+    xy, k = _jac_case("heaviside1024")
+    n = xy.shape[0]
+    cutoff, cut_sq, one_per = orc.cutoff(n, k["mult"], True)
+    cap2 = 4e-6
+    p = pkg.jac_params(h=1e-4, cutoff=cutoff, heaviside=True, tol_cap2=cap2)
+    tg = np.full(n, cap2)
+    G = ctx.jacobian_pass(xy, p, tol=tg, cap=16)  # too small: the wrapper retries with the reported size
+    to = np.full(n, cap2)
+    R = orc.jacobian_pass(xy, 1e-4, 0, 0, cutoff, cut_sq, one_per, True, "c", tol=to, cap2=cap2)
+    assert G["nnz"] == R["nnz"]
+    assert np.array_equal(tg, to)
+
+
+def test_jacobian_rescale_vs_oracle(pkg, ctx, orc):
+    xy, k = _jac_case("taper300")
+    n = xy.shape[0]
+    cutoff, cut_sq, one_per = orc.cutoff(n, k["mult"], False)
+    R = orc.jacobian_pass(xy, k["h"], k["ta"], k["tb"], cutoff, cut_sq, one_per, False, "m")
+    Ax1, d1, sp1 = R["Ax"].copy(), R["dvec"].copy(), R["vB"].copy()
+    Ax2, d2, sp2 = R["Ax"].copy(), R["dvec"].copy(), R["vB"].copy()
+    orc.jacobian_from_prev(R["Ap"], R["Ai"], Ax1, d1, "m", 0.0, 0.5e-3, 1e-3, 0.05, 0.035, sp1)
+    ctx.jacobian_rescale(R["Ap"], R["Ai"], Ax2, d2, "m", 0.5, 0.035 - 0.05, sp2)
+    assert np.allclose(Ax2, Ax1, rtol=4e-16, atol=0)
+    assert np.allclose(d2, d1, rtol=1e-14, atol=0)
+    assert np.allclose(sp2, sp1, rtol=0, atol=4e-16 * (np.abs(sp1).max() + 1))
+
+
+# ------------------------------------------------------------------ Jacobian (dense)
+
+@pytest.mark.parametrize("n", [64, 300])
+def test_jacobian_dense_vs_oracle(pkg, ctx, orc, n):
+    xy = _dconf64() if n == 64 else random_config(n, 11)
+    h = 1e-3
+    print(f"[dense] n={n}", flush=True)
+    p = pkg.jac_params(h=h, tau_a=0.1, tau_b=0.2, n=n, wfunc="p")
+    vA, vB, M, d = ctx.jacobian_dense(xy, p)
+    vA0, vB0, M0, d0 = orc.jacobian_dense(xy, h, 0.1, 0.2, "p")
+    assert np.array_equal(M, M.T)
+    i, j, dx, dy = all_pairs(xy)
+    e, c = orc.inputs(dx, dy)
+    ad, _ = pair_allowance(orc, dx, dy, e, c, deriv=True)
+    aV = np.zeros((n, n))
+    aV[i, j] = aV[j, i] = h * ad
+    np.fill_diagonal(aV, aV.sum(axis=1) + 1e-13 * np.abs(M0).sum(axis=1))
+    assert np.all(np.abs(M - M0) <= aV)
+    allow = speed_allowance(orc, xy)
+    assert np.all(np.abs(vA - vA0) <= allow + 1e-15) and np.all(np.abs(vB - vB0) <= allow + 1e-15)
+    assert np.all(np.abs(d - d0) <= np.diag(aV))
+    # scaled form equals the CSC pass's A (dense cutoff) to rounding
+    _, _, A, _ = ctx.jacobian_dense(xy, p, scale=True)
+    G = ctx.jacobian_pass(xy, p)
+    Acsc = np.zeros((n, n))
+    cols = np.repeat(np.arange(n), np.diff(G["Ap"]))
+    Acsc[G["Ai"], cols] = G["Ax"]
+    assert np.allclose(A, Acsc, rtol=1e-13, atol=1e-13 * np.abs(Acsc).max())
+
+
+# ------------------------------------------------------------------ full size, device-resident
+
+def test_full_size_speed_and_dense_invariants(pkg, ctx, orc):
+    """N = 16384 (the headline size): exact symmetry of the dense Jacobian, zero row sums of
+    V + diag(s), cancelling pair sums of the speeds, and sampled rows against the oracle."""
+    import torch
+    n = 16384
+    xy = random_config(n, 12)
+    dev = torch.device("cuda:0")
+    t = torch.from_numpy(xy).to(dev)
+    print(f"[full] speed pass n={n}", flush=True)
+    v = ctx.speed_pass_t(t, tau=0.0)
+    torch.cuda.synchronize()
+    vs = v.double()
+    assert abs(vs.sum().item()) <= 1e-12 * vs.abs().sum().item()
+    print(f"[full] dense jacobian n={n}", flush=True)
+    p = pkg.jac_params(h=1e-6, tau_a=0.0, tau_b=0.1, n=n)
+    vA, vB, Mt, d = ctx.jacobian_dense_t(t, p)  # Mt[col, row]
+    torch.cuda.synchronize()
+    assert torch.equal(Mt, Mt.t())
+    rs = Mt.sum(dim=0)
+    assert (rs.abs() <= 1e-12 * Mt.abs().sum(dim=0)).all().item()
+    assert torch.equal(vA, v)
+    rng = np.random.default_rng(13)
+    rows = rng.choice(n, 6, replace=False)
+    Mh_rows = Mt[torch.as_tensor(rows, device=dev)].cpu().numpy()  # columns of M = rows by symmetry
+    vh = v.cpu().numpy()
+    b = bsign(n)
+    for k, r in enumerate(rows.tolist()):
+        others = np.delete(np.arange(n), r)
+        lo = np.minimum(others, r)
+        hi = np.maximum(others, r)
+        dx = wrap(xy[lo, 0] - xy[hi, 0])
+        dy = wrap(xy[lo, 1] - xy[hi, 1])
+        e, c = orc.inputs(dx, dy)
+        a, s0 = pair_allowance(orc, dx, dy, e, c)
+        ad, d0 = pair_allowance(orc, dx, dy, e, c, deriv=True)
+        sgn = np.where(r == lo, 1.0, -1.0)
+        f = s0 * b[r] * b[others]
+        vr = (sgn * f).sum()
+        assert abs(vh[r] - vr) <= a.sum() + 1e-13 * np.abs(f).sum()
+        got = Mh_rows[k][others]
+        want = 1e-6 * b[r] * b[others] * d0
+        assert np.all(np.abs(got - want) <= 1e-6 * ad)
