@@ -53,6 +53,8 @@ SIGNATURES = {
     "d4_last_error": (C.c_char_p, [_vp]),
     "d4_ctx_set_stream": (C.c_int, [_vp, _vp]),
     "d4_ctx_synchronize": (C.c_int, [_vp]),
+    "d4_ctx_set_profiling": (C.c_int, [_vp, C.c_int]),
+    "d4_ctx_pair_kernel_time": (C.c_int, [_vp, _pd, _pi]),
     "d4_dev_alloc": (C.c_int, [_vp, C.c_int64, _P(_vp)]),
     "d4_dev_free": (C.c_int, [_vp, _vp]),
     "d4_memcpy_h2d": (C.c_int, [_vp, _vp, _vp, C.c_int64]),
@@ -74,11 +76,20 @@ _lib = None
 
 
 def lib():
-    """Loads lib2d4.so (built by __graft_entry__.build()); raises if it is missing."""
+    """Loads lib2d4.so (built by __graft_entry__.build()); raises if it is missing.
+
+    torch is imported first when available: it bundles its own HIP runtime
+    (soname libamdhip64.so.7), and loading it before lib2d4 makes the dynamic
+    linker bind lib2d4 to that same runtime, so one process holds one HIP
+    runtime and device pointers pass freely between torch and lib2d4."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
@@ -134,6 +145,15 @@ class Context:
 
     def synchronize(self):
         self.check(self.L.d4_ctx_synchronize(self.h))
+
+    def set_profiling(self, on=True):
+        self.check(self.L.d4_ctx_set_profiling(self.h, 1 if on else 0))
+
+    def pair_kernel_time(self):
+        """(summed ms, launches) of the pair kernels since the last call (profiling on)."""
+        ms, k = C.c_double(0.0), C.c_int(0)
+        self.check(self.L.d4_ctx_pair_kernel_time(self.h, C.byref(ms), C.byref(k)))
+        return ms.value, k.value
 
     # ---------------------------------------------------------- numpy API
     def field_eval(self, dx, dy, e2pix=None, c2piy=None, exact=False, want_sigma=True, want_dsigma=True):

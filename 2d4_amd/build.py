@@ -15,7 +15,8 @@ LIB = os.path.join(LIBDIR, "lib2d4.so")
 
 ARCH = "gfx950"
 # -packed-fp32-ops: keep hipcc from forming packed FP32 instructions
-HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+# code object v5: loadable by the HIP runtime torch bundles as well as /opt/rocm's
+HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-mcode-object-version=5",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wall", "-Wno-unused-command-line-argument"]
 

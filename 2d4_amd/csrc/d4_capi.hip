@@ -16,6 +16,11 @@ d4_ctx::~d4_ctx()
     for (auto* b : {&part, &tiles, &xy, &v0, &v1, &tol, &coo, &coo_count, &cnt, &cursor, &Ap, &Ai, &Ax, &idx, &dvec,
                     &nw, &aux0, &aux1, &aux2})
         b->release();
+    for (auto& m : ev_marks) {
+        ev_pool.push_back(m.first);
+        ev_pool.push_back(m.second);
+    }
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
     if (own_stream) (void)hipStreamDestroy(own_stream);
 }
 
@@ -42,6 +47,40 @@ int d4_ctx::ensure(DevBuf& b, size_t bytes, const char* what)
         return hip_fail(e, what);
     }
     b.bytes = cap;
+    return D4_OK;
+}
+
+hipEvent_t d4_ctx::take_event()
+{
+    if (!ev_pool.empty()) {
+        hipEvent_t e = ev_pool.back();
+        ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+int d4_ctx::mark_begin(hipEvent_t* e)
+{
+    *e = nullptr;
+    if (!profiling) return D4_OK;
+    *e = take_event();
+    if (!*e) return fail(D4_ERR_HIP, "event create");
+    hipError_t r = hipEventRecord(*e, stream);
+    if (r != hipSuccess) return hip_fail(r, "event record");
+    return D4_OK;
+}
+
+int d4_ctx::mark_end(hipEvent_t b)
+{
+    if (!profiling || !b) return D4_OK;
+    hipEvent_t e = take_event();
+    if (!e) return fail(D4_ERR_HIP, "event create");
+    hipError_t r = hipEventRecord(e, stream);
+    if (r != hipSuccess) return hip_fail(r, "event record");
+    ev_marks.emplace_back(b, e);
     return D4_OK;
 }
 
@@ -118,8 +157,13 @@ int d4_ctx::single_pass(int mode, int n, const double* d_xy, const RealParams& r
     a.w = d_w;
     a.exact = exact;
     a.want_dy = want_dy;
+    hipEvent_t ev = nullptr;
+    rc = mark_begin(&ev);
+    if (rc) return rc;
     hipError_t e = launch_pair_pass(mode, a, stream);
     if (e != hipSuccess) return hip_fail(e, "pair pass launch");
+    rc = mark_end(ev);
+    if (rc) return rc;
     ReduceArgs r{};
     r.part = a.part;
     r.nq = nq;
@@ -128,7 +172,8 @@ int d4_ctx::single_pass(int mode, int n, const double* d_xy, const RealParams& r
     r.rp0 = rp;
     r.R = 1;
     r.total = n;
-    r.elem_real = nullptr;
+    r.blocks = nullptr;
+    r.nblocks = 0;
     r.vA = d_vA;
     r.vB = d_vB;
     r.q1 = d_q1;
@@ -216,6 +261,35 @@ int d4_ctx_synchronize(d4_ctx* ctx)
 {
     if (!ctx) return D4_ERR_INVALID;
     return ctx->sync("synchronize");
+}
+
+int d4_ctx_set_profiling(d4_ctx* ctx, int on)
+{
+    if (!ctx) return D4_ERR_INVALID;
+    ctx->profiling = on != 0;
+    return D4_OK;
+}
+
+int d4_ctx_pair_kernel_time(d4_ctx* ctx, double* total_ms, int* launches)
+{
+    if (!ctx || !total_ms || !launches) return D4_ERR_INVALID;
+    double tot = 0.0;
+    int cnt = 0;
+    for (auto& m : ctx->ev_marks) {
+        hipError_t r = hipEventSynchronize(m.second);
+        if (r != hipSuccess) return ctx->hip_fail(r, "event sync");
+        float ms = 0.f;
+        r = hipEventElapsedTime(&ms, m.first, m.second);
+        if (r != hipSuccess) return ctx->hip_fail(r, "event elapsed");
+        tot += ms;
+        cnt++;
+        ctx->ev_pool.push_back(m.first);
+        ctx->ev_pool.push_back(m.second);
+    }
+    ctx->ev_marks.clear();
+    *total_ms = tot;
+    *launches = cnt;
+    return D4_OK;
 }
 
 int d4_dev_alloc(d4_ctx* ctx, int64_t bytes, void** out)

@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "d4_kernels.h"
 
@@ -28,6 +29,13 @@ struct d4_ctx {
     std::string err;
     int tiles_n = -1;  // n of the cached single-realization tile table
     DevBuf part, tiles, xy, v0, v1, tol, coo, coo_count, cnt, cursor, Ap, Ai, Ax, idx, dvec, nw, aux0, aux1, aux2;
+    // pair-kernel timing: event pairs recorded around launches while profiling is on
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_marks;
+    hipEvent_t take_event();
+    int mark_begin(hipEvent_t* e);
+    int mark_end(hipEvent_t b);
 
     ~d4_ctx();
     int fail(int code, const std::string& msg);

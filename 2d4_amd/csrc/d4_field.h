@@ -48,13 +48,17 @@ constexpr double kSh2 = 143375.656566582978695241314640;
 constexpr double kCh1h = 0.5 * kCh1, kSh1h = 0.5 * kSh1, kCh2h = 0.5 * kCh2, kSh2h = 0.5 * kSh2;
 
 // Reciprocal: hardware v_rcp_f64 seed refined by two Newton steps (each
-// doubles the correct bits); within ~1 ulp for the positive normal range used here.
+// doubles the correct bits).  The last step, y + y(1 - x y) with one FMA
+// rounding from a faithful y, returns the correctly rounded 1/x (Markstein's
+// theorem) for the positive normal range used here.
 D4_HD double rcp(double x)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
     double r = __builtin_amdgcn_rcp(x);
-#else
-    double r = (double)(1.0f / (float)x);
+#else  // host build (tests): a float-accurate seed over the whole double range
+    int ex = 0;
+    const double m = frexp(x, &ex);
+    double r = ldexp((double)(1.0f / (float)m), -ex);
 #endif
     double e = fma(-x, r, 1.0);
     r = fma(r, e, r);
@@ -104,23 +108,26 @@ D4_HD double wall(double u, double C, double S, double c, double w, double& sig,
 template <bool NEED_D>
 D4_HD void wall0_series(double dx, double dy, double& sig, double& dsig)
 {
+    // 1/n! and the other series denominators as products with their reciprocals
+    constexpr double i24 = 1.0 / 24, i720 = 1.0 / 720;
     double a = kTwoPi * dx, b = kTwoPi * dy;
     double a2 = a * a, a4 = a2 * a2, a6 = a4 * a2;
     double b2 = b * b, b4 = b2 * b2, b6 = b4 * b2;
-    double chm1 = a6 / 720 + a4 / 24 + a2 * 0.5;
-    double cm1 = -b6 / 720 + b4 / 24 - b2 * 0.5;
-    double D = (b6 + a6) / 720 + (a4 - b4) / 24 + (a2 + b2) * 0.5;
+    double chm1 = a6 * i720 + a4 * i24 + a2 * 0.5;
+    double cm1 = -b6 * i720 + b4 * i24 - b2 * 0.5;
+    double D = (b6 + a6) * i720 + (a4 - b4) * i24 + (a2 + b2) * 0.5;
     double N = chm1 * cm1 + cm1 + chm1;
     double r = rcp(D);
     double t = N * r;
     sig = fma(dx, t * r, sig);
     if (NEED_D) {
         double a3 = a2 * a, a5 = a4 * a, a7 = a6 * a;
-        double cs = -(a7 * b6) / 3628800 - (a5 * b6) / 86400 - (a3 * b6) / 4320 - (b6 * a) / 720
-                  + (a7 * b4) / 120960 + (a5 * b4) / 2880 + (a3 * b4) / 144 + (b4 * a) / 24
-                  - (a7 * b2) / 1080 - (a5 * b2) / 240 - (a3 * b2) / 12 - (b2 * a) * 0.5
-                  + a7 / 5040 + a5 / 120 + a3 / 6 + a;
-        double sh = a + a3 / 6 + a5 / 120 + a7 / 5040;  // sinh(a), |a| <= 2 pi 1e-3
+        double cs = -(a7 * b6) * (1.0 / 3628800) - (a5 * b6) * (1.0 / 86400) - (a3 * b6) * (1.0 / 4320)
+                  - (b6 * a) * i720 + (a7 * b4) * (1.0 / 120960) + (a5 * b4) * (1.0 / 2880)
+                  + (a3 * b4) * (1.0 / 144) + (b4 * a) * i24 - (a7 * b2) * (1.0 / 1080)
+                  - (a5 * b2) * (1.0 / 240) - (a3 * b2) * (1.0 / 12) - (b2 * a) * 0.5
+                  + a7 * (1.0 / 5040) + a5 * (1.0 / 120) + a3 * (1.0 / 6) + a;
+        double sh = a + a3 * (1.0 / 6) + a5 * (1.0 / 120) + a7 * (1.0 / 5040);  // sinh(a), |a| <= 2 pi 1e-3
         double g = fma(kTwoPi * dx, fma(-2.0 * t, sh, cs), N);
         dsig += g * r * r;
     }
@@ -133,11 +140,11 @@ struct Hyp {
     double Cp1, Cm1, Sp1, Sm1, Cp2, Cm2, Sp2, Sm2;
 };
 
-D4_HD Hyp hyperbolics(double E)
+D4_HD Hyp hyperbolics(double E, double R)
 {
 #pragma clang fp contract(off)
     Hyp h;
-    h.R = 1.0 / E;
+    h.R = R;
     h.P = E + h.R;
     h.M = E - h.R;
     double a = h.P * kCh1h, b = h.M * kSh1h, as = h.M * kCh1h, bs = h.P * kSh1h;
@@ -177,55 +184,185 @@ D4_HD void wall0(double dx, double C, double S, double c, double& sig, double& d
     }
 }
 
-// sigma (and d sigma/dx if NEED_D) of one pair from (dx, dy) already wrapped
-// and the two transcendental inputs E = 2^(kPi2Ln2Inv dx), c = cos(2 pi dy).
-template <bool NEED_D, bool EXACT>
-D4_HD void pair_field(double dx, double dy, double E, double c, double& sig_out, double& dsig_out)
+// Ten reciprocals from one: Montgomery's batch inversion over a balanced
+// product tree (9 products, one rcp(), 18 products back).  Each result is within
+// a few ulp of 1/x.  The products stay far inside binary64's range here
+// (wall denominators lie between ~2e-5 and ~1e15).
+D4_HD void batch_rcp10(const double* x, double* r)
 {
-    const Hyp h = hyperbolics(E);
-    const double R = h.R;
+    const double p01 = x[0] * x[1], p23 = x[2] * x[3], p45 = x[4] * x[5], p67 = x[6] * x[7], p89 = x[8] * x[9];
+    const double p03 = p01 * p23, p47 = p45 * p67;
+    const double p07 = p03 * p47;
+    const double iall = rcp(p07 * p89);
+    const double i07 = iall * p89, i89 = iall * p07;
+    const double i03 = i07 * p47, i47 = i07 * p03;
+    const double i01 = i03 * p23, i23 = i03 * p01, i45 = i47 * p67, i67 = i47 * p45;
+    r[0] = i01 * x[1];
+    r[1] = i01 * x[0];
+    r[2] = i23 * x[3];
+    r[3] = i23 * x[2];
+    r[4] = i45 * x[5];
+    r[5] = i45 * x[4];
+    r[6] = i67 * x[7];
+    r[7] = i67 * x[6];
+    r[8] = i89 * x[9];
+    r[9] = i89 * x[8];
+}
+
+// sigma (and d sigma/dx if NEED_D) of one pair from (dx, dy) already wrapped
+// and the transcendental inputs E = 2^(kPi2Ln2Inv dx), R = 1/E, c = cos(2 pi dy).
+// Walls (index: offset u, weight w): 0: dx + 3; 1: dx - 3; 2: dx - 4s (1);
+// 3: dx + 4s (1 - |dx|); 4: dx - 5s (|dx|); 5, 6: dx +- 2; 7, 8: dx +- 1;
+// 9: dx (the central wall, or the series when dx^2 + dy^2 <= 1e-6).
+template <bool NEED_D, bool EXACT>
+D4_HD void pair_field(double dx, double dy, double E, double R, double c, double& sig_out, double& dsig_out)
+{
+    const Hyp h = hyperbolics(E, R);
     const bool pos = dx >= 0.0;
     const double sg = pos ? 1.0 : -1.0;
     const double ad = pos ? dx : -dx;
+    const bool series = !(dist2(dx, dy) > 1e-6);
 
-    // |k| >= 3 walls: C = cosh(2 pi k) E^{+-1}; "sinh" = C (reference) or +-C (exact)
-    const double Cp3 = kCh3 * E, Cm3 = kCh3 * R;
-    const double Cp4 = kCh4 * E, Cm4 = kCh4 * R;
-    const double Cp5 = kCh5 * E, Cm5 = kCh5 * R;
-    const double CA = pos ? Cm4 : Cp4;  // dx - 4s, weight 1
-    const double CB = pos ? Cp4 : Cm4;  // dx + 4s, weight 1 - |dx|
-    const double CC = pos ? Cm5 : Cp5;  // dx - 5s, weight |dx|
-    const double neg = EXACT ? -1.0 : 1.0;
-    const double SA = EXACT ? -sg * CA : CA;
-    const double SB = EXACT ? sg * CB : CB;
-    const double SC = EXACT ? -sg * CC : CC;
+    // cosh of the ten walls; offsets u and "sinh" values are formed again when used
+    const double Cp4 = kCh4 * E, Cm4 = kCh4 * R, Cp5 = kCh5 * E, Cm5 = kCh5 * R;
+    double C[10];
+    C[0] = kCh3 * E;
+    C[1] = kCh3 * R;
+    C[2] = pos ? Cm4 : Cp4;
+    C[3] = pos ? Cp4 : Cm4;
+    C[4] = pos ? Cm5 : Cp5;
+    C[5] = h.Cp2;
+    C[6] = h.Cm2;
+    C[7] = h.Cp1;
+    C[8] = h.Cm1;
+    C[9] = 0.5 * h.P;
 
-    double sig = 0.0, dsig = 0.0;
-    wall<NEED_D>(dx - 4.0 * sg, CA, SA, c, 1.0, sig, dsig);
-    const double fB = wall<NEED_D>(dx + 4.0 * sg, CB, SB, c, 1.0 - ad, sig, dsig);
-    const double fC = wall<NEED_D>(dx - 5.0 * sg, CC, SC, c, ad, sig, dsig);
-    wall<NEED_D>(dx + 3.0, Cp3, Cp3, c, 1.0, sig, dsig);
-    wall<NEED_D>(dx - 3.0, Cm3, neg * Cm3, c, 1.0, sig, dsig);
-    wall<NEED_D>(dx + 2.0, h.Cp2, h.Sp2, c, 1.0, sig, dsig);
-    wall<NEED_D>(dx - 2.0, h.Cm2, h.Sm2, c, 1.0, sig, dsig);
-    wall<NEED_D>(dx + 1.0, h.Cp1, h.Sp1, c, 1.0, sig, dsig);
-    wall<NEED_D>(dx - 1.0, h.Cm1, h.Sm1, c, 1.0, sig, dsig);
-    // central wall
-    if (dist2(dx, dy) > 1e-6)
-        wall0<NEED_D>(dx, 0.5 * h.P, 0.5 * h.M, c, sig, dsig);
-    else
-        wall0_series<NEED_D>(dx, dy, sig, dsig);
+    double D[10], N[10], r[10];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        D[k] = C[k] - c;
+        N[k] = fma(C[k], c, -1.0);
+    }
+    {
+        // central wall: C - c and C c - 1 rounded as the reference rounds them
+        // (Field.cpp:178-179), since for near pairs both cancel
+#pragma clang fp contract(off)
+        D[9] = C[9] - c;
+        N[9] = C[9] * c - 1.0;
+    }
+    if (series) D[9] = 1.0;  // the series branch brings its own denominator
+    batch_rcp10(D, r);
+
+    double sig = 0.0, dsig = 0.0, f3 = 0.0, f4 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) {
+        if (k == 9 && series) break;
+        const double uk = k == 0 ? dx + 3.0 : k == 1 ? dx - 3.0 : k == 2 ? dx - 4.0 * sg : k == 3 ? dx + 4.0 * sg
+                        : k == 4 ? dx - 5.0 * sg : k == 5 ? dx + 2.0 : k == 6 ? dx - 2.0 : k == 7 ? dx + 1.0
+                        : k == 8 ? dx - 1.0 : dx;
+        const double r2 = r[k] * r[k];
+        const double Nr2 = N[k] * r2;
+        const double f = uk * Nr2;
+        const double w = k == 3 ? 1.0 - ad : (k == 4 ? ad : 1.0);
+        sig = fma(w, f, sig);
+        if (k == 3) f3 = f;
+        if (k == 4) f4 = f;
+        if (NEED_D) {
+            // sinh of the wall: identities for |k| <= 2, the cosh value itself (reference,
+            // Field.cpp:269-283) or its shift-signed copy (EXACT) beyond
+            double Sk;
+            if (k == 9) Sk = 0.5 * h.M;
+            else if (k == 5) Sk = h.Sp2;
+            else if (k == 6) Sk = h.Sm2;
+            else if (k == 7) Sk = h.Sp1;
+            else if (k == 8) Sk = h.Sm1;
+            else if (!EXACT) Sk = C[k];
+            else Sk = (k == 0 ? 1.0 : k == 1 ? -1.0 : k == 3 ? sg : -sg) * C[k];
+            const double q = fma(-2.0, N[k] * r[k], c);
+            const double g = fma(kTwoPi * uk, Sk * q, N[k]);
+            dsig = fma(w * g, r2, dsig);
+        }
+    }
+    if (series) wall0_series<NEED_D>(dx, dy, sig, dsig);
 
     sig_out = kTwoPiSq * sig;
-    if (NEED_D) dsig_out = kTwoPiSq * fma(sg, fC - fB, dsig);
+    if (NEED_D) dsig_out = kTwoPiSq * fma(sg, f4 - f3, dsig);
 }
 
 // The reference's transcendental inputs of a pair (simulation.cpp:456-457,
-// Field.cpp:335-336).
-D4_HD void field_inputs(double dx, double dy, double& E, double& c)
+// Field.cpp:335-336): E = exp2(PI2LN2INV * dx), c = cos(2 pi dy), here with
+// R = 1/E as well.  Own implementations, each within ~1 ulp:
+//   exp2_pair: y = k + f, |f| <= 1/2, 2^f from the Taylor series of e^{f ln 2}
+//     through f^13 (truncation < 5e-18), scaled by 2^k; R = 1/E by rcp();
+//   cos2pi_reduced: half-angle form with the Taylor series of sin through
+//     r^17 on |r| <= pi/4 (truncation < 6e-17 relative).
+namespace tc {
+// ln2^n / n!
+constexpr double e1 = 0x1.62e42fefa39efp-1, e2 = 0x1.ebfbdff82c58fp-3, e3 = 0x1.c6b08d704a0c0p-5,
+                 e4 = 0x1.3b2ab6fba4e77p-7, e5 = 0x1.5d87fe78a6731p-10, e6 = 0x1.430912f86c787p-13,
+                 e7 = 0x1.ffcbfc588b0c7p-17, e8 = 0x1.62c0223a5c824p-20, e9 = 0x1.b5253d395e7c4p-24,
+                 e10 = 0x1.e4cf5158b8ecap-28, e11 = 0x1.e8cac7351bb25p-32, e12 = 0x1.c3bd650fc2986p-36,
+                 e13 = 0x1.816193166d0f9p-40;
+// (-1)^k / n!
+constexpr double s3 = -0x1.5555555555555p-3, s5 = 0x1.1111111111111p-7, s7 = -0x1.a01a01a01a01ap-13,
+                 s9 = 0x1.71de3a556c734p-19, s11 = -0x1.ae64567f544e4p-26, s13 = 0x1.6124613a86d09p-33,
+                 s15 = -0x1.ae7f3e733b81fp-41, s17 = 0x1.952c77030ad4ap-49;
+constexpr double c2 = -0x1.0000000000000p-1, c4 = 0x1.5555555555555p-5, c6 = -0x1.6c16c16c16c17p-10,
+                 c8 = 0x1.a01a01a01a01ap-16, c10 = -0x1.27e4fb7789f5cp-22, c12 = 0x1.1eed8eff8d898p-29,
+                 c14 = -0x1.93974a8c07c9dp-37, c16 = 0x1.ae7f3e733b81fp-45;
+constexpr double pio2_1 = 0x1.921fb54442d18p+0, pio2_2 = 0x1.1a62633145c07p-54, two_over_pi = 0x1.45f306dc9c883p-1;
+}  // namespace tc
+
+D4_HD void exp2_pair(double y, double& E, double& R)
 {
-    E = exp2(kPi2Ln2Inv * dx);
-    c = cos(kTwoPi * dy);
+    const double k = rint(y);
+    const double f = y - k;
+    double p = fma(f, tc::e13, tc::e12);
+    p = fma(f, p, tc::e11);
+    p = fma(f, p, tc::e10);
+    p = fma(f, p, tc::e9);
+    p = fma(f, p, tc::e8);
+    p = fma(f, p, tc::e7);
+    p = fma(f, p, tc::e6);
+    p = fma(f, p, tc::e5);
+    p = fma(f, p, tc::e4);
+    p = fma(f, p, tc::e3);
+    p = fma(f, p, tc::e2);
+    p = fma(f, p, tc::e1);
+    p = fma(f, p, 1.0);
+    E = ldexp(p, (int)k);
+    // 1/E correctly rounded, as the reference's identities take it (constants.h:63-79):
+    // the last Newton step of rcp() runs from a faithful estimate, which rounds correctly
+    R = rcp(E);
+}
+
+// cos(2 pi dy) for |dy| <= 1/2 from one sine polynomial: with theta = pi dy,
+// cos 2 theta = 1 - 2 sin^2 theta for |theta| <= pi/4 and
+// -(1 - 2 sin^2 phi), phi = pi/2 - |theta|, above; |rho| <= pi/4 either way, and
+// 1 - 2 s^2 keeps c accurate to rounding where c -> 1 (near pairs).
+D4_HD double cos2pi_reduced(double dy)
+{
+    const double th = kPi * dy;  // = fl(2 pi dy) / 2 exactly
+    const double at = fabs(th);
+    const bool hi = at > 0.5 * tc::pio2_1;
+    const double rho = hi ? (tc::pio2_1 - at) + tc::pio2_2 : th;
+    const double r2 = rho * rho;
+    double sp = fma(r2, tc::s17, tc::s15);
+    sp = fma(r2, sp, tc::s13);
+    sp = fma(r2, sp, tc::s11);
+    sp = fma(r2, sp, tc::s9);
+    sp = fma(r2, sp, tc::s7);
+    sp = fma(r2, sp, tc::s5);
+    sp = fma(r2, sp, tc::s3);
+    const double sn = fma(rho * r2, sp, rho);
+    const double v = fma(-2.0 * sn, sn, 1.0);
+    return hi ? -v : v;
+}
+
+D4_HD void field_inputs(double dx, double dy, double& E, double& R, double& c)
+{
+    exp2_pair(kPi2Ln2Inv * dx, E, R);
+    c = cos2pi_reduced(dy);
 }
 
 // d sigma / d dy (no reference counterpart; adjoint only): every wall's
@@ -239,10 +376,9 @@ D4_HD double wall_dy(double u, double C, double c, double w, double acc)
     return fma(w * u * num, r * r * r, acc);
 }
 
-D4_HD double pair_field_dy(double dx, double E, double c, double s2piy)
+D4_HD double pair_field_dy(double dx, double E, double R, double c, double s2piy)
 {
-    const Hyp h = hyperbolics(E);
-    const double R = h.R;
+    const Hyp h = hyperbolics(E, R);
     const bool pos = dx >= 0.0;
     const double sg = pos ? 1.0 : -1.0;
     const double ad = pos ? dx : -dx;
@@ -261,9 +397,10 @@ D4_HD double pair_field_dy(double dx, double E, double c, double s2piy)
     return -kTwoPiSq * kTwoPi * s2piy * acc;
 }
 
-// sim_utils.cpp:36-54
+// sim_utils.cpp:36-54 (unfused, as the reference rounds it)
 D4_HD double weight(double s, char kind)
 {
+#pragma clang fp contract(off)
     if (s > 0) {
         if (kind == 'c') { double q = s / (s + 1); return q * q; }
         if (kind == 'p') return s / (s + 1);
@@ -272,9 +409,10 @@ D4_HD double weight(double s, char kind)
     return 0;
 }
 
-// sim_utils.cpp:57-75
+// sim_utils.cpp:57-75 (unfused)
 D4_HD double weight_inv(double w, char kind)
 {
+#pragma clang fp contract(off)
     if (w > 0) {
         if (kind == 'c') return sqrt(w) / (1 - sqrt(w));
         if (kind == 'p') return w / (1 - w);

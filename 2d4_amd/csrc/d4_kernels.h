@@ -90,7 +90,8 @@ struct ReduceArgs {
     RealParams rp0;
     int R;
     int total;               // sum of n_r
-    const int* elem_real;    // element -> realization (null when R == 1)
+    const int2* blocks;      // (realization, element block) per reduce block (null: one realization)
+    int nblocks;
     double* vA;              // quantity 0 + tauA b
     double* vB;              // quantity 0 + tauB b (may be null)
     double* q1;              // quantity 1 raw (may be null)

@@ -25,8 +25,12 @@ __device__ __forceinline__ void atomic_min_nonneg(double* p, double v)
     atomicMin(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
+// 64 threads (one wavefront) per block; at least 4 waves per SIMD (<= 128
+// VGPRs) for the sigma-only pass, 3 (<= 168 VGPRs) for the dense pass, 2 for
+// the pair-list and adjoint passes (their extra live values would spill at 168)
 template <int MODE, bool EXACT>
-__global__ __launch_bounds__(64) void pair_pass_kernel(PassArgs a)
+__global__ __launch_bounds__(64, MODE == kModeSpeed ? 4 : (MODE == kModeJacDense ? 3 : 2)) void pair_pass_kernel(
+    PassArgs a)
 {
     constexpr bool kDense = MODE == kModeJacDense;
     constexpr bool kCoo = MODE == kModeJacCoo;
@@ -80,16 +84,16 @@ __global__ __launch_bounds__(64) void pair_pass_kernel(PassArgs a)
                     atomic_min_nonneg(a.tol + base + j, d2);
                 }
                 const double bj = j < half ? 1.0 : -1.0;
-                double E, c;
-                field_inputs(dx, dy, E, c);
+                double E, R, c;
+                field_inputs(dx, dy, E, R, c);
                 double sig = 0.0, dsig = 0.0;
                 if (MODE == kModeSpeed) {
-                    pair_field<false, false>(dx, dy, E, c, sig, dsig);
+                    pair_field<false, false>(dx, dy, E, R, c, sig, dsig);
                     const double f = sig * bi * bj;
                     q0 += f;
                     atomicAdd(&jq0[jl], -f);
                 } else if (kDense) {
-                    pair_field<true, EXACT>(dx, dy, E, c, sig, dsig);
+                    pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
                     const double f = sig * bi * bj;
                     V = rp.h * bj * bi * dsig;
                     q0 += f;
@@ -99,27 +103,27 @@ __global__ __launch_bounds__(64) void pair_pass_kernel(PassArgs a)
                 } else if (kCoo) {
                     inr = d2 < rp.range_sq;
                     if (inr) {
-                        pair_field<true, EXACT>(dx, dy, E, c, sig, dsig);
+                        pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
                         V = rp.h * bj * bi * dsig;
                         if (!rp.heaviside && d2 > rp.cut_sq) {  // simulation.cpp:469-470
                             const double e = sqrt(d2) - rp.cutoff;
                             V *= exp(-(e * e) * rp.one_per);
                         }
                     } else {
-                        pair_field<false, false>(dx, dy, E, c, sig, dsig);
+                        pair_field<false, false>(dx, dy, E, R, c, sig, dsig);
                     }
                     const double f = sig * bi * bj;
                     q0 += f;
                     atomicAdd(&jq0[jl], -f);
                 } else {  // kVjp
-                    pair_field<true, EXACT>(dx, dy, E, c, sig, dsig);
+                    pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
                     const double bb = bi * bj;
                     const double coef = (wi - sw[jl]) * bb;
                     const double gx = coef * dsig;
                     q0 += gx;
                     atomicAdd(&jq0[jl], -gx);
                     if (a.want_dy) {
-                        const double gy = coef * pair_field_dy(dx, E, c, sin(kTwoPi * dy));
+                        const double gy = coef * pair_field_dy(dx, E, R, c, sin(kTwoPi * dy));
                         q1 += gy;
                         atomicAdd(&jq1[jl], -gy);
                     }
@@ -151,11 +155,29 @@ __global__ __launch_bounds__(64) void pair_pass_kernel(PassArgs a)
             const int g = lane >> 4, q = lane & 15;
             const int ib = td.I0 + 16 * g;
             const int jb = td.J0 + 16 * (g ^ ph);
-            for (int k = 0; k < 16; ++k) {
-                const int i1 = ib + k, j1 = jb + q;  // V(i1, j1) -> column i1, row j1
-                if (i1 < n && j1 < n && (!diag || i1 < j1)) a.M[(int64_t)i1 * a.ld + j1] = stage[(g * 16 + k) * 17 + q];
-                const int i2 = ib + q, j2 = jb + k;  // V(i2, j2) -> column j2, row i2
-                if (i2 < n && j2 < n && (!diag || i2 < j2)) a.M[(int64_t)j2 * a.ld + i2] = stage[(g * 16 + q) * 17 + k];
+            const double* st1 = stage + (g * 16) * 17 + q;  // stage[g][k][q], k = 0..15
+            const double* st2 = stage + (g * 16 + q) * 17;  // stage[g][q][k]
+            // V(ib+k, jb+q) -> column ib+k, row jb+q;  V(ib+q, jb+k) -> column jb+k, row ib+q
+            double* p1 = a.M + (int64_t)ib * a.ld + jb + q;
+            double* p2 = a.M + (int64_t)jb * a.ld + ib + q;
+            if (!diag && td.I0 + kTile <= n && td.J0 + kTile <= n) {
+                // interior tile: every entry valid
+#pragma unroll 4
+                for (int k = 0; k < 16; ++k) {
+                    *p1 = st1[k * 17];
+                    *p2 = st2[k];
+                    p1 += a.ld;
+                    p2 += a.ld;
+                }
+            } else {
+                for (int k = 0; k < 16; ++k) {
+                    const int i1 = ib + k, j1 = jb + q;
+                    if (i1 < n && j1 < n && (!diag || i1 < j1)) *p1 = st1[k * 17];
+                    const int i2 = ib + q, j2 = jb + k;
+                    if (i2 < n && j2 < n && (!diag || i2 < j2)) *p2 = st2[k];
+                    p1 += a.ld;
+                    p2 += a.ld;
+                }
             }
             __syncthreads();
         }
@@ -200,16 +222,21 @@ __global__ void field_eval_kernel(int64_t n, const double* dx, const double* dy,
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const double x = dx[k], y = dy[k];
-    double e, cc;
-    if (E && c) { e = E[k]; cc = c[k]; }
-    else field_inputs(x, y, e, cc);
+    double e, r, cc;
+    if (E && c) {  // the 4-arg overloads: R as the reference forms it, 1/E
+        e = E[k];
+        cc = c[k];
+        r = 1.0 / e;
+    } else {
+        field_inputs(x, y, e, r, cc);
+    }
     double s = 0.0, ds = 0.0;
     if (dsxy) {
-        if (exact) pair_field<true, true>(x, y, e, cc, s, ds);
-        else pair_field<true, false>(x, y, e, cc, s, ds);
+        if (exact) pair_field<true, true>(x, y, e, r, cc, s, ds);
+        else pair_field<true, false>(x, y, e, r, cc, s, ds);
         dsxy[k] = ds;
     } else {
-        pair_field<false, false>(x, y, e, cc, s, ds);
+        pair_field<false, false>(x, y, e, r, cc, s, ds);
     }
     if (sxy) sxy[k] = s;
 }
@@ -226,30 +253,52 @@ hipError_t launch_field_eval(int64_t n, const double* dx, const double* dy, cons
 
 // ------------------------------------------------------------- reduction
 
-__global__ void reduce_kernel(ReduceArgs a)
+// One block of kReduceWaves wavefronts per 64-element block b of a realization:
+// wave w adds the partial items w, w + W, w + 2W, ... (items: target-side sums
+// of tiles (b, J), J >= b, then source-side sums of tiles (I, b), I <= b), and
+// wave 0 adds the W wave sums in order: a fixed order, so runs are bitwise
+// repeatable.
+constexpr int kReduceWaves = 8;
+
+__global__ __launch_bounds__(64 * kReduceWaves) void reduce_kernel(ReduceArgs a)
 {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.total) return;
-    const int r = a.elem_real ? a.elem_real[g] : 0;
+#pragma clang fp contract(off)
+    __shared__ double sm[kReduceWaves][2][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int r = 0, b = blockIdx.x;
+    if (a.blocks) {
+        r = a.blocks[blockIdx.x].x;
+        b = a.blocks[blockIdx.x].y;
+    }
     const int base = a.off ? a.off[r] : 0;
     const int n = a.off ? a.off[r + 1] - base : a.total;
-    const int k = g - base;
-    const int b = k / kTile, l = k % kTile;
     const RealParams rp = a.rp ? a.rp[r] : a.rp0;
     const int nb = rp.nb;
-    double sum[2] = {0.0, 0.0};
-    for (int q = 0; q < a.nq; ++q) {
-        double acc = 0.0;
-        // target side: tiles (b, J), J >= b
-        const int row0 = rp.slot0 + b * nb - b * (b - 1) / 2;
-        for (int J = b; J < nb; ++J) acc += a.part[((size_t)(row0 + J - b) * a.nq + q) * 128 + l];
-        // source side: tiles (I, b), I <= b
-        for (int I = 0; I <= b; ++I) {
+    const int items = nb + 1;
+    const int row0 = rp.slot0 + b * nb - b * (b - 1) / 2;  // slot of tile (b, b)
+    double acc[2] = {0.0, 0.0};
+    for (int it = w; it < items; it += kReduceWaves) {
+        size_t off;
+        if (it < nb - b) {  // target side of tile (b, b + it)
+            off = (size_t)(row0 + it) * a.nq * 128 + lane;
+        } else {           // source side of tile (I, b), I = it - (nb - b)
+            const int I = it - (nb - b);
             const int slot = rp.slot0 + I * nb - I * (I - 1) / 2 + (b - I);
-            acc += a.part[((size_t)slot * a.nq + q) * 128 + 64 + l];
+            off = (size_t)slot * a.nq * 128 + 64 + lane;
         }
-        sum[q] = acc;
+        acc[0] += a.part[off];
+        if (a.nq == 2) acc[1] += a.part[off + 128];
     }
+    sm[w][0][lane] = acc[0];
+    sm[w][1][lane] = acc[1];
+    __syncthreads();
+    if (w != 0) return;
+    const int k = b * kTile + lane;
+    if (k >= n) return;
+    double sum[2] = {0.0, 0.0};
+    for (int q = 0; q < 2; ++q)
+        for (int v = 0; v < kReduceWaves; ++v) sum[q] += sm[v][q][lane];
+    const int g = base + k;
     const double bsg = k < (n >> 1) ? 1.0 : -1.0;
     if (a.vA) a.vA[g] = a.add_tau ? sum[0] + rp.tauA * bsg : sum[0];
     if (a.vB) a.vB[g] = a.add_tau ? sum[0] + rp.tauB * bsg : sum[0];
@@ -261,8 +310,8 @@ __global__ void reduce_kernel(ReduceArgs a)
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s)
 {
     if (a.total <= 0) return hipSuccess;
-    const int bs = 256;
-    hipLaunchKernelGGL(reduce_kernel, dim3((a.total + bs - 1) / bs), dim3(bs), 0, s, a);
+    const int nblocks = a.blocks ? a.nblocks : (a.total + kTile - 1) / kTile;
+    hipLaunchKernelGGL(reduce_kernel, dim3(nblocks), dim3(64 * kReduceWaves), 0, s, a);
     return hipGetLastError();
 }
 
@@ -402,6 +451,7 @@ hipError_t launch_csc_finish(const int* Ap, int* Ai, double* Ax, int* indexes, d
 __global__ void csc_scale_kernel(const int* Ap, const int* Ai, double* Ax, const int* indexes, const double* dvec,
                                  int total)
 {
+#pragma clang fp contract(off)
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= total) return;
     for (int k = Ap[c]; k < Ap[c + 1]; ++k) Ax[k] *= (1 + dvec[Ai[k]]) / 2;
@@ -431,6 +481,7 @@ __global__ void rescale_weights_kernel(RescaleArgs a)
 
 __global__ void rescale_entries_kernel(RescaleArgs a)
 {
+#pragma clang fp contract(off)
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.total) return;
     for (int k = a.Ap[c]; k < a.Ap[c + 1]; ++k) {  // simulation.cpp:575-594
@@ -443,6 +494,7 @@ __global__ void rescale_entries_kernel(RescaleArgs a)
 
 __global__ void rescale_commit_kernel(RescaleArgs a)
 {
+#pragma clang fp contract(off)
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.total) return;
     const int r = a.elem_real ? a.elem_real[g] : 0;

@@ -1,0 +1,224 @@
+"""Benchmark of the periodic pair-interaction path on one MI355X.
+
+Headline (one JSON line on stdout): FP64 pair evaluations per second of the
+Jacobian + speed pass at N = 16384 -- speeds for all N plus the full dense
+dv_i/dx_j (as M = V + diag(s) = -h dv/dx, column-major), every ordered pair
+(i, j), i != j, counted once: N (N - 1) = 268 419 072 per pass.  Inputs are
+resident in HBM before the timed region; a step is one such pass.
+
+Also reported: the pair kernel's roofline (algorithmic FP64 operations per
+launch over its HIP-event-timed duration against the FP64 vector peak), the
+CPU baseline (the oracle's C restatement, timed on this host), the speed-only
+pass and the reference's final row-scaled matrix.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector (= FP64 matrix) peak
+PEAK_HBM_GBS = 8000.0
+# algorithmic FP64 operations per unordered pair, counted from the reference
+# source (+ - * / = 1, exp2 and cos = 1 each, shared subexpressions once per function):
+OPS_JAC_PAIR = 390          # setup 7 + Field::xy 125 + Field::xy_diff_x 248 + assembly 8 + 2 transcendental
+OPS_SPEED_PAIR = 138        # setup 7 + Field::xy 125 + assembly 4 + 2 transcendental
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def sorted_config(n, seed):
+    rng = np.random.default_rng(seed)
+    xy = rng.uniform(-0.5, 0.5, (n, 2))
+    b = np.where(np.arange(n) % 2 == 1, 1.0, -1.0)
+    return np.ascontiguousarray(xy[np.argsort(-(xy[:, 1] + b), kind="stable")])
+
+
+def cpu_baseline(n_single, n_omp, min_seconds):
+    """The oracle (oracle/liboracle_fast.so: reference flags, OpenMP) on this host's cores."""
+    import oracle
+    import ctypes as C
+    orc = oracle.Oracle(fast=True)
+    out = {}
+    for label, n, fn in (("1core", n_single, "orc_jacobian_dense"), ("omp", n_omp, "orc_jacobian_dense_omp")):
+        xy = sorted_config(n, 5)
+        vA, vB, d = np.empty(n), np.empty(n), np.empty(n)
+        M = np.empty((n, n))
+        p = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
+        f = getattr(orc.L, fn)
+
+        def run():
+            if fn == "orc_jacobian_dense":
+                f(n, p(xy), 1e-6, 0.0, 0.1, b"c", p(vA), p(vB), p(M), p(d), None, 0.0)
+            else:
+                f(n, p(xy), 1e-6, 0.0, 0.1, b"c", p(vA), p(vB), p(M), p(d))
+        run()  # warm-up
+        times = []
+        t_all = time.perf_counter()
+        while len(times) < 5 or time.perf_counter() - t_all < min_seconds:
+            t0 = time.perf_counter()
+            run()
+            times.append(time.perf_counter() - t0)
+            if len(times) >= 50:
+                break
+        med = float(np.median(times))
+        out[label] = dict(n=n, seconds=med, reps=len(times), pair_evals_per_s=n * (n - 1) / med,
+                          threads=1 if label == "1core" else orc.L.orc_num_threads())
+    return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--cpu-seconds", type=float, default=6.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the speed-only and scaled-matrix side measurements")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB three times (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is described "
+              "in DESIGN.md and not run here", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if args.calib_copy_mib:
+        nel = args.calib_copy_mib * (1 << 20) // 8
+        src = torch.rand(nel, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes": nel * 8}), flush=True)
+        return
+    pkg = importlib.import_module("2d4_amd")
+    dev = torch.device("cuda:0")
+    n = args.n
+    ctx = pkg.Context(0)
+    xy = torch.from_numpy(sorted_config(n, 1234)).to(dev)
+    M = torch.empty((n, n), dtype=torch.float64, device=dev)
+    p = pkg.jac_params(h=1e-6, tau_a=0.0, tau_b=0.1, n=n, wfunc="c")
+    pairs = n * (n - 1)
+    unordered = pairs // 2
+
+    log(f"[bench] n={n} warmup={args.warmup}")
+    for _ in range(args.warmup):
+        ctx.jacobian_dense_t(xy, p, M=M)
+    torch.cuda.synchronize()
+    ctx.set_profiling(True)
+    ctx.pair_kernel_time()
+    log(f"[bench] timing {args.steps} passes")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.jacobian_dense_t(xy, p, M=M)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    k_ms, k_launches = ctx.pair_kernel_time()
+    ctx.set_profiling(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    k_avg_ms = k_ms / max(k_launches, 1)
+    value = pairs / (ms_step * 1e-3)
+    achieved = OPS_JAC_PAIR * unordered / (k_avg_ms * 1e-3) / 1e12
+
+    extra = {}
+    if not args.no_extra:
+        v = torch.empty(n, dtype=torch.float64, device=dev)
+        for _ in range(2):
+            ctx.speed_pass_t(xy, 0.1, out=v)
+        torch.cuda.synchronize()
+        ctx.set_profiling(True)
+        ctx.pair_kernel_time()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.speed_pass_t(xy, 0.1, out=v)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        s_ms, s_l = ctx.pair_kernel_time()
+        ctx.set_profiling(False)
+        s_kavg = s_ms / max(s_l, 1)
+        extra["speed_pass"] = dict(ms_per_pass=(t1 - t0) * 1e3 / args.steps, kernel_ms=s_kavg,
+                                   pair_evals_per_s=pairs / ((t1 - t0) / args.steps),
+                                   kernel_tflops=OPS_SPEED_PAIR * unordered / (s_kavg * 1e-3) / 1e12)
+        # the reference's final matrix A = I + diag((1+d)/2) M (simulation.cpp:516-522)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.jacobian_dense_t(xy, p, M=M, scale=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        extra["scaled_matrix_pass"] = dict(ms_per_pass=(t1 - t0) * 1e3 / args.steps,
+                                           pair_evals_per_s=pairs / ((t1 - t0) / args.steps))
+    del M
+    torch.cuda.empty_cache()
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_pair_dense_n16384.json")
+    if os.path.exists(tpath) and n == 16384:
+        traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline (oracle, this host)")
+        c = cpu_baseline(2048, 4096, args.cpu_seconds)
+        cpu = {"value": c["omp"]["pair_evals_per_s"], "unit": "pair evals/s", "cores": c["omp"]["threads"],
+               "kind": "port",
+               "sample": (f"oracle dense Jacobian+speed pass (liboracle_fast.so, reference flags -Ofast "
+                          f"-ffast-math), N={c['omp']['n']} with OpenMP over rows (each pair evaluated per row), "
+                          f"median of {c['omp']['reps']} on {cpu_model()}; single-thread i<j loop as the "
+                          f"reference runs it: N={c['1core']['n']}, "
+                          f"{c['1core']['pair_evals_per_s']:.4g} pair evals/s"),
+               "single_thread_value": c["1core"]["pair_evals_per_s"]}
+
+    line = {
+        "metric": "FP64 pair evals/s (sigma + d sigma) at N=16384; dense Jacobian + speed pass",
+        "value": value,
+        "unit": "pair evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: uniform random positions in the unit periodic cell, sorted as SimulationData sorts",
+        "config": {"workload": f"Jacobian+speed pass, N={n} (half b=+1, half b=-1), unlimited cutoff, dense "
+                               f"M = V + diag(s) column-major, speeds at two stresses, weights d",
+                   "n": n, "pair_evals_per_step": pairs, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "pipe": "FP64 VALU (MI355X FP64 vector peak)", "achieved": achieved,
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": traffic, "kernel": "pair_pass_kernel<kModeJacDense>",
+                     "kernel_avg_ms": k_avg_ms, "ops_per_launch": OPS_JAC_PAIR * unordered},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

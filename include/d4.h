@@ -44,6 +44,12 @@ const char* d4_last_error(const d4_ctx* ctx);
 /* hipStream_t passed as void*; NULL restores the context's own stream */
 int d4_ctx_set_stream(d4_ctx* ctx, void* stream);
 int d4_ctx_synchronize(d4_ctx* ctx);
+/* Kernel timing (no reference counterpart): when on, HIP events recorded on
+ * the context's stream bracket every pair-kernel launch; d4_ctx_pair_kernel_time
+ * waits for them, returns the summed milliseconds and launch count since the
+ * last query, and clears the record. */
+int d4_ctx_set_profiling(d4_ctx* ctx, int on);
+int d4_ctx_pair_kernel_time(d4_ctx* ctx, double* total_ms, int* launches);
 /* device memory helpers for callers without a HIP toolchain (ctypes, tests) */
 int d4_dev_alloc(d4_ctx* ctx, int64_t bytes, void** out);
 int d4_dev_free(d4_ctx* ctx, void* p);

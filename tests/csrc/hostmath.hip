@@ -8,12 +8,12 @@ void hm_field(long n, const double* dx, const double* dy, const double* e, const
               int exact)
 {
     for (long k = 0; k < n; ++k) {
-        double E, C;
-        if (e && c) { E = e[k]; C = c[k]; }
-        else d4::field_inputs(dx[k], dy[k], E, C);
+        double E, R, C;
+        if (e && c) { E = e[k]; C = c[k]; R = 1.0 / E; }
+        else d4::field_inputs(dx[k], dy[k], E, R, C);
         double a = 0, b = 0;
-        if (exact) d4::pair_field<true, true>(dx[k], dy[k], E, C, a, b);
-        else d4::pair_field<true, false>(dx[k], dy[k], E, C, a, b);
+        if (exact) d4::pair_field<true, true>(dx[k], dy[k], E, R, C, a, b);
+        else d4::pair_field<true, false>(dx[k], dy[k], E, R, C, a, b);
         s[k] = a;
         ds[k] = b;
     }
@@ -21,10 +21,14 @@ void hm_field(long n, const double* dx, const double* dy, const double* e, const
 void hm_field_dy(long n, const double* dx, const double* dy, double* out)
 {
     for (long k = 0; k < n; ++k) {
-        double E, C;
-        d4::field_inputs(dx[k], dy[k], E, C);
-        out[k] = d4::pair_field_dy(dx[k], E, C, sin(d4::kTwoPi * dy[k]));
+        double E, R, C;
+        d4::field_inputs(dx[k], dy[k], E, R, C);
+        out[k] = d4::pair_field_dy(dx[k], E, R, C, sin(d4::kTwoPi * dy[k]));
     }
 }
 double hm_wrap(double v) { return d4::wrap(v); }
+void hm_inputs(long n, const double* dx, const double* dy, double* e, double* r, double* c)
+{
+    for (long k = 0; k < n; ++k) d4::field_inputs(dx[k], dy[k], e[k], r[k], c[k]);
+}
 }

@@ -81,3 +81,41 @@ def test_kernel_wrap_matches_reference_normalize(hm, golden):
     want = np.array([wrap(x) for x in v])
     assert np.array_equal(got, want)
     assert np.all((got >= -0.5) & (got < 0.5))
+
+
+def test_kernel_transcendentals_within_two_ulp(hm, orc):
+    """exp2_pair / cos2pi_reduced (d4_field.h) against the libm values the reference uses:
+    E within 2 ulp of exp2(PI2LN2INV dx), R the correctly rounded 1/E, and c within 2 ulp of
+    cos(2 pi dy) in the units that matter to the field (c enters only as C - c and C c - 1):
+    relative for |c| >= 1/2, absolute below (3 ulp(1/2): 1 - 2 sin^2 near c = 0)."""
+    rng = np.random.default_rng(24)
+    e_ = np.nextafter(0.5, 0)
+    dx = np.concatenate([rng.uniform(-0.5, 0.5, 200000), [-0.5, e_, 0.0, -1e-300, 0.25, -0.25]])
+    dy = np.concatenate([rng.uniform(-0.5, 0.5, 200000), [-0.5, e_, 0.0, 1e-300, 0.25, -0.25]])
+    n = dx.size
+    E, R, c = np.empty(n), np.empty(n), np.empty(n)
+    hm.hm_inputs.argtypes = [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    hm.hm_inputs(n, _p(dx), _p(dy), _p(E), _p(R), _p(c))
+    e0, c0 = orc.inputs(dx, dy)
+    assert np.all(np.abs(E - e0) <= 2 * np.spacing(e0))
+    assert np.array_equal(R, 1.0 / E)
+    assert np.all(np.abs(c - c0) <= 3 * np.spacing(np.maximum(np.abs(c0), 0.5)))
+    # the median error is at rounding level
+    assert np.median(np.abs(E - e0) / np.spacing(e0)) <= 0.5
+
+
+def test_kernel_own_inputs_within_bound_b(hm, orc):
+    """The kernels' own exp2/cos/1-over-E feeding their field stay inside bound (B) of helpers.py,
+    near pairs (where C0 - c cancels) included."""
+    from helpers import pair_allowance
+    rng = np.random.default_rng(31)
+    dx = np.concatenate([rng.uniform(-0.5, 0.5, 100000), rng.uniform(-2e-3, 2e-3, 4000)])
+    dy = np.concatenate([rng.uniform(-0.5, 0.5, 100000), rng.uniform(-2e-3, 2e-3, 4000)])
+    n = dx.size
+    s, d = np.empty(n), np.empty(n)
+    hm.hm_field(n, _p(dx), _p(dy), None, None, _p(s), _p(d), 0)
+    e, c = orc.inputs(dx, dy)
+    a, s0 = pair_allowance(orc, dx, dy, e, c)
+    ad, d0 = pair_allowance(orc, dx, dy, e, c, deriv=True)
+    assert np.all(np.abs(s - s0) <= a)
+    assert np.all(np.abs(d - d0) <= ad)
