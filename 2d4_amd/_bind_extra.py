@@ -1,8 +1,40 @@
 """ctypes signatures of the entry points declared outside include/d4.h
-(include/d4_batch.h, include/d4_sim.h); bound when those symbols exist."""
+(include/d4_sim.h and later headers); bound when those symbols exist."""
 import ctypes as C
 
-SIGNATURES = {}
+_P = C.POINTER
+_d = C.c_double
+_vp = C.c_void_p
+_pi = _P(C.c_int)
+
+
+class SimParams(C.Structure):
+    """d4_sim_params (include/d4_sim.h)."""
+    _fields_ = [("init_step_size", _d), ("max_step_size", _d), ("prec", _d), ("dipole_prec", _d),
+                ("cutoff_mult", _d), ("heaviside", C.c_int), ("wfunc", C.c_char), ("stress_kind", C.c_int),
+                ("stress_init", _d), ("stress_rate", _d), ("stress_period", _d), ("stress_amp", _d),
+                ("time_limit", _d), ("step_limit", C.c_int64), ("strain_limit", _d), ("speed_threshold", _d)]
+
+
+class SimStats(C.Structure):
+    """d4_sim_stats (include/d4_sim.h)."""
+    _fields_ = [("sim_time", _d), ("step_size", _d), ("strain", _d), ("last_max_error_ratio_sq", _d),
+                ("successful_steps", C.c_int64), ("failed_steps", C.c_int64), ("pair_passes", C.c_int64),
+                ("factorizations", C.c_int64), ("singular_solves", C.c_int64)]
+
+
+SIGNATURES = {
+    "d4_sim_default_params": (None, [_P(SimParams)]),
+    "d4_sim_create": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _P(SimParams), _P(_vp)]),
+    "d4_sim_create_from_file": (C.c_int, [_vp, C.c_char_p, _P(SimParams), _P(_vp)]),
+    "d4_sim_destroy": (None, [_vp]),
+    "d4_sim_last_error": (C.c_char_p, [_vp]),
+    "d4_sim_run": (C.c_int, [_vp]),
+    "d4_sim_step": (C.c_int, [_vp, _pi, _pi]),
+    "d4_sim_positions": (C.c_int, [_vp, _vp, C.c_int]),
+    "d4_sim_get_stats": (C.c_int, [_vp, _P(SimStats)]),
+    "d4_sim_write_dconf": (C.c_int, [_vp, C.c_char_p]),
+}
 
 
 def bind(L):

@@ -20,7 +20,7 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-mcode-obje
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wall", "-Wno-unused-command-line-argument"]
 
-SOURCES = ["d4_kernels.hip", "d4_capi.hip", "d4_batch.hip", "d4_sim.hip", "d4_solver.cpp"]
+SOURCES = ["d4_kernels.hip", "d4_capi.hip", "d4_batch.hip", "d4_sim.cpp", "d4_solver.cpp"]
 
 
 def _hipcc():
@@ -52,14 +52,28 @@ def build_lib(verbose=False):
 
 
 def build_test_helpers(verbose=False):
-    """Test-only helpers under tests/csrc (the kernel math compiled for the host)."""
-    src = os.path.join(ROOT, "tests", "csrc", "hostmath.hip")
+    """Test-only helpers under tests/csrc: the kernel math compiled for the host,
+    and the host sparse LU behind a C shim."""
+    tdir = os.path.join(ROOT, "tests", "csrc")
+    src = os.path.join(tdir, "hostmath.hip")
     if not os.path.exists(src):
         return None
-    out = os.path.join(ROOT, "tests", "csrc", "libhostmath.so")
+    out = os.path.join(tdir, "libhostmath.so")
     cmd = [_hipcc(), "-O2", "-std=c++17", "-fPIC", "-shared", f"--offload-arch={ARCH}",
            "-Wno-unused-command-line-argument", "-o", out, src]
     _run(cmd, verbose)
+    shim = os.path.join(tdir, "solver_shim.cpp")
+    if os.path.exists(shim):
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", os.path.join(tdir, "libsolver_shim.so"),
+              shim, os.path.join(CSRC, "d4_solver.cpp")], verbose)
+    sim_shim = os.path.join(tdir, "sim_cpu_shim.cpp")
+    if os.path.exists(sim_shim) and os.path.exists(os.path.join(CSRC, "d4_sim.cpp")):
+        orc_src = os.path.join(ROOT, "oracle", "d4_oracle.c")
+        orc_obj = os.path.join(tdir, "d4_oracle_sim.o")
+        _run(["gcc", "-O2", "-fPIC", "-std=c11", "-c", "-o", orc_obj, orc_src], verbose)
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", os.path.join(tdir, "libsim_cpu.so"),
+              sim_shim, os.path.join(CSRC, "d4_sim.cpp"), os.path.join(CSRC, "d4_solver.cpp"), orc_obj, "-lm"],
+             verbose)
     return out
 
 

@@ -1,0 +1,525 @@
+// d4_sim.cpp -- host simulation (d4_host.h) and its C ABI (include/d4_sim.h).
+#include "../../include/d4_sim.h"
+#include "d4_host.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <limits>
+#include <numeric>
+
+namespace sdddstCore {
+
+// ------------------------------------------------------------------ Field
+
+static double field_one(d4_ctx* ctx, double dx, double dy, const double* e, const double* c, bool deriv)
+{
+    double s = NAN, d = NAN;
+    if (!ctx) return NAN;
+    d4_field_eval(ctx, 1, &dx, &dy, e, c, deriv ? nullptr : &s, deriv ? &d : nullptr, 0);
+    return deriv ? d : s;
+}
+double Field::xy(double dx, double dy) const { return field_one(ctx_, dx, dy, nullptr, nullptr, false); }
+double Field::xy_diff_x(double dx, double dy) const { return field_one(ctx_, dx, dy, nullptr, nullptr, true); }
+double Field::xy(double dx, double dy, double e, double c) const { return field_one(ctx_, dx, dy, &e, &c, false); }
+double Field::xy_diff_x(double dx, double dy, double e, double c) const
+{
+    return field_one(ctx_, dx, dy, &e, &c, true);
+}
+
+// ------------------------------------------------------------------ stress
+
+double CyclicLoadProtocol::extStress(double t) const  // stress_protocol.cpp:47-56
+{
+    const double pt = t - std::floor((t / m_timePeriod + 1. / 4)) * m_timePeriod;
+    double s = m_rate * pt + initExtStress;
+    if (pt > m_timePeriod / 4) s = m_rate * m_timePeriod / 2 - s + initExtStress;
+    return s;
+}
+
+// ------------------------------------------------------------------ precision
+
+void PrecisionHandler::reset()  // precision_handler.cpp:26-31
+{
+    maxErrorToleranceRatioSq = 0;
+    if (use_dipoleCrit) tolerance.assign(tolerance.size(), minPrecisitySq);
+}
+
+void PrecisionHandler::updateMaxErrorToleranceRatioSq(double error, unsigned ID)  // precision_handler.cpp:41-54
+{
+    const double tmp = use_dipoleCrit ? error * error / tolerance[ID] : error * error / minPrecisitySq;
+    if (tmp > maxErrorToleranceRatioSq) {
+        maxErrorToleranceRatioSq = tmp;
+        selectedID = ID;
+    }
+}
+
+double PrecisionHandler::getNewStepSize(double oldStepSize) const  // precision_handler.cpp:57-62
+{
+    const double factor = std::min(2., std::pow(getMaxErrorRatioSqr(), -1. / 6));
+    return 0.9 * oldStepSize * factor;
+}
+
+void PrecisionHandler::setDipolePrecisity(double v)  // precision_handler.cpp:72-89
+{
+    if (v == 0) {
+        use_dipoleCrit = false;
+        dipolePrecisity = 1;
+        dipolePrecisitySq = 1;
+    } else {
+        use_dipoleCrit = true;
+        dipolePrecisity = v;
+        minPrecisity /= v;
+        dipolePrecisitySq = v * v;
+        minPrecisitySq /= v * v;
+    }
+}
+
+// ------------------------------------------------------------------ data
+
+static void normalize(double& v)  // sim_utils.cpp:6-13
+{
+    while (v < -0.5) v += 1;
+    while (v >= 0.5) v -= 1;
+}
+
+SimulationData::SimulationData(const std::string& path)
+{
+    std::ifstream in(path);
+    if (!in) {
+        error = "cannot open dislocation data file " + path;
+        return;
+    }
+    std::vector<double> xs, ys, bs;
+    for (double x; in >> x;) {
+        double y, b;
+        if (!(in >> y && in >> b)) {
+            error = "cannot read the y coordinate and Burgers vector of a dislocation in " + path;
+            return;
+        }
+        xs.push_back(x);
+        ys.push_back(y);
+        bs.push_back(b);
+    }
+    build(xs, ys, bs);
+}
+
+SimulationData::SimulationData(int n, const double* x, const double* y, const double* b)
+{
+    if (n < 0 || (n > 0 && (!x || !y || !b))) {
+        error = "invalid arrays";
+        return;
+    }
+    build(std::vector<double>(x, x + n), std::vector<double>(y, y + n), std::vector<double>(b, b + n));
+}
+
+// simulation_data.cpp:98-154: integer b, sum b == 0, sort by y + b descending
+void SimulationData::build(std::vector<double> xs, std::vector<double> ys, std::vector<double> bs)
+{
+    const size_t n = xs.size();
+    double sum_b = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (std::fabs(bs[i] - std::rint(bs[i])) > 1e-5) {
+            error = "Burgers vector supposed to be an integer";
+            return;
+        }
+        sum_b += bs[i];
+    }
+    if (sum_b != 0) {
+        error = "the sum of the Burgers vectors supposed to be 0";
+        return;
+    }
+    std::vector<unsigned> idx(n);
+    std::iota(idx.begin(), idx.end(), 0u);
+    std::stable_sort(idx.begin(), idx.end(), [&](unsigned l, unsigned r) { return ys[l] + bs[l] > ys[r] + bs[r]; });
+    dc = (unsigned)n;
+    disl_order.resize(n);
+    disl_sorted.clear();
+    for (size_t i = 0; i < n; ++i) {
+        disl_sorted.emplace_back(xs[idx[i]], ys[idx[i]]);
+        disl_order[idx[i]] = (unsigned)i;
+    }
+    g.resize(n);
+    initSpeed.resize(n);
+    initSpeed2.resize(n);
+    speed.resize(n);
+    speed2.resize(n);
+    dVec.resize(n);
+    bigStep_sorted.resize(n);
+    firstSmall_sorted.resize(n);
+    secondSmall_sorted.resize(n);
+    Ap.assign(n + 1, 0);
+    indexes.resize(n);
+    x.resize(n);
+    updateCutOff();
+}
+
+void SimulationData::updateCutOff()
+{
+    double m = cutOffMultiplier;
+    if (!heavisideCutoff && cutOffMultiplier >= std::sqrt(2.0 * dc) / 12) m = 1e20;
+    cutOff = m / std::sqrt((double)dc);
+    cutOffSqr = cutOff * cutOff;
+    onePerCutOffSqr = 1 / cutOffSqr;
+}
+
+void SimulationData::writeDislocationDataToFile(const std::string& path) const
+{
+    std::ofstream of(path);
+    if (!of) return;
+    of << std::setprecision(14);
+    for (auto id : disl_order) of << disl_sorted[id].x << "\t" << disl_sorted[id].y << "\t" << b(id) << "\n";
+}
+
+// ------------------------------------------------------------------ simulation
+
+Simulation::Simulation(std::shared_ptr<SimulationData> s, d4_ctx* ctx) : sD(std::move(s)), pH(new PrecisionHandler), ctx_(ctx)
+{
+    pH->setMinPrecisity(sD->prec);  // simulation.cpp:24-26
+    pH->setDipolePrecisity(sD->dipole_prec);
+    pH->setSize(sD->dc);
+    sD->tau.bind(ctx);
+}
+
+bool Simulation::check(int rc, const char* where)
+{
+    if (rc == D4_OK) return true;
+    if (!status_) {
+        status_ = rc;
+        error_ = std::string(where) + ": " + d4_last_error(ctx_);
+    }
+    return false;
+}
+
+void Simulation::calculateSpeedsAtTime(const std::vector<DislwoB>& dis, std::vector<double>& forces, double simTime)
+{
+    const double tau = sD->externalStressProtocol->extStress(simTime);
+    pairPasses++;
+    check(d4_speed_pass(ctx_, (int)sD->dc, &dis[0].x, tau, forces.data(), pH->toleranceData(), pH->toleranceCap2()),
+          "speed pass");
+}
+
+int Simulation::calcJacobianAndSpeedsAtTimes(double stepsize, const std::vector<DislwoB>& dislocs,
+                                             std::vector<double>& forces_A, std::vector<double>& forces_B,
+                                             double simTime_A, double simTime_B)
+{
+    d4_jac_params p;
+    p.h = stepsize;
+    p.tau_a = sD->externalStressProtocol->extStress(simTime_A);
+    p.tau_b = sD->externalStressProtocol->extStress(simTime_B);
+    p.cutoff = sD->cutOff;
+    p.cut_sq = sD->cutOffSqr;
+    p.one_per_cut_sq = sD->onePerCutOffSqr;
+    p.heaviside = sD->heavisideCutoff ? 1 : 0;
+    p.wfunc = sD->weightFunc;
+    p.tol_cap2 = pH->toleranceCap2();
+    const int n = (int)sD->dc;
+    int nnz = 0;
+    if (sD->Ai.empty()) {
+        sD->Ai.resize((size_t)n * 16 + 16);
+        sD->Ax.resize(sD->Ai.size());
+    }
+    pairPasses++;
+    int rc = d4_jacobian_pass(ctx_, n, &dislocs[0].x, &p, forces_A.data(), forces_B.data(), sD->Ap.data(),
+                              sD->Ai.data(), sD->Ax.data(), (int)sD->Ai.size(), &nnz, sD->indexes.data(),
+                              sD->dVec.data(), pH->toleranceData());
+    if (rc == D4_ERR_CAPACITY) {  // grow (simulation_data.cpp:157-175 grows by dc) and run the pass again
+        sD->Ai.resize((size_t)nnz + n);
+        sD->Ax.resize(sD->Ai.size());
+        rc = d4_jacobian_pass(ctx_, n, &dislocs[0].x, &p, forces_A.data(), forces_B.data(), sD->Ap.data(),
+                              sD->Ai.data(), sD->Ax.data(), (int)sD->Ai.size(), &nnz, sD->indexes.data(),
+                              sD->dVec.data(), pH->toleranceData());
+    }
+    check(rc, "jacobian pass");
+    calculateSparseFormForJacobian();
+    return nnz;
+}
+
+void Simulation::calcJacobianAndSpeedsFromPrev(double baseTime, double newTime, double oldTime)
+{
+    const double factor = (newTime - baseTime) / (oldTime - baseTime);
+    const double dtau = sD->externalStressProtocol->extStress(newTime) - sD->externalStressProtocol->extStress(oldTime);
+    check(d4_jacobian_rescale(ctx_, (int)sD->dc, sD->Ap.data(), sD->Ai.data(), sD->Ax.data(), sD->dVec.data(),
+                              sD->weightFunc, factor, dtau, sD->speed.data()),
+          "jacobian rescale");
+    calculateSparseFormForJacobian();
+}
+
+void Simulation::calculateSparseFormForJacobian()
+{
+    factorizations++;
+    lu_.factor((int)sD->dc, sD->Ap.data(), sD->Ai.data(), sD->Ax.data());
+}
+
+void Simulation::solveEQSys()  // simulation.cpp:643-659
+{
+    lu_.solve(sD->g.data(), sD->x.data());
+    if (lu_.singular()) {
+        singularSolves++;
+        for (unsigned i = 0; i < sD->dc; ++i)
+            if (!((-1 < sD->x[i]) && (sD->x[i] < 1))) sD->x[i] = 0;
+    }
+}
+
+void Simulation::calcGSolveAndUpdate(std::vector<DislwoB>& new_disloc, const std::vector<DislwoB>& old_config,
+                                     double stepSize, const std::vector<double>& endSpeed,
+                                     const std::vector<double>& initSpeed)
+{
+    for (unsigned i = 0; i < sD->dc; i++)  // simulation.cpp:620-624
+        sD->g[i] = new_disloc[i].x - old_config[i].x -
+                   stepSize * ((1 + sD->dVec[i]) * endSpeed[i] + (1 - sD->dVec[i]) * initSpeed[i]) / 2;
+    solveEQSys();
+    for (unsigned i = 0; i < sD->dc; i++) new_disloc[i].x -= sD->x[i];
+}
+
+void Simulation::calculateXError()  // simulation.cpp:692-700
+{
+    for (unsigned i = 0; i < sD->dc; i++)
+        pH->updateMaxErrorToleranceRatioSq(std::fabs(sD->bigStep_sorted[i].x - sD->secondSmall_sorted[i].x), i);
+}
+
+double Simulation::calculateStrainIncrement(const std::vector<DislwoB>& old, const std::vector<DislwoB>& newD) const
+{
+    double ret = 0;  // simulation.cpp:713-720
+    for (size_t i = 0; i < old.size(); i++) ret += sD->b((unsigned)i) * (newD[i].x - old[i].x);
+    return ret;
+}
+
+bool Simulation::interrupt() const  // simulation.cpp:722-750 (time, strain and step-count limits)
+{
+    if (sD->isTimeLimit && sD->simTime > sD->timeLimit) return true;
+    if (sD->isStrainIncreaseLimit && sD->totalAccumulatedStrainIncrease > sD->totalAccumulatedStrainIncreaseLimit)
+        return true;
+    if (sD->isStepCountLimit && (int64_t)sD->succesfulSteps > sD->stepCountLimit) return true;
+    return false;
+}
+
+bool Simulation::step()  // one iteration of simulation.cpp:88-277
+{
+    SimulationData& d = *sD;
+    const unsigned n = d.dc;
+    const double t0 = d.simTime, t1 = d.simTime + d.stepSize, th = d.simTime + d.stepSize / 2;
+    // stage I
+    calcJacobianAndSpeedsAtTimes(d.stepSize, d.disl_sorted, d.initSpeed, d.speed, t0, t1);
+    for (unsigned i = 0; i < n; i++)
+        d.g[i] = -d.stepSize * ((1 + d.dVec[i]) * d.speed[i] + (1 - d.dVec[i]) * d.initSpeed[i]) / 2;
+    solveEQSys();
+    for (unsigned i = 0; i < n; i++) d.bigStep_sorted[i] = DislwoB(d.disl_sorted[i].x - d.x[i], d.disl_sorted[i].y);
+    calculateSpeedsAtTime(d.bigStep_sorted, d.speed2, t1);
+    calcGSolveAndUpdate(d.bigStep_sorted, d.disl_sorted, d.stepSize, d.speed2, d.initSpeed);
+    // stage II
+    calcJacobianAndSpeedsFromPrev(t0, th, t1);
+    for (unsigned i = 0; i < n; i++)
+        d.g[i] = -d.stepSize / 2 * ((1 + d.dVec[i]) * d.speed[i] + (1 - d.dVec[i]) * d.initSpeed[i]) / 2;
+    solveEQSys();
+    for (unsigned i = 0; i < n; i++)
+        d.firstSmall_sorted[i] = DislwoB(d.disl_sorted[i].x - d.x[i], d.disl_sorted[i].y);
+    calculateSpeedsAtTime(d.firstSmall_sorted, d.speed2, th);
+    calcGSolveAndUpdate(d.firstSmall_sorted, d.disl_sorted, d.stepSize / 2, d.speed2, d.initSpeed);
+    // stage III
+    calcJacobianAndSpeedsAtTimes(d.stepSize / 2, d.firstSmall_sorted, d.initSpeed2, d.speed2, th, t1);
+    for (unsigned i = 0; i < n; i++)
+        d.g[i] = -d.stepSize / 2 * ((1 + d.dVec[i]) * d.speed2[i] + (1 - d.dVec[i]) * d.initSpeed2[i]) / 2;
+    solveEQSys();
+    for (unsigned i = 0; i < n; i++)
+        d.secondSmall_sorted[i] = DislwoB(d.firstSmall_sorted[i].x - d.x[i], d.firstSmall_sorted[i].y);
+    calculateSpeedsAtTime(d.secondSmall_sorted, d.speed2, t1);
+    calcGSolveAndUpdate(d.secondSmall_sorted, d.firstSmall_sorted, d.stepSize / 2, d.speed2, d.initSpeed2);
+    // stage IV
+    calculateXError();
+    const bool accept = pH->getMaxErrorRatioSqr() < 1;
+    if (accept) {
+        d.simTime += d.stepSize;
+        d.succesfulSteps++;
+        if (d.calculateStrainDuringSimulation) {
+            d.totalAccumulatedStrainIncrease += calculateStrainIncrement(d.disl_sorted, d.firstSmall_sorted);
+            d.totalAccumulatedStrainIncrease += calculateStrainIncrement(d.firstSmall_sorted, d.secondSmall_sorted);
+        }
+        d.disl_sorted.swap(d.secondSmall_sorted);
+        for (auto& p : d.disl_sorted) normalize(p.x);
+        if (d.isSpeedThresholdForCutoffChange) {  // simulation.cpp:177, 215-219
+            double avg = 0;
+            for (double v : d.initSpeed2) avg += std::fabs(v);
+            avg /= n;
+            if (d.speedThresholdForCutoffChange > avg) {
+                d.cutOffMultiplier = 1e20;
+                d.updateCutOff();
+            }
+        }
+    } else {
+        d.failedSteps++;
+    }
+    lastRatio_ = pH->getMaxErrorRatioSqr();
+    d.stepSize = pH->getNewStepSize(d.stepSize);  // simulation.cpp:253
+    pH->reset();                                   // simulation.cpp:271 (no write-out times here)
+    if (d.isMaxStepSizeLimit && d.maxStepSizeLimit < d.stepSize) d.stepSize = d.maxStepSizeLimit;
+    return accept;
+}
+
+void Simulation::run_prologue()
+{
+    calculateSpeedsAtTime(sD->disl_sorted, sD->initSpeed, sD->simTime);  // simulation.cpp:46
+}
+
+void Simulation::run()
+{
+    run_prologue();
+    while (!status_ && !interrupt()) step();
+}
+
+}  // namespace sdddstCore
+
+// ------------------------------------------------------------------ C ABI
+
+struct d4_sim {
+    std::shared_ptr<sdddstCore::SimulationData> data;
+    std::unique_ptr<sdddstCore::Simulation> sim;
+    bool started = false;
+    double last_ratio = 0;
+    std::string err;
+};
+
+static int sim_setup(d4_ctx* ctx, std::shared_ptr<sdddstCore::SimulationData> sd, const d4_sim_params* p, d4_sim** out)
+{
+    if (!sd->error.empty()) return D4_ERR_INVALID;
+    sdddstCore::SimulationData& d = *sd;
+    d.initStepSize = d.stepSize = p->init_step_size;
+    d.isMaxStepSizeLimit = p->max_step_size > 0;
+    d.maxStepSizeLimit = p->max_step_size;
+    d.prec = p->prec;
+    d.dipole_prec = p->dipole_prec;
+    d.cutOffMultiplier = p->cutoff_mult;
+    d.heavisideCutoff = p->heaviside != 0;
+    d.weightFunc = p->wfunc;
+    d.isTimeLimit = p->time_limit > 0;
+    d.timeLimit = p->time_limit;
+    d.isStepCountLimit = p->step_limit >= 0;
+    d.stepCountLimit = p->step_limit;
+    d.isStrainIncreaseLimit = p->strain_limit > 0;
+    d.totalAccumulatedStrainIncreaseLimit = p->strain_limit;
+    d.isSpeedThresholdForCutoffChange = p->speed_threshold > 0;
+    d.speedThresholdForCutoffChange = p->speed_threshold;
+    switch (p->stress_kind) {  // project_parser.cpp:287-326
+    case 1: d.externalStressProtocol.reset(new sdddstCore::FixedRateProtocol(p->stress_init, p->stress_rate)); break;
+    case 2:
+        d.externalStressProtocol.reset(
+            new sdddstCore::CyclicLoadProtocol(p->stress_init, p->stress_rate, p->stress_period));
+        break;
+    case 3:
+        d.externalStressProtocol.reset(new sdddstCore::IncreasingCyclicLoadProtocol(
+            p->stress_init, p->stress_rate, p->stress_period, p->stress_amp));
+        break;
+    default: d.externalStressProtocol.reset(new sdddstCore::StressProtocol(p->stress_init)); break;
+    }
+    d.updateCutOff();
+    d4_sim* s = new d4_sim();
+    s->data = sd;
+    s->sim.reset(new sdddstCore::Simulation(sd, ctx));
+    *out = s;
+    return D4_OK;
+}
+
+extern "C" {
+
+void d4_sim_default_params(d4_sim_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->init_step_size = 1e-6;
+    p->prec = 1e-5;
+    p->dipole_prec = 0.05;
+    p->cutoff_mult = 1e20;
+    p->wfunc = 'c';
+    p->step_limit = -1;
+}
+
+int d4_sim_create(d4_ctx* ctx, int n, const double* x, const double* y, const double* b, const d4_sim_params* p,
+                  d4_sim** out)
+{
+    if (!ctx || !p || !out || n <= 0) return D4_ERR_INVALID;
+    *out = nullptr;
+    return sim_setup(ctx, std::make_shared<sdddstCore::SimulationData>(n, x, y, b), p, out);
+}
+
+int d4_sim_create_from_file(d4_ctx* ctx, const char* path, const d4_sim_params* p, d4_sim** out)
+{
+    if (!ctx || !p || !out || !path) return D4_ERR_INVALID;
+    *out = nullptr;
+    return sim_setup(ctx, std::make_shared<sdddstCore::SimulationData>(std::string(path)), p, out);
+}
+
+void d4_sim_destroy(d4_sim* s) { delete s; }
+
+const char* d4_sim_last_error(const d4_sim* s)
+{
+    if (!s) return "null simulation";
+    return s->sim->error().c_str();
+}
+
+int d4_sim_step(d4_sim* s, int* accepted, int* stop)
+{
+    if (!s) return D4_ERR_INVALID;
+    sdddstCore::Simulation& sim = *s->sim;
+    if (!s->started) {
+        s->started = true;
+        sim.run_prologue();
+    }
+    if (accepted) *accepted = 0;
+    if (sim.interrupt()) {
+        if (stop) *stop = 1;
+        return sim.status();
+    }
+    if (stop) *stop = 0;
+    const bool acc = sim.step();
+    if (accepted) *accepted = acc ? 1 : 0;
+    return sim.status();
+}
+
+int d4_sim_run(d4_sim* s)
+{
+    if (!s) return D4_ERR_INVALID;
+    int stop = 0, acc = 0;
+    while (true) {
+        int rc = d4_sim_step(s, &acc, &stop);
+        if (rc || stop) return rc;
+    }
+}
+
+int d4_sim_positions(const d4_sim* s, double* xy, int original_order)
+{
+    if (!s || !xy) return D4_ERR_INVALID;
+    const auto& d = *s->data;
+    for (unsigned i = 0; i < d.dc; ++i) {
+        const unsigned k = original_order ? d.disl_order[i] : i;
+        xy[2 * i] = d.disl_sorted[k].x;
+        xy[2 * i + 1] = d.disl_sorted[k].y;
+    }
+    return D4_OK;
+}
+
+int d4_sim_get_stats(const d4_sim* s, d4_sim_stats* st)
+{
+    if (!s || !st) return D4_ERR_INVALID;
+    const auto& d = *s->data;
+    st->sim_time = d.simTime;
+    st->step_size = d.stepSize;
+    st->strain = d.totalAccumulatedStrainIncrease;
+    st->last_max_error_ratio_sq = s->sim->lastRatio();
+    st->successful_steps = (int64_t)d.succesfulSteps;
+    st->failed_steps = (int64_t)d.failedSteps;
+    st->pair_passes = s->sim->pairPasses;
+    st->factorizations = s->sim->factorizations;
+    st->singular_solves = s->sim->singularSolves;
+    return D4_OK;
+}
+
+int d4_sim_write_dconf(const d4_sim* s, const char* path)
+{
+    if (!s || !path) return D4_ERR_INVALID;
+    s->data->writeDislocationDataToFile(path);
+    return D4_OK;
+}
+
+}  // extern "C"

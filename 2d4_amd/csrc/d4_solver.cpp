@@ -1,0 +1,261 @@
+// d4_solver.cpp -- host sparse LU (see d4_solver.h).
+#include "d4_solver.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+namespace d4host {
+
+std::vector<int> min_degree_order(int n, const int* Ap, const int* Ai)
+{
+    // elimination graph as bit-set rows; degree = live neighbours
+    const int W = (n + 63) / 64;
+    std::vector<uint64_t> adj((size_t)n * W, 0ull), alive(W, 0ull), nb(W);
+    auto setb = [&](int r, int c) { adj[(size_t)r * W + (c >> 6)] |= 1ull << (c & 63); };
+    for (int c = 0; c < n; ++c)
+        for (int k = Ap[c]; k < Ap[c + 1]; ++k) {
+            const int r = Ai[k];
+            if (r == c) continue;
+            setb(r, c);
+            setb(c, r);
+        }
+    for (int v = 0; v < n; ++v) alive[v >> 6] |= 1ull << (v & 63);
+    std::vector<int> deg(n);
+    for (int v = 0; v < n; ++v) {
+        int d = 0;
+        for (int w = 0; w < W; ++w) d += __builtin_popcountll(adj[(size_t)v * W + w]);
+        deg[v] = d;
+    }
+    std::vector<char> done(n, 0);
+    std::vector<int> order;
+    order.reserve(n);
+    for (int step = 0; step < n; ++step) {
+        int v = -1;
+        for (int u = 0; u < n; ++u)
+            if (!done[u] && (v < 0 || deg[u] < deg[v])) v = u;
+        order.push_back(v);
+        done[v] = 1;
+        alive[v >> 6] &= ~(1ull << (v & 63));
+        const uint64_t* av = &adj[(size_t)v * W];
+        for (int w = 0; w < W; ++w) nb[w] = av[w] & alive[w];
+        for (int w = 0; w < W; ++w) {
+            uint64_t bits = nb[w];
+            while (bits) {
+                const int u = w * 64 + __builtin_ctzll(bits);
+                bits &= bits - 1;
+                uint64_t* au = &adj[(size_t)u * W];
+                int d = 0;
+                for (int x = 0; x < W; ++x) {
+                    au[x] = (au[x] | nb[x]) & alive[x];
+                    d += 0;
+                }
+                au[u >> 6] &= ~(1ull << (u & 63));
+                for (int x = 0; x < W; ++x) d += __builtin_popcountll(au[x]);
+                deg[u] = d;
+            }
+        }
+    }
+    return order;
+}
+
+int SparseLU::factor_dense(int n, const int* Ap, const int* Ai, const double* Ax)
+{
+    dense_ = true;
+    D_.assign((size_t)n * n, 0.0);
+    dpiv_.assign(n, 0);
+    for (int c = 0; c < n; ++c)
+        for (int k = Ap[c]; k < Ap[c + 1]; ++k) D_[(size_t)c * n + Ai[k]] = Ax[k];
+    double* a = D_.data();
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double best = std::fabs(a[(size_t)k * n + k]);
+        for (int r = k + 1; r < n; ++r) {
+            const double v = std::fabs(a[(size_t)k * n + r]);
+            if (v > best) {
+                best = v;
+                p = r;
+            }
+        }
+        dpiv_[k] = p;
+        if (p != k)
+            for (int c = 0; c < n; ++c) std::swap(a[(size_t)c * n + k], a[(size_t)c * n + p]);
+        const double d = a[(size_t)k * n + k];
+        if (d == 0.0) {
+            singular_ = true;
+            continue;
+        }
+        const double inv = 1.0 / d;
+        for (int r = k + 1; r < n; ++r) a[(size_t)k * n + r] *= inv;
+        for (int c = k + 1; c < n; ++c) {
+            double* col = a + (size_t)c * n;
+            const double m = col[k];
+            if (m == 0.0) continue;
+            const double* lk = a + (size_t)k * n;
+            for (int r = k + 1; r < n; ++r) col[r] -= lk[r] * m;
+        }
+    }
+    return singular_ ? 1 : 0;
+}
+
+int SparseLU::factor(int n, const int* Ap, const int* Ai, const double* Ax)
+{
+    if (n < 0 || !Ap || (n > 0 && (!Ai || !Ax))) return -1;
+    n_ = n;
+    singular_ = false;
+    dense_ = false;
+    const int64_t nnz = Ap[n];
+    if (n <= 96 || nnz > (int64_t)n * n / 4) return factor_dense(n, Ap, Ai, Ax);
+
+    q_ = min_degree_order(n, Ap, Ai);
+    pinv_.assign(n, -1);
+    prow_.assign(n, -1);
+    Lp_.assign(n + 1, 0);
+    Up_.assign(n + 1, 0);
+    Li_.clear();
+    Lx_.clear();
+    Ui_.clear();
+    Ux_.clear();
+    Li_.reserve((size_t)nnz * 4);
+    Lx_.reserve((size_t)nnz * 4);
+    Ui_.reserve((size_t)nnz * 4);
+    Ux_.reserve((size_t)nnz * 4);
+
+    std::vector<double> x(n, 0.0);
+    std::vector<int> stack(n), pstack(n), xi(n);
+    std::vector<char> mark(n, 0);
+    for (int k = 0; k < n; ++k) {
+        const int col = q_[k];
+        // reach of the column's pattern in the graph of L (pivot-indexed), topological order
+        int top = n;
+        for (int p = Ap[col]; p < Ap[col + 1]; ++p) {
+            const int r0 = Ai[p];
+            if (mark[r0]) continue;
+            int head = 0;
+            stack[0] = r0;
+            while (head >= 0) {
+                const int j = stack[head];
+                const int jc = pinv_[j];  // L column of row j, if j is pivotal
+                if (!mark[j]) {
+                    mark[j] = 1;
+                    pstack[head] = jc < 0 ? 0 : Lp_[jc];
+                }
+                bool pushed = false;
+                if (jc >= 0) {
+                    const int end = Lp_[jc + 1];
+                    for (int t = pstack[head]; t < end; ++t) {
+                        const int i = Li_[t];
+                        if (mark[i]) continue;
+                        pstack[head] = t + 1;
+                        stack[++head] = i;
+                        pushed = true;
+                        break;
+                    }
+                }
+                if (!pushed) {
+                    head--;
+                    xi[--top] = j;
+                }
+            }
+        }
+        // x = L \ A(:, col) over the reach
+        for (int p = Ap[col]; p < Ap[col + 1]; ++p) x[Ai[p]] = Ax[p];
+        for (int t = top; t < n; ++t) {
+            const int j = xi[t];
+            const int jc = pinv_[j];
+            if (jc < 0) continue;
+            const double xj = x[j];
+            for (int s = Lp_[jc]; s < Lp_[jc + 1]; ++s) x[Li_[s]] -= Lx_[s] * xj;
+        }
+        // pivot: diagonal preferred (row col), else largest
+        int ipiv = -1;
+        double amax = -1.0;
+        for (int t = top; t < n; ++t) {
+            const int i = xi[t];
+            if (pinv_[i] < 0) {
+                const double v = std::fabs(x[i]);
+                if (v > amax) {
+                    amax = v;
+                    ipiv = i;
+                }
+            } else {
+                Ui_.push_back(pinv_[i]);
+                Ux_.push_back(x[i]);
+            }
+        }
+        if (ipiv < 0) {  // structurally singular column
+            singular_ = true;
+            for (int i = 0; i < n; ++i)
+                if (pinv_[i] < 0) {
+                    ipiv = i;
+                    break;
+                }
+            amax = 0.0;
+            x[ipiv] = 0.0;
+        }
+        if (pinv_[col] < 0 && mark[col] && std::fabs(x[col]) >= 0.1 * amax) ipiv = col;
+        const double piv = x[ipiv];
+        if (piv == 0.0) singular_ = true;
+        Ui_.push_back(k);
+        Ux_.push_back(piv);
+        Up_[k + 1] = (int)Ui_.size();
+        pinv_[ipiv] = k;
+        prow_[k] = ipiv;
+        const double inv = piv != 0.0 ? 1.0 / piv : 0.0;
+        for (int t = top; t < n; ++t) {
+            const int i = xi[t];
+            if (pinv_[i] < 0) {
+                Li_.push_back(i);
+                Lx_.push_back(x[i] * inv);
+            }
+            x[i] = 0.0;
+            mark[i] = 0;
+        }
+        if (!mark[ipiv]) x[ipiv] = 0.0;
+        Lp_[k + 1] = (int)Li_.size();
+    }
+    // L row indices from original rows to pivot order
+    for (auto& i : Li_) i = pinv_[i];
+    return singular_ ? 1 : 0;
+}
+
+void SparseLU::solve(const double* b, double* xout) const
+{
+    const int n = n_;
+    if (dense_) {
+        std::vector<double> c(b, b + n);
+        for (int k = 0; k < n; ++k) std::swap(c[k], c[dpiv_[k]]);
+        const double* a = D_.data();
+        for (int k = 0; k < n; ++k) {
+            const double v = c[k];
+            if (v != 0.0)
+                for (int r = k + 1; r < n; ++r) c[r] -= a[(size_t)k * n + r] * v;
+        }
+        for (int k = n - 1; k >= 0; --k) {
+            c[k] /= a[(size_t)k * n + k];
+            const double v = c[k];
+            if (v != 0.0)
+                for (int r = 0; r < k; ++r) c[r] -= a[(size_t)k * n + r] * v;
+        }
+        std::memcpy(xout, c.data(), sizeof(double) * (size_t)n);
+        return;
+    }
+    std::vector<double> c(n);
+    for (int i = 0; i < n; ++i) c[pinv_[i]] = b[i];
+    for (int k = 0; k < n; ++k) {
+        const double v = c[k];
+        if (v != 0.0)
+            for (int s = Lp_[k]; s < Lp_[k + 1]; ++s) c[Li_[s]] -= Lx_[s] * v;
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        const int last = Up_[k + 1] - 1;  // diagonal stored last
+        c[k] /= Ux_[last];
+        const double v = c[k];
+        if (v != 0.0)
+            for (int s = Up_[k]; s < last; ++s) c[Ui_[s]] -= Ux_[s] * v;
+    }
+    for (int k = 0; k < n; ++k) xout[q_[k]] = c[k];
+}
+
+}  // namespace d4host

@@ -1,0 +1,73 @@
+/*
+ * d4_sim.h -- C ABI of the host simulation around lib2d4's pair passes: the
+ * reference's SimulationData + Simulation::run (simulation_data.cpp,
+ * simulation.cpp:30-281) with every O(N^2) pass on the GPU through include/d4.h
+ * and the linear solves on the host (a sparse LU in place of UMFPACK,
+ * simulation.cpp:635-659).
+ *
+ * Parameters mirror the reference's command-line flags (project_parser.cpp:28-68)
+ * as SimulationData stores them (simulation_data.h:126-328).
+ */
+#ifndef D4_SIM_H
+#define D4_SIM_H
+
+#include <stdint.h>
+
+#include "d4.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct d4_sim d4_sim;
+
+typedef struct {
+    double init_step_size;   /* --initial-stepsize (default 1e-6) */
+    double max_step_size;    /* -M; <= 0: no limit */
+    double prec;             /* -P position precision (default 1e-5) */
+    double dipole_prec;      /* -p dipole precision (default 0.05; 0 disables) */
+    double cutoff_mult;      /* -u cutoff multiplier (default 1e20) */
+    int heaviside;           /* -h */
+    char wfunc;              /* -w weight function 'c' | 'p' | 'm' */
+    int stress_kind;         /* 0 StressProtocol (always 0), 1 FixedRate, 2 Cyclic, 3 IncreasingCyclic */
+    double stress_init, stress_rate, stress_period, stress_amp;
+    double time_limit;       /* -t; <= 0: none */
+    int64_t step_limit;      /* -C; < 0: none (stop once successful steps exceed it) */
+    double strain_limit;     /* <= 0: none */
+    double speed_threshold;  /* cutoff change threshold (simulation.cpp:215-219); <= 0: none */
+} d4_sim_params;
+
+typedef struct {
+    double sim_time, step_size, strain, last_max_error_ratio_sq;
+    int64_t successful_steps, failed_steps;
+    int64_t pair_passes, factorizations, singular_solves;
+} d4_sim_stats;
+
+/* Fills p with the reference's defaults (project_parser.cpp:44-50, 64). */
+void d4_sim_default_params(d4_sim_params* p);
+
+/* SimulationData from n dislocations (x, y, b = +-1) in any order: sorted and
+ * numbered as simulation_data.cpp:76-154 does (sum b must be 0). */
+int d4_sim_create(d4_ctx* ctx, int n, const double* x, const double* y, const double* b, const d4_sim_params* p,
+                  d4_sim** out);
+/* the same from a .dconf file (x y b per line, simulation_data.cpp:76-113) */
+int d4_sim_create_from_file(d4_ctx* ctx, const char* path, const d4_sim_params* p, d4_sim** out);
+void d4_sim_destroy(d4_sim* s);
+const char* d4_sim_last_error(const d4_sim* s);
+
+/* Simulation::run (simulation.cpp:30-281) until interrupt() (simulation.cpp:722-750) */
+int d4_sim_run(d4_sim* s);
+/* one iteration of run()'s loop (stages I-IV); *accepted = 1 if the step was taken;
+ * *stop = 1 when interrupt() holds before the step (nothing done then) */
+int d4_sim_step(d4_sim* s, int* accepted, int* stop);
+
+/* sorted positions (n {x, y}), or in the input order (disl_order, simulation_data.cpp:118-123) */
+int d4_sim_positions(const d4_sim* s, double* xy, int original_order);
+int d4_sim_get_stats(const d4_sim* s, d4_sim_stats* st);
+/* writeDislocationDataToFile (simulation_data.cpp:177-191) */
+int d4_sim_write_dconf(const d4_sim* s, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,124 @@
+"""The host simulation (include/d4_sim.h) with its pair passes on the GPU,
+against the oracle's run of the same loop (orc_run, dense LU).
+
+Trajectory metric: the largest periodic x deviation between the two final
+configurations in the same (sorted) order, the quantity conf_compare reports
+as its maximum difference (conf_compare.cpp:180-279)."""
+import os
+from importlib import import_module
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = os.path.join(HERE, "golden", "1000_64.dconf")
+dconf = import_module("2d4_amd.dconf")
+simmod = import_module("2d4_amd.sim")
+
+TRAJ_TOL = 1e-9
+C1 = dict(init_step_size=1e-3, max_step_size=1e-3, prec=1e10, dipole_prec=0.0, cutoff_mult=1e20, step_limit=99)
+
+
+def _oracle_kw(kw):
+    m = {"init_step_size": "dt0", "max_step_size": "max_dt"}
+    out = {}
+    for k, v in kw.items():
+        if k == "stress":
+            kind, *vals = v
+            out["stress_kind"] = {"none": 0, "fixed": 1, "cyclic": 2, "increasing": 3}[kind]
+            for name, val in zip(("stress_init", "stress_rate", "stress_period", "stress_amp"), vals):
+                out[name] = val
+        elif k == "heaviside":
+            out[k] = 1 if v else 0
+        else:
+            out[m.get(k, k)] = v
+    return out
+
+
+def _sorted_xy(x, y, b):
+    o = dconf.sort_order(y, b)
+    return np.stack([x[o], y[o]], axis=1)
+
+
+def _random(n, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-0.5, 0.5, n)
+    y = rng.uniform(-0.5, 0.5, n)
+    b = np.where(np.arange(n) % 2 == 0, -1.0, 1.0)
+    return x, y, b
+
+
+def _compare(ctx, orc, x, y, b, kw, tol=TRAJ_TOL):
+    s = simmod.Simulation.from_arrays(ctx, x, y, b, **kw)
+    st = s.run()
+    ref, rst = orc.run(_sorted_xy(x, y, b), **_oracle_kw(kw))
+    dev = dconf.max_periodic_dx(s.positions(), ref)
+    print(f"steps {st['successful_steps']}+{st['failed_steps']} (oracle {rst.ok_steps}+{rst.failed_steps}) "
+          f"max periodic dx {dev:.3e}", flush=True)
+    assert (st["successful_steps"], st["failed_steps"]) == (rst.ok_steps, rst.failed_steps)
+    np.testing.assert_array_equal(s.positions()[:, 1], ref[:, 1])
+    assert dev <= tol
+    return st, dev
+
+
+def test_c1_fixture_first_steps(ctx, orc):
+    xyb = dconf.read_dconf(FIX)
+    _compare(ctx, orc, xyb[:, 0], xyb[:, 1], xyb[:, 2], dict(C1, step_limit=4))
+
+
+def test_c1_fixture_each_step_from_oracle_trajectory(ctx, orc):
+    # every one of C1's 100 accept-all steps, each started from the oracle's
+    # configuration (the whole run is chaotic; see the CPU test of the loop)
+    xyb = dconf.read_dconf(FIX)
+    state = _sorted_xy(xyb[:, 0], xyb[:, 1], xyb[:, 2])
+    b = np.where(np.arange(len(state)) < len(state) // 2, 1.0, -1.0)
+    one = dict(C1, step_limit=0)
+    worst = 0.0
+    for _ in range(100):
+        nxt, _ = orc.run(state, **_oracle_kw(one))
+        s = simmod.Simulation.from_arrays(ctx, state[:, 0], state[:, 1], b, **one)
+        s.run()
+        worst = max(worst, dconf.max_periodic_dx(s.positions(), nxt))
+        s.close()
+        state = nxt
+    print(f"worst single-step deviation {worst:.3e}", flush=True)
+    assert worst <= TRAJ_TOL
+
+
+def test_c1_fixture_adaptive_100_steps(ctx, orc):
+    kw = dict(C1)
+    del kw["prec"], kw["dipole_prec"]
+    xyb = dconf.read_dconf(FIX)
+    st, _ = _compare(ctx, orc, xyb[:, 0], xyb[:, 1], xyb[:, 2], kw)
+    assert st["successful_steps"] == 100
+
+
+def test_c2_heaviside_n1024(ctx, orc):
+    # -r 0 -s 0.1 -u 1.6 -h with the default precision (the first 100 steps)
+    x, y, b = _random(1024, 2024)
+    kw = dict(cutoff_mult=1.6, heaviside=True, stress=("fixed", 0.1, 0.0), step_limit=99)
+    _compare(ctx, orc, x, y, b, kw)
+
+
+def test_taper_dipole_cyclic_n256(ctx, orc):
+    x, y, b = _random(256, 5)
+    kw = dict(cutoff_mult=0.9, wfunc="p", stress=("cyclic", 0.05, 2.0, 0.01), prec=1e-6, dipole_prec=0.1,
+              step_limit=40, max_step_size=1e-3)
+    _compare(ctx, orc, x, y, b, kw)
+
+
+def test_sim_from_file_matches_arrays(ctx, tmp_path):
+    xyb = dconf.read_dconf(FIX)
+    kw = dict(C1, step_limit=2)
+    a = simmod.Simulation.from_file(ctx, FIX, **kw)
+    b = simmod.Simulation.from_arrays(ctx, xyb[:, 0], xyb[:, 1], xyb[:, 2], **kw)
+    a.run()
+    b.run()
+    np.testing.assert_array_equal(a.positions(), b.positions())
+    out = tmp_path / "end.dconf"
+    a.write_dconf(out)
+    back = dconf.read_dconf(out)
+    np.testing.assert_array_equal(back[:, 2], xyb[:, 2])
+    assert dconf.conf_compare(back, np.column_stack([a.positions(True), xyb[:, 2]]))["max_diff"] <= 1e-13

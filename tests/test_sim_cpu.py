@@ -1,0 +1,180 @@
+"""Host simulation logic (2d4_amd/csrc/d4_sim.cpp) on the CPU.
+
+tests/csrc/libsim_cpu.so links d4_sim.cpp and the host sparse LU with the
+oracle standing in for the five GPU entry points the simulation calls, so
+these runs differ from the oracle's own run (orc_run: the same loop with a
+dense LU) only by the host code under test: the step control of
+simulation.cpp:88-276, the CSC capacity growth and the sparse solver."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+from importlib import import_module
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SHIM = os.path.join(HERE, "csrc", "libsim_cpu.so")
+FIX = os.path.join(HERE, "golden", "1000_64.dconf")
+
+extra = import_module("2d4_amd._bind_extra")
+simmod = import_module("2d4_amd.sim")
+dconf = import_module("2d4_amd.dconf")
+
+# C1: -u 1e20 --initial-stepsize 1e-3 -M 1e-3 -P 1e10 -p 0 -C 99 (100 accepted steps)
+C1 = dict(init_step_size=1e-3, max_step_size=1e-3, prec=1e10, dipole_prec=0.0, cutoff_mult=1e20, step_limit=99)
+
+
+@pytest.fixture(scope="module")
+def L():
+    if not os.path.exists(SHIM):
+        pytest.skip("tests/csrc/libsim_cpu.so not built (__graft_entry__.build())")
+    lib = C.CDLL(SHIM)
+    extra.bind(lib)
+    return lib
+
+
+def _new(L, x, y, b, **kw):
+    return simmod.Simulation._create(L, C.c_void_p(1), x, y, b, **kw)
+
+
+def _oracle_kw(kw):
+    m = {"init_step_size": "dt0", "max_step_size": "max_dt"}
+    out = {}
+    for k, v in kw.items():
+        if k == "stress":
+            kind, *vals = v
+            out["stress_kind"] = {"none": 0, "fixed": 1, "cyclic": 2, "increasing": 3}[kind]
+            for name, val in zip(("stress_init", "stress_rate", "stress_period", "stress_amp"), vals):
+                out[name] = val
+        elif k == "heaviside":
+            out[k] = 1 if v else 0
+        else:
+            out[m.get(k, k)] = v
+    return out
+
+
+def _sorted_xy(x, y, b):
+    o = dconf.sort_order(y, b)
+    return np.stack([x[o], y[o]], axis=1)
+
+
+def test_default_params_match_reference_defaults(L):
+    p = extra.SimParams()
+    L.d4_sim_default_params(C.byref(p))
+    # project_parser.cpp:44-50, 64
+    assert (p.init_step_size, p.prec, p.dipole_prec, p.cutoff_mult) == (1e-6, 1e-5, 0.05, 1e20)
+    assert p.wfunc == b"c" and p.heaviside == 0 and p.stress_kind == 0 and p.step_limit == -1
+    assert p.max_step_size <= 0 and p.time_limit <= 0 and p.strain_limit <= 0
+
+
+def test_invalid_configurations_rejected(L):
+    with pytest.raises(ValueError):
+        _new(L, [0.1, 0.2], [0.0, 0.1], [1.0, 1.0])        # sum b != 0
+    with pytest.raises(ValueError):
+        _new(L, [0.1, 0.2], [0.0, 0.1], [0.5, -0.5])       # non-integer b
+
+
+def test_sort_and_order_match_fixture(L, orc):
+    xyb = dconf.read_dconf(FIX)
+    s = _new(L, xyb[:, 0], xyb[:, 1], xyb[:, 2], **C1)
+    ref_xy, ref_order = orc.read_dconf(FIX)
+    np.testing.assert_array_equal(s.positions(), ref_xy)
+    # original order round trip (disl_order, simulation_data.cpp:118-123)
+    np.testing.assert_array_equal(s.positions(original_order=True), xyb[:, :2])
+
+
+def test_c1_counts_and_first_steps_match_oracle_run(L, orc):
+    xyb = dconf.read_dconf(FIX)
+    s = _new(L, xyb[:, 0], xyb[:, 1], xyb[:, 2], **C1)
+    st = s.run()
+    xy0 = _sorted_xy(xyb[:, 0], xyb[:, 1], xyb[:, 2])
+    ref, rst = orc.run(xy0, **_oracle_kw(C1))
+    assert st["successful_steps"] == rst.ok_steps == 100 and st["failed_steps"] == rst.failed_steps == 0
+    assert st["sim_time"] == pytest.approx(rst.sim_time, rel=1e-15)
+    np.testing.assert_array_equal(s.positions()[:, 1], ref[:, 1])
+    # 5 passes per step (Jacobian, speed, speed, Jacobian, speed) and the initial speed pass
+    assert st["pair_passes"] == 1 + 100 * 5 and st["factorizations"] == 100 * 3
+    # Accepting every step of 1e-3 makes this run chaotic: near-collisions amplify a
+    # 1e-16 difference (here only the LU operation order) to O(1) within 20 steps.
+    # The whole trajectory is therefore compared only over its first steps; every
+    # later step is compared from a common start (test below).
+    kw = dict(C1, step_limit=4)
+    s5 = _new(L, xyb[:, 0], xyb[:, 1], xyb[:, 2], **kw)
+    s5.run()
+    ref5, _ = orc.run(xy0, **_oracle_kw(kw))
+    assert dconf.max_periodic_dx(s5.positions(), ref5) <= 1e-13
+
+
+def test_c1_each_step_from_the_oracle_trajectory(L, orc):
+    xyb = dconf.read_dconf(FIX)
+    state = _sorted_xy(xyb[:, 0], xyb[:, 1], xyb[:, 2])
+    b = np.where(np.arange(len(state)) < len(state) // 2, 1.0, -1.0)
+    one = dict(C1, step_limit=0)
+    worst = 0.0
+    for _ in range(100):
+        nxt, _ = orc.run(state, **_oracle_kw(one))
+        s = _new(L, state[:, 0], state[:, 1], b, **one)
+        s.run()
+        worst = max(worst, dconf.max_periodic_dx(s.positions(), nxt))
+        state = nxt
+    assert worst <= 1e-12
+
+
+def test_c1_adaptive_trajectory_matches_oracle_run(L, orc):
+    # C1 with the default precision control (-P 1e-5 -p 0.05): the step
+    # controller keeps the trajectory well conditioned, so it is compared whole
+    kw = dict(C1)
+    del kw["prec"], kw["dipole_prec"]
+    xyb = dconf.read_dconf(FIX)
+    s = _new(L, xyb[:, 0], xyb[:, 1], xyb[:, 2], **kw)
+    st = s.run()
+    ref, rst = orc.run(_sorted_xy(xyb[:, 0], xyb[:, 1], xyb[:, 2]), **_oracle_kw(kw))
+    assert (st["successful_steps"], st["failed_steps"]) == (rst.ok_steps, rst.failed_steps) == (100, 48)
+    assert dconf.max_periodic_dx(s.positions(), ref) <= 1e-12
+
+
+@pytest.mark.parametrize("case", ["heaviside_adaptive", "taper_dipole_cyclic"])
+def test_adaptive_runs_match_oracle_run(L, orc, case):
+    rng = np.random.default_rng(7)
+    n = 128
+    x = rng.uniform(-0.5, 0.5, n)
+    y = rng.uniform(-0.5, 0.5, n)
+    b = np.where(np.arange(n) % 2 == 0, -1.0, 1.0)
+    if case == "heaviside_adaptive":
+        # C2 style: -r 0 -s 0.1 -u 0.05*sqrt(N) -h, defaults -P 1e-5 -p 0.05
+        kw = dict(cutoff_mult=0.05 * np.sqrt(n), heaviside=True, stress=("fixed", 0.1, 0.0), step_limit=30)
+    else:
+        kw = dict(cutoff_mult=0.9, wfunc="p", stress=("cyclic", 0.05, 2.0, 0.01), prec=1e-6, dipole_prec=0.1,
+                  step_limit=20, max_step_size=1e-3)
+    s = _new(L, x, y, b, **kw)
+    st = s.run()
+    ref, rst = orc.run(_sorted_xy(x, y, b), **_oracle_kw(kw))
+    assert st["successful_steps"] == rst.ok_steps and st["failed_steps"] == rst.failed_steps
+    assert st["step_size"] == pytest.approx(rst.step_size, rel=1e-9)
+    assert dconf.max_periodic_dx(s.positions(), ref) <= 1e-10
+
+
+def test_capacity_growth_keeps_results(L, orc):
+    # a dense pattern needs far more than the initial 16 entries per column
+    xyb = dconf.read_dconf(FIX)
+    kw = dict(C1, step_limit=2)
+    s = _new(L, xyb[:, 0], xyb[:, 1], xyb[:, 2], **kw)
+    s.run()
+    ref, _ = orc.run(_sorted_xy(xyb[:, 0], xyb[:, 1], xyb[:, 2]), **_oracle_kw(kw))
+    assert dconf.max_periodic_dx(s.positions(), ref) <= 1e-13
+
+
+def test_step_interface_and_stop(L):
+    xyb = dconf.read_dconf(FIX)
+    s = _new(L, xyb[:, 0], xyb[:, 1], xyb[:, 2], **dict(C1, step_limit=1))
+    seen = [s.step() for _ in range(3)]
+    assert seen == [(True, False), (True, False), (False, True)]
+
+
+def test_write_dconf_round_trip(L, tmp_path):
+    xyb = dconf.read_dconf(FIX)
+    s = _new(L, xyb[:, 0], xyb[:, 1], xyb[:, 2], **C1)
+    p = tmp_path / "out.dconf"
+    s.write_dconf(p)
+    back = dconf.read_dconf(p)
+    np.testing.assert_allclose(back, xyb, rtol=0, atol=1e-13)
