@@ -23,6 +23,11 @@ class SimStats(C.Structure):
                 ("factorizations", C.c_int64), ("singular_solves", C.c_int64)]
 
 
+class EnsConfig(C.Structure):
+    """d4_ens_config (include/d4_batch.h)."""
+    _fields_ = [("pass_grid", C.c_int), ("tile_order", C.c_int), ("use_graph", C.c_int), ("host_threads", C.c_int)]
+
+
 SIGNATURES = {
     "d4_sim_default_params": (None, [_P(SimParams)]),
     "d4_sim_create": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _P(SimParams), _P(_vp)]),
@@ -34,6 +39,20 @@ SIGNATURES = {
     "d4_sim_positions": (C.c_int, [_vp, _vp, C.c_int]),
     "d4_sim_get_stats": (C.c_int, [_vp, _P(SimStats)]),
     "d4_sim_write_dconf": (C.c_int, [_vp, C.c_char_p]),
+    # include/d4_batch.h
+    "d4_batch_speed_pass": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "d4_batch_jacobian_pass": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _pi,
+                                         _vp, _vp, _vp]),
+    "d4_ens_create": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _P(SimParams), C.c_int, _P(_vp)]),
+    "d4_ens_destroy": (None, [_vp]),
+    "d4_ens_last_error": (C.c_char_p, [_vp]),
+    "d4_ens_step": (C.c_int, [_vp, C.c_int, _pi]),
+    "d4_ens_positions": (C.c_int, [_vp, _vp, C.c_int]),
+    "d4_ens_stats": (C.c_int, [_vp, _vp]),
+    "d4_ens_set_config": (C.c_int, [_vp, _P(EnsConfig)]),
+    "d4_ens_get_config": (C.c_int, [_vp, _P(EnsConfig)]),
+    "d4_ens_autotune": (C.c_int, [_vp, _P(EnsConfig), _P(_d)]),
+    "d4_ens_timing": (C.c_int, [_vp, _vp]),
 }
 
 

@@ -13,6 +13,7 @@ using namespace d4;
 
 d4_ctx::~d4_ctx()
 {
+    if (batch) batch_passes_free(batch);
     for (auto* b : {&part, &tiles, &xy, &v0, &v1, &tol, &coo, &coo_count, &cnt, &cursor, &Ap, &Ai, &Ax, &idx, &dvec,
                     &nw, &aux0, &aux1, &aux2})
         b->release();
@@ -102,29 +103,6 @@ int d4_ctx::single_tiles(int n, int* ntiles)
     if (e != hipSuccess) return hip_fail(e, "tile table upload");
     tiles_n = n;
     return D4_OK;
-}
-
-static RealParams single_params(int n, const d4_jac_params* p, double tau)
-{
-    RealParams r{};
-    r.nb = (n + kTile - 1) / kTile;
-    r.slot0 = 0;
-    if (p) {
-        r.h = p->h;
-        r.tauA = p->tau_a;
-        r.tauB = p->tau_b;
-        r.cut_sq = p->cut_sq;
-        r.cutoff = p->cutoff;
-        r.one_per = p->one_per_cut_sq;
-        r.range_sq = (p->heaviside ? 1.0 : 49.0) * p->cut_sq;  // simulation.cpp:461-465
-        r.heaviside = p->heaviside;
-        r.wfunc = p->wfunc;
-        r.tol_cap2 = p->tol_cap2;
-    } else {
-        r.tauA = r.tauB = tau;
-        r.wfunc = 'c';
-    }
-    return r;
 }
 
 // Runs one pair pass over a single realization whose positions are already on
@@ -359,7 +337,7 @@ int d4_speed_pass_dev(d4_ctx* ctx, int n, const double* xy, double tau, double* 
 {
     if (!ctx || n < 0 || (n > 0 && (!xy || !v))) return D4_ERR_INVALID;
     if (n == 0) return D4_OK;
-    RealParams rp = single_params(n, nullptr, tau);
+    RealParams rp = real_params(n, nullptr, tau);
     rp.tol_cap2 = tol ? tol_cap2 : 0.0;
     return ctx->single_pass(kModeSpeed, n, xy, rp, tol, v, nullptr, nullptr, nullptr, nullptr, 0, nullptr, 0, 0, 1);
 }
@@ -390,7 +368,7 @@ int d4_jacobian_dense_dev(d4_ctx* ctx, int n, const double* xy, const d4_jac_par
 {
     if (!ctx || !p || n < 0 || (n > 0 && (!xy || !vA || !M || ld < n))) return D4_ERR_INVALID;
     if (n == 0) return D4_OK;
-    RealParams rp = single_params(n, p, 0.0);
+    RealParams rp = real_params(n, p, 0.0);
     if (!tol) rp.tol_cap2 = 0.0;
     return ctx->single_pass(kModeJacDense, n, xy, rp, tol, vA, vB, nullptr, dvec, M, ld, nullptr, 0, 0, 1);
 }
@@ -461,7 +439,7 @@ int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* 
         D4_TRY(ctx->h2d(dtol, tol, b, "tolerance upload"));
     }
     D4_TRY(ctx->h2d(ctx->xy.p, xy, 2 * b, "position upload"));
-    RealParams rp = single_params(n, p, 0.0);
+    RealParams rp = real_params(n, p, 0.0);
     if (!dtol) rp.tol_cap2 = 0.0;
     int ncoo = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
@@ -562,7 +540,7 @@ int d4_vjp_dev(d4_ctx* ctx, int n, const double* xy, const double* w, double* gx
         if (gtau) *gtau = 0;
         return D4_OK;
     }
-    RealParams rp = single_params(n, nullptr, 0.0);
+    RealParams rp = real_params(n, nullptr, 0.0);
     D4_TRY(ctx->single_pass(kModeVjp, n, xy, rp, nullptr, gx, nullptr, gy, nullptr, nullptr, 0, w, flags & 1,
                             gy != nullptr, 0));
     if (gtau) {  // dL/dtau = sum_i b_i w_i: host-side sum over a downloaded w

@@ -9,7 +9,38 @@
 #include <string>
 #include <vector>
 
+#include "../../include/d4.h"
 #include "d4_kernels.h"
+
+namespace d4 {
+// RealParams of one realization of n from a Jacobian pass's parameters, or
+// (p == null) of a speed pass at stress tau
+inline RealParams real_params(int n, const d4_jac_params* p, double tau)
+{
+    RealParams r{};
+    r.nb = (n + kTile - 1) / kTile;
+    r.slot0 = 0;
+    if (p) {
+        r.h = p->h;
+        r.tauA = p->tau_a;
+        r.tauB = p->tau_b;
+        r.cut_sq = p->cut_sq;
+        r.cutoff = p->cutoff;
+        r.one_per = p->one_per_cut_sq;
+        r.range_sq = (p->heaviside ? 1.0 : 49.0) * p->cut_sq;  // simulation.cpp:461-465
+        r.heaviside = p->heaviside;
+        r.wfunc = p->wfunc;
+        r.tol_cap2 = p->tol_cap2;
+    } else {
+        r.tauA = r.tauB = tau;
+        r.wfunc = 'c';
+    }
+    return r;
+}
+}  // namespace d4
+
+struct BatchPasses;
+void batch_passes_free(BatchPasses* b);
 
 struct DevBuf {
     void* p = nullptr;
@@ -33,6 +64,7 @@ struct d4_ctx {
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_marks;
+    BatchPasses* batch = nullptr;  // tables and buffers of d4_batch_* (d4_batch.hip)
     hipEvent_t take_event();
     int mark_begin(hipEvent_t* e);
     int mark_end(hipEvent_t b);

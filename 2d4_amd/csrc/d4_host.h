@@ -137,6 +137,34 @@ private:
     void build(std::vector<double> x, std::vector<double> y, std::vector<double> b);
 };
 
+class Simulation;
+
+// One of the pair passes a step needs (simulation.cpp:98-152 makes six GPU
+// calls per step: two Jacobian passes, three speed passes and the stage-II
+// rescale; run() adds one speed pass before the loop).
+struct PassJob {
+    enum Kind { None = 0, Speed, Jacobian, Rescale };
+    Kind kind = None;
+    const std::vector<DislwoB>* xy = nullptr;  // Speed, Jacobian
+    double tau = 0;                            // Speed: extStress(t)
+    std::vector<double>* v = nullptr;          // Speed: out
+    d4_jac_params jp{};                        // Jacobian
+    std::vector<double>* vA = nullptr;         // Jacobian: speeds at tau_a, tau_b
+    std::vector<double>* vB = nullptr;
+    double factor = 0, dtau = 0;               // Rescale (simulation.cpp:546-608)
+    std::vector<double>* speed = nullptr;      // Rescale: speeds moved by dtau b
+    int nnz = 0;                               // Jacobian: entries written to the step matrix
+};
+
+// Runs PassJobs.  The default engine runs each job at once through include/d4.h
+// on the simulation's context; a batching engine collects the same job from
+// many simulations and runs them in one launch.
+class PassEngine {
+public:
+    virtual ~PassEngine() = default;
+    virtual int run(Simulation& sim, PassJob& job);
+};
+
 class Simulation {
 public:
     Simulation(std::shared_ptr<SimulationData> sD, d4_ctx* ctx);
@@ -149,26 +177,43 @@ public:
     const std::string& error() const { return error_; }
     int64_t pairPasses = 0, factorizations = 0, singularSolves = 0;
 
+    // the step split at its pair passes: begin_step() (false when interrupt()
+    // holds) sets the first job, then each after_pass() consumes job() and sets
+    // the next; job().kind == None when the step is complete
+    void begin_prologue();
+    bool begin_step();
+    void after_pass();
+    PassJob& job() { return job_; }
+    bool lastAccepted() const { return lastAccepted_; }
+    SimulationData& data() { return *sD; }
+    PrecisionHandler& precision() { return *pH; }
+    d4_ctx* ctx() const { return ctx_; }
+    bool check(int rc, const char* where);
+
 private:
     std::shared_ptr<SimulationData> sD;
     std::unique_ptr<PrecisionHandler> pH;
     d4_ctx* ctx_;
     d4host::SparseLU lu_;
+    PassEngine default_engine_;
     int status_ = 0;
     double lastRatio_ = 0;
+    bool lastAccepted_ = false;
     std::string error_;
-    void calculateSpeedsAtTime(const std::vector<DislwoB>& dis, std::vector<double>& forces, double simTime);
-    int calcJacobianAndSpeedsAtTimes(double stepsize, const std::vector<DislwoB>& dislocs,
-                                     std::vector<double>& forces_A, std::vector<double>& forces_B, double simTime_A,
-                                     double simTime_B);
-    void calcJacobianAndSpeedsFromPrev(double baseTime, double newTime, double oldTime);
+    PassJob job_;
+    int phase_ = 0;
+    double t0_ = 0, t1_ = 0, th_ = 0;
+    void set_speed_job(const std::vector<DislwoB>& dis, std::vector<double>& forces, double simTime);
+    void set_jacobian_job(double stepsize, const std::vector<DislwoB>& dislocs, std::vector<double>& forces_A,
+                          std::vector<double>& forces_B, double simTime_A, double simTime_B);
+    void set_rescale_job(double baseTime, double newTime, double oldTime);
     void calcGSolveAndUpdate(std::vector<DislwoB>& new_disloc, const std::vector<DislwoB>& old_config, double stepSize,
                              const std::vector<double>& endSpeed, const std::vector<double>& initSpeed);
     void calculateSparseFormForJacobian();
     void solveEQSys();
     void calculateXError();
     double calculateStrainIncrement(const std::vector<DislwoB>& old, const std::vector<DislwoB>& newD) const;
-    bool check(int rc, const char* where);
+    bool finish_step();
 };
 
 }  // namespace sdddstCore

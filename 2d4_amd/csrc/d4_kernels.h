@@ -46,6 +46,8 @@ enum PassMode : int {
     kModeJacDense = 1,   // sigma + d sigma: speeds, V dense column-major, row sums
     kModeJacCoo = 2,     // sigma all pairs, d sigma inside the storage radius -> COO list
     kModeVjp = 3,        // d sigma/dx (+ d sigma/dy): w-weighted pair sums
+    kModeJacDenseCut = 4,  // as kModeJacDense with the storage radius of kModeJacCoo:
+                           // V = 0 outside it (the reference's cutoff, dense storage)
 };
 
 struct PassArgs {
@@ -58,9 +60,11 @@ struct PassArgs {
     RealParams rp0;
     double* part;             // partial sums [slot][NQ][128]
     double* tol;              // per element (may be null)
-    // dense Jacobian output (mode 1, single realization)
+    // dense Jacobian output (modes 1, 4): one matrix M (leading dimension ld),
+    // or, with moff, matrix r at M + moff[r] with leading dimension n_r
     double* M;
     int64_t ld;
+    const int64_t* moff;
     // COO output (mode 2)
     CooEntry* coo;
     int* coo_count;
@@ -74,7 +78,9 @@ struct PassArgs {
 // number of accumulated quantities per mode
 inline int pass_nq(int mode) { return mode == kModeSpeed || mode == kModeJacCoo ? 1 : 2; }
 
-hipError_t launch_pair_pass(int mode, const PassArgs& a, hipStream_t s);
+// grid = 0: one workgroup per tile; grid > 0: `grid` persistent workgroups
+// that stride over the tile table
+hipError_t launch_pair_pass(int mode, const PassArgs& a, hipStream_t s, int grid = 0);
 
 // Field values for n independent pairs (E, c may be null -> computed on device).
 hipError_t launch_field_eval(int64_t n, const double* dx, const double* dy, const double* E, const double* c,
@@ -98,15 +104,20 @@ struct ReduceArgs {
     double* dvec;            // weight(q1) (may be null)
     double* M;               // dense diagonal <- q1 (may be null)
     int64_t ld;
+    const int64_t* moff;     // per-realization matrix offsets (leading dimension n_r), or null
     int add_tau;             // 1: add tau b to vA/vB
 };
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
 
 // CSC assembly from a COO list over a block-diagonal batch (global indices).
-hipError_t launch_csc_count(const CooEntry* coo, int ncoo, int* cnt, int total, hipStream_t s);
+// ncoo_dev (optional): device-side pair count, capped by ncoo (the grid is sized by ncoo)
+hipError_t launch_csc_count(const CooEntry* coo, int ncoo, int* cnt, int total, hipStream_t s,
+                            const int* ncoo_dev = nullptr);
 hipError_t launch_scan(const int* cnt, int* Ap, int total, hipStream_t s);  // Ap[0..total]
 hipError_t launch_csc_fill(const CooEntry* coo, int ncoo, const int* Ap, int* cursor, int* Ai, double* Ax,
-                           int total, hipStream_t s);
+                           int total, hipStream_t s, const int* ncoo_dev = nullptr);
+// p[0..n) = v (a kernel, so it can sit in a captured graph)
+hipError_t launch_fill_int(int* p, int v, int n, hipStream_t s);
 // wfunc per realization from wf_arr (null: wf0 for all)
 hipError_t launch_csc_finish(const int* Ap, int* Ai, double* Ax, int* indexes, double* dvec, const int* elem_real,
                              const int* wf_arr, int wf0, int total, hipStream_t s);
@@ -133,5 +144,23 @@ struct RescaleArgs {
 hipError_t launch_rescale(const RescaleArgs& a, hipStream_t s);
 // dense final scaling: M[j,i] *= (1+d_j)/2, diagonal += 1
 hipError_t launch_dense_scale(double* M, int64_t ld, int n, const double* dvec, hipStream_t s);
+
+// Batched dense matrices: realization r's n_r x n_r column-major matrix at
+// moff[r]; elements g in [off[r], off[r+1]) are its rows and columns.
+struct DenseBatch {
+    const int* off;
+    const int* elem_real;
+    const int64_t* moff;
+    int total;               // sum of n_r
+    int maxn;                // max n_r
+};
+// F = the reference's final matrix from M = V + diag(s): F[j,i] = M[j,i] (1+d_j)/2 (+1 on the diagonal)
+hipError_t launch_dense_scale_batched(const DenseBatch& b, const double* M, double* F, const double* dvec,
+                                      hipStream_t s);
+// stage-II matrix from the same M (simulation.cpp:546-598): with A = the stage-I
+// final matrix, F[j,i] = A[j,i]/2 f_j off the diagonal, 1 + (A[j,j]-1)/2 f_j on it,
+// f_j = (1 + dnew_j)/(1 + dold_j)
+hipError_t launch_dense_rescale_batched(const DenseBatch& b, const double* M, double* F, const double* dold,
+                                        const double* dnew, hipStream_t s);
 
 }  // namespace d4

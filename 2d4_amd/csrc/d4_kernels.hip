@@ -16,6 +16,8 @@
 #include "d4_field.h"
 #include "d4_kernels.h"
 
+#include <algorithm>
+
 namespace d4 {
 
 // minimum of a non-negative double via its bit pattern (ordering of
@@ -25,14 +27,13 @@ __device__ __forceinline__ void atomic_min_nonneg(double* p, double v)
     atomicMin(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
-// 64 threads (one wavefront) per block; at least 4 waves per SIMD (<= 128
-// VGPRs) for the sigma-only pass, 3 (<= 168 VGPRs) for the dense pass, 2 for
-// the pair-list and adjoint passes (their extra live values would spill at 168)
+// One 64 x 64 tile of a pass, evaluated by one wavefront (the LDS arrays are
+// per-workgroup allocations of this function's instantiation).
 template <int MODE, bool EXACT>
-__global__ __launch_bounds__(64, MODE == kModeSpeed ? 4 : (MODE == kModeJacDense ? 3 : 2)) void pair_pass_kernel(
-    PassArgs a)
+__device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
 {
-    constexpr bool kDense = MODE == kModeJacDense;
+    constexpr bool kDense = MODE == kModeJacDense || MODE == kModeJacDenseCut;
+    constexpr bool kCut = MODE == kModeJacCoo || MODE == kModeJacDenseCut;  // storage radius applies
     constexpr bool kCoo = MODE == kModeJacCoo;
     constexpr bool kVjp = MODE == kModeVjp;
     constexpr int NQ = (MODE == kModeSpeed || MODE == kModeJacCoo) ? 1 : 2;
@@ -42,13 +43,13 @@ __global__ __launch_bounds__(64, MODE == kModeSpeed ? 4 : (MODE == kModeJacDense
     __shared__ double stage[kDense ? 4 * 16 * 17 : 1];
 
     const int lane = threadIdx.x;
-    const TileDesc td = a.tiles[blockIdx.x];
     const int base = a.off ? a.off[td.r] : 0;
     const int n = a.off ? a.off[td.r + 1] - base : a.n0;
     const RealParams rp = a.rp ? a.rp[td.r] : a.rp0;
     const bool diag = td.I0 == td.J0;
     const int half = n >> 1;
-
+    double* const Mb = (kDense && a.moff) ? a.M + a.moff[td.r] : a.M;
+    const int64_t ld = (kDense && a.moff) ? (int64_t)n : a.ld;
     const int i = td.I0 + lane;
     const bool vi = i < n;
     const double2 pi = vi ? a.xy[base + i] : make_double2(0.0, 0.0);
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(64, MODE == kModeSpeed ? 4 : (MODE == kModeJacDense
                     const double f = sig * bi * bj;
                     q0 += f;
                     atomicAdd(&jq0[jl], -f);
-                } else if (kDense) {
+                } else if (MODE == kModeJacDense) {
                     pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
                     const double f = sig * bi * bj;
                     V = rp.h * bj * bi * dsig;
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(64, MODE == kModeSpeed ? 4 : (MODE == kModeJacDense
                     atomicAdd(&jq0[jl], -f);
                     q1 -= V;
                     atomicAdd(&jq1[jl], -V);
-                } else if (kCoo) {
+                } else if (kCut) {
                     inr = d2 < rp.range_sq;
                     if (inr) {
                         pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
@@ -115,6 +116,10 @@ __global__ __launch_bounds__(64, MODE == kModeSpeed ? 4 : (MODE == kModeJacDense
                     const double f = sig * bi * bj;
                     q0 += f;
                     atomicAdd(&jq0[jl], -f);
+                    if (MODE == kModeJacDenseCut) {
+                        q1 -= V;
+                        atomicAdd(&jq1[jl], -V);
+                    }
                 } else {  // kVjp
                     pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
                     const double bb = bi * bj;
@@ -158,16 +163,16 @@ __global__ __launch_bounds__(64, MODE == kModeSpeed ? 4 : (MODE == kModeJacDense
             const double* st1 = stage + (g * 16) * 17 + q;  // stage[g][k][q], k = 0..15
             const double* st2 = stage + (g * 16 + q) * 17;  // stage[g][q][k]
             // V(ib+k, jb+q) -> column ib+k, row jb+q;  V(ib+q, jb+k) -> column jb+k, row ib+q
-            double* p1 = a.M + (int64_t)ib * a.ld + jb + q;
-            double* p2 = a.M + (int64_t)jb * a.ld + ib + q;
+            double* p1 = Mb + (int64_t)ib * ld + jb + q;
+            double* p2 = Mb + (int64_t)jb * ld + ib + q;
             if (!diag && td.I0 + kTile <= n && td.J0 + kTile <= n) {
                 // interior tile: every entry valid
 #pragma unroll 4
                 for (int k = 0; k < 16; ++k) {
                     *p1 = st1[k * 17];
                     *p2 = st2[k];
-                    p1 += a.ld;
-                    p2 += a.ld;
+                    p1 += ld;
+                    p2 += ld;
                 }
             } else {
                 for (int k = 0; k < 16; ++k) {
@@ -175,8 +180,8 @@ __global__ __launch_bounds__(64, MODE == kModeSpeed ? 4 : (MODE == kModeJacDense
                     if (i1 < n && j1 < n && (!diag || i1 < j1)) *p1 = st1[k * 17];
                     const int i2 = ib + q, j2 = jb + k;
                     if (i2 < n && j2 < n && (!diag || i2 < j2)) *p2 = st2[k];
-                    p1 += a.ld;
-                    p2 += a.ld;
+                    p1 += ld;
+                    p2 += ld;
                 }
             }
             __syncthreads();
@@ -192,24 +197,49 @@ __global__ __launch_bounds__(64, MODE == kModeSpeed ? 4 : (MODE == kModeJacDense
     }
 }
 
+
+// 64 threads (one wavefront) per block; at least 4 waves per SIMD (<= 128
+// VGPRs) for the sigma-only pass, 3 (<= 168 VGPRs) for the dense pass, 2 for
+// the pair-list, dense-with-cutoff and adjoint passes (their extra live values
+// would spill at 168).  PERSIST: the grid strides over the tile table.
+template <int MODE, bool EXACT, bool PERSIST>
+__global__ __launch_bounds__(64, MODE == kModeSpeed ? 4 : (MODE == kModeJacDense ? 3 : 2)) void pair_pass_kernel(
+    PassArgs a)
+{
+    if constexpr (PERSIST) {
+        for (int t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+            pair_tile<MODE, EXACT>(a, a.tiles[t]);
+            __syncthreads();
+        }
+    } else {
+        pair_tile<MODE, EXACT>(a, a.tiles[blockIdx.x]);
+    }
+}
+
 template <int MODE>
-static hipError_t launch_mode(const PassArgs& a, hipStream_t s)
+static hipError_t launch_mode(const PassArgs& a, hipStream_t s, int grid)
 {
     if (a.ntiles <= 0) return hipSuccess;
-    if (a.exact)
-        hipLaunchKernelGGL((pair_pass_kernel<MODE, true>), dim3(a.ntiles), dim3(kTile), 0, s, a);
-    else
-        hipLaunchKernelGGL((pair_pass_kernel<MODE, false>), dim3(a.ntiles), dim3(kTile), 0, s, a);
+    if (grid > 0) {
+        const dim3 g(std::min(grid, a.ntiles));
+        if (a.exact) hipLaunchKernelGGL((pair_pass_kernel<MODE, true, true>), g, dim3(kTile), 0, s, a);
+        else hipLaunchKernelGGL((pair_pass_kernel<MODE, false, true>), g, dim3(kTile), 0, s, a);
+    } else if (a.exact) {
+        hipLaunchKernelGGL((pair_pass_kernel<MODE, true, false>), dim3(a.ntiles), dim3(kTile), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((pair_pass_kernel<MODE, false, false>), dim3(a.ntiles), dim3(kTile), 0, s, a);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_pair_pass(int mode, const PassArgs& a, hipStream_t s)
+hipError_t launch_pair_pass(int mode, const PassArgs& a, hipStream_t s, int grid)
 {
     switch (mode) {
-    case kModeSpeed: return launch_mode<kModeSpeed>(a, s);
-    case kModeJacDense: return launch_mode<kModeJacDense>(a, s);
-    case kModeJacCoo: return launch_mode<kModeJacCoo>(a, s);
-    case kModeVjp: return launch_mode<kModeVjp>(a, s);
+    case kModeSpeed: return launch_mode<kModeSpeed>(a, s, grid);
+    case kModeJacDense: return launch_mode<kModeJacDense>(a, s, grid);
+    case kModeJacCoo: return launch_mode<kModeJacCoo>(a, s, grid);
+    case kModeVjp: return launch_mode<kModeVjp>(a, s, grid);
+    case kModeJacDenseCut: return launch_mode<kModeJacDenseCut>(a, s, grid);
     default: return hipErrorInvalidValue;
     }
 }
@@ -304,7 +334,10 @@ __global__ __launch_bounds__(64 * kReduceWaves) void reduce_kernel(ReduceArgs a)
     if (a.vB) a.vB[g] = a.add_tau ? sum[0] + rp.tauB * bsg : sum[0];
     if (a.q1) a.q1[g] = sum[1];
     if (a.dvec) a.dvec[g] = weight(sum[1], (char)rp.wfunc);
-    if (a.M) a.M[(int64_t)k * a.ld + k] = sum[1];
+    if (a.M) {
+        if (a.moff) a.M[a.moff[r] + (int64_t)k * n + k] = sum[1];
+        else a.M[(int64_t)k * a.ld + k] = sum[1];
+    }
 }
 
 hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s)
@@ -317,20 +350,23 @@ hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s)
 
 // ------------------------------------------------------------------ CSC
 
-__global__ void csc_count_kernel(const CooEntry* coo, int ncoo, int* cnt)
+// ncoo_dev (when not null): the pair count the pass left on the device,
+// capped by ncoo (the list capacity); the grid then covers the capacity
+__global__ void csc_count_kernel(const CooEntry* coo, int ncoo, const int* ncoo_dev, int* cnt)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ncoo_dev) ncoo = min(ncoo, *ncoo_dev);
     if (k >= ncoo) return;
     const CooEntry e = coo[k];
     atomicAdd(&cnt[e.i], 1);
     if (e.v != 0.0) atomicAdd(&cnt[e.j], 1);  // simulation.cpp:365: exact zeros are not mirrored
 }
 
-hipError_t launch_csc_count(const CooEntry* coo, int ncoo, int* cnt, int total, hipStream_t s)
+hipError_t launch_csc_count(const CooEntry* coo, int ncoo, int* cnt, int total, hipStream_t s, const int* ncoo_dev)
 {
     (void)total;
     if (ncoo <= 0) return hipSuccess;
-    hipLaunchKernelGGL(csc_count_kernel, dim3((ncoo + 255) / 256), dim3(256), 0, s, coo, ncoo, cnt);
+    hipLaunchKernelGGL(csc_count_kernel, dim3((ncoo + 255) / 256), dim3(256), 0, s, coo, ncoo, ncoo_dev, cnt);
     return hipGetLastError();
 }
 
@@ -375,9 +411,10 @@ __global__ void csc_diag_kernel(const int* Ap, int* cursor, int* Ai, double* Ax,
     cursor[c] = p + 1;
 }
 
-__global__ void csc_fill_kernel(const CooEntry* coo, int ncoo, int* cursor, int* Ai, double* Ax)
+__global__ void csc_fill_kernel(const CooEntry* coo, int ncoo, const int* ncoo_dev, int* cursor, int* Ai, double* Ax)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ncoo_dev) ncoo = min(ncoo, *ncoo_dev);
     if (k >= ncoo) return;
     const CooEntry e = coo[k];
     int p = atomicAdd(&cursor[e.i], 1);
@@ -391,11 +428,25 @@ __global__ void csc_fill_kernel(const CooEntry* coo, int ncoo, int* cursor, int*
 }
 
 hipError_t launch_csc_fill(const CooEntry* coo, int ncoo, const int* Ap, int* cursor, int* Ai, double* Ax,
-                           int total, hipStream_t s)
+                           int total, hipStream_t s, const int* ncoo_dev)
 {
     hipLaunchKernelGGL(csc_diag_kernel, dim3((total + 255) / 256), dim3(256), 0, s, Ap, cursor, Ai, Ax, total);
     if (ncoo > 0)
-        hipLaunchKernelGGL(csc_fill_kernel, dim3((ncoo + 255) / 256), dim3(256), 0, s, coo, ncoo, cursor, Ai, Ax);
+        hipLaunchKernelGGL(csc_fill_kernel, dim3((ncoo + 255) / 256), dim3(256), 0, s, coo, ncoo, ncoo_dev, cursor,
+                           Ai, Ax);
+    return hipGetLastError();
+}
+
+__global__ void fill_int_kernel(int* p, int v, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+hipError_t launch_fill_int(int* p, int v, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, s, p, v, n);
     return hipGetLastError();
 }
 
@@ -533,6 +584,58 @@ hipError_t launch_dense_scale(double* M, int64_t ld, int n, const double* dvec, 
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(dense_scale_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, M, ld, n, dvec);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------ batched dense matrices
+
+// x: global column index g (realization r, local column c), y: 256-row chunks
+__global__ void dense_scale_batched_kernel(DenseBatch b, const double* M, double* F, const double* dvec)
+{
+#pragma clang fp contract(off)
+    const int g = blockIdx.x;
+    const int r = b.elem_real[g];
+    const int base = b.off[r], n = b.off[r + 1] - base, c = g - base;
+    const int row = blockIdx.y * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    const int64_t k = b.moff[r] + (int64_t)c * n + row;
+    double v = M[k] * ((1 + dvec[base + row]) / 2);  // simulation.cpp:516-522
+    if (row == c) v += 1;
+    F[k] = v;
+}
+
+hipError_t launch_dense_scale_batched(const DenseBatch& b, const double* M, double* F, const double* dvec,
+                                      hipStream_t s)
+{
+    if (b.total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(dense_scale_batched_kernel, dim3(b.total, (b.maxn + 255) / 256), dim3(256), 0, s, b, M, F,
+                       dvec);
+    return hipGetLastError();
+}
+
+__global__ void dense_rescale_batched_kernel(DenseBatch b, const double* M, double* F, const double* dold,
+                                             const double* dnew)
+{
+#pragma clang fp contract(off)
+    const int g = blockIdx.x;
+    const int r = b.elem_real[g];
+    const int base = b.off[r], n = b.off[r + 1] - base, c = g - base;
+    const int row = blockIdx.y * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    const int64_t k = b.moff[r] + (int64_t)c * n + row;
+    const double d0 = dold[base + row];
+    double v = M[k] * ((1 + d0) / 2);  // the stage-I entry, formed as the scale kernel forms it
+    if (row == c) v += 1;
+    const double f = (1 + dnew[base + row]) / (1 + d0);  // simulation.cpp:575-594
+    F[k] = row == c ? 1 + (v - 1) / 2 * f : v / 2 * f;
+}
+
+hipError_t launch_dense_rescale_batched(const DenseBatch& b, const double* M, double* F, const double* dold,
+                                        const double* dnew, hipStream_t s)
+{
+    if (b.total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(dense_rescale_batched_kernel, dim3(b.total, (b.maxn + 255) / 256), dim3(256), 0, s, b, M, F,
+                       dold, dnew);
     return hipGetLastError();
 }
 

@@ -193,19 +193,22 @@ bool Simulation::check(int rc, const char* where)
     return false;
 }
 
-void Simulation::calculateSpeedsAtTime(const std::vector<DislwoB>& dis, std::vector<double>& forces, double simTime)
+void Simulation::set_speed_job(const std::vector<DislwoB>& dis, std::vector<double>& forces, double simTime)
 {
-    const double tau = sD->externalStressProtocol->extStress(simTime);
-    pairPasses++;
-    check(d4_speed_pass(ctx_, (int)sD->dc, &dis[0].x, tau, forces.data(), pH->toleranceData(), pH->toleranceCap2()),
-          "speed pass");
+    job_ = PassJob();  // calculateSpeedsAtTime, simulation.cpp:287-333
+    job_.kind = PassJob::Speed;
+    job_.xy = &dis;
+    job_.tau = sD->externalStressProtocol->extStress(simTime);
+    job_.v = &forces;
 }
 
-int Simulation::calcJacobianAndSpeedsAtTimes(double stepsize, const std::vector<DislwoB>& dislocs,
-                                             std::vector<double>& forces_A, std::vector<double>& forces_B,
-                                             double simTime_A, double simTime_B)
+void Simulation::set_jacobian_job(double stepsize, const std::vector<DislwoB>& dislocs, std::vector<double>& forces_A,
+                                  std::vector<double>& forces_B, double simTime_A, double simTime_B)
 {
-    d4_jac_params p;
+    job_ = PassJob();  // calcJacobianAndSpeedsAtTimes, simulation.cpp:345-527
+    job_.kind = PassJob::Jacobian;
+    job_.xy = &dislocs;
+    d4_jac_params& p = job_.jp;
     p.h = stepsize;
     p.tau_a = sD->externalStressProtocol->extStress(simTime_A);
     p.tau_b = sD->externalStressProtocol->extStress(simTime_B);
@@ -215,36 +218,54 @@ int Simulation::calcJacobianAndSpeedsAtTimes(double stepsize, const std::vector<
     p.heaviside = sD->heavisideCutoff ? 1 : 0;
     p.wfunc = sD->weightFunc;
     p.tol_cap2 = pH->toleranceCap2();
-    const int n = (int)sD->dc;
-    int nnz = 0;
-    if (sD->Ai.empty()) {
-        sD->Ai.resize((size_t)n * 16 + 16);
-        sD->Ax.resize(sD->Ai.size());
-    }
-    pairPasses++;
-    int rc = d4_jacobian_pass(ctx_, n, &dislocs[0].x, &p, forces_A.data(), forces_B.data(), sD->Ap.data(),
-                              sD->Ai.data(), sD->Ax.data(), (int)sD->Ai.size(), &nnz, sD->indexes.data(),
-                              sD->dVec.data(), pH->toleranceData());
-    if (rc == D4_ERR_CAPACITY) {  // grow (simulation_data.cpp:157-175 grows by dc) and run the pass again
-        sD->Ai.resize((size_t)nnz + n);
-        sD->Ax.resize(sD->Ai.size());
-        rc = d4_jacobian_pass(ctx_, n, &dislocs[0].x, &p, forces_A.data(), forces_B.data(), sD->Ap.data(),
-                              sD->Ai.data(), sD->Ax.data(), (int)sD->Ai.size(), &nnz, sD->indexes.data(),
-                              sD->dVec.data(), pH->toleranceData());
-    }
-    check(rc, "jacobian pass");
-    calculateSparseFormForJacobian();
-    return nnz;
+    job_.vA = &forces_A;
+    job_.vB = &forces_B;
 }
 
-void Simulation::calcJacobianAndSpeedsFromPrev(double baseTime, double newTime, double oldTime)
+void Simulation::set_rescale_job(double baseTime, double newTime, double oldTime)
 {
-    const double factor = (newTime - baseTime) / (oldTime - baseTime);
-    const double dtau = sD->externalStressProtocol->extStress(newTime) - sD->externalStressProtocol->extStress(oldTime);
-    check(d4_jacobian_rescale(ctx_, (int)sD->dc, sD->Ap.data(), sD->Ai.data(), sD->Ax.data(), sD->dVec.data(),
-                              sD->weightFunc, factor, dtau, sD->speed.data()),
-          "jacobian rescale");
-    calculateSparseFormForJacobian();
+    job_ = PassJob();  // calcJacobianAndSpeedsFromPrev, simulation.cpp:546-608
+    job_.kind = PassJob::Rescale;
+    job_.factor = (newTime - baseTime) / (oldTime - baseTime);
+    job_.dtau = sD->externalStressProtocol->extStress(newTime) - sD->externalStressProtocol->extStress(oldTime);
+    job_.speed = &sD->speed;
+}
+
+int PassEngine::run(Simulation& sim, PassJob& job)
+{
+    SimulationData& d = sim.data();
+    PrecisionHandler& pH = sim.precision();
+    d4_ctx* ctx = sim.ctx();
+    const int n = (int)d.dc;
+    switch (job.kind) {
+    case PassJob::Speed:
+        sim.pairPasses++;
+        return d4_speed_pass(ctx, n, &(*job.xy)[0].x, job.tau, job.v->data(), pH.toleranceData(), pH.toleranceCap2());
+    case PassJob::Jacobian: {
+        sim.pairPasses++;
+        if (d.Ai.empty()) {
+            d.Ai.resize((size_t)n * 16 + 16);
+            d.Ax.resize(d.Ai.size());
+        }
+        int nnz = 0;
+        int rc = d4_jacobian_pass(ctx, n, &(*job.xy)[0].x, &job.jp, job.vA->data(), job.vB->data(), d.Ap.data(),
+                                  d.Ai.data(), d.Ax.data(), (int)d.Ai.size(), &nnz, d.indexes.data(), d.dVec.data(),
+                                  pH.toleranceData());
+        if (rc == D4_ERR_CAPACITY) {  // grow (simulation_data.cpp:157-175 grows by dc) and run the pass again
+            d.Ai.resize((size_t)nnz + n);
+            d.Ax.resize(d.Ai.size());
+            rc = d4_jacobian_pass(ctx, n, &(*job.xy)[0].x, &job.jp, job.vA->data(), job.vB->data(), d.Ap.data(),
+                                  d.Ai.data(), d.Ax.data(), (int)d.Ai.size(), &nnz, d.indexes.data(), d.dVec.data(),
+                                  pH.toleranceData());
+        }
+        job.nnz = nnz;
+        return rc;
+    }
+    case PassJob::Rescale:
+        return d4_jacobian_rescale(ctx, n, d.Ap.data(), d.Ai.data(), d.Ax.data(), d.dVec.data(), d.weightFunc,
+                                   job.factor, job.dtau, job.speed->data());
+    default: return D4_OK;
+    }
 }
 
 void Simulation::calculateSparseFormForJacobian()
@@ -296,38 +317,88 @@ bool Simulation::interrupt() const  // simulation.cpp:722-750 (time, strain and 
     return false;
 }
 
-bool Simulation::step()  // one iteration of simulation.cpp:88-277
+void Simulation::begin_prologue()
+{
+    set_speed_job(sD->disl_sorted, sD->initSpeed, sD->simTime);  // simulation.cpp:46
+    phase_ = -1;
+}
+
+bool Simulation::begin_step()  // simulation.cpp:88-94
+{
+    job_ = PassJob();
+    lastAccepted_ = false;
+    if (status_ || interrupt()) return false;
+    SimulationData& d = *sD;
+    t0_ = d.simTime;
+    t1_ = d.simTime + d.stepSize;
+    th_ = d.simTime + d.stepSize / 2;
+    // stage I
+    set_jacobian_job(d.stepSize, d.disl_sorted, d.initSpeed, d.speed, t0_, t1_);
+    phase_ = 1;
+    return true;
+}
+
+void Simulation::after_pass()
 {
     SimulationData& d = *sD;
     const unsigned n = d.dc;
-    const double t0 = d.simTime, t1 = d.simTime + d.stepSize, th = d.simTime + d.stepSize / 2;
-    // stage I
-    calcJacobianAndSpeedsAtTimes(d.stepSize, d.disl_sorted, d.initSpeed, d.speed, t0, t1);
-    for (unsigned i = 0; i < n; i++)
-        d.g[i] = -d.stepSize * ((1 + d.dVec[i]) * d.speed[i] + (1 - d.dVec[i]) * d.initSpeed[i]) / 2;
-    solveEQSys();
-    for (unsigned i = 0; i < n; i++) d.bigStep_sorted[i] = DislwoB(d.disl_sorted[i].x - d.x[i], d.disl_sorted[i].y);
-    calculateSpeedsAtTime(d.bigStep_sorted, d.speed2, t1);
-    calcGSolveAndUpdate(d.bigStep_sorted, d.disl_sorted, d.stepSize, d.speed2, d.initSpeed);
-    // stage II
-    calcJacobianAndSpeedsFromPrev(t0, th, t1);
-    for (unsigned i = 0; i < n; i++)
-        d.g[i] = -d.stepSize / 2 * ((1 + d.dVec[i]) * d.speed[i] + (1 - d.dVec[i]) * d.initSpeed[i]) / 2;
-    solveEQSys();
-    for (unsigned i = 0; i < n; i++)
-        d.firstSmall_sorted[i] = DislwoB(d.disl_sorted[i].x - d.x[i], d.disl_sorted[i].y);
-    calculateSpeedsAtTime(d.firstSmall_sorted, d.speed2, th);
-    calcGSolveAndUpdate(d.firstSmall_sorted, d.disl_sorted, d.stepSize / 2, d.speed2, d.initSpeed);
-    // stage III
-    calcJacobianAndSpeedsAtTimes(d.stepSize / 2, d.firstSmall_sorted, d.initSpeed2, d.speed2, th, t1);
-    for (unsigned i = 0; i < n; i++)
-        d.g[i] = -d.stepSize / 2 * ((1 + d.dVec[i]) * d.speed2[i] + (1 - d.dVec[i]) * d.initSpeed2[i]) / 2;
-    solveEQSys();
-    for (unsigned i = 0; i < n; i++)
-        d.secondSmall_sorted[i] = DislwoB(d.firstSmall_sorted[i].x - d.x[i], d.firstSmall_sorted[i].y);
-    calculateSpeedsAtTime(d.secondSmall_sorted, d.speed2, t1);
-    calcGSolveAndUpdate(d.secondSmall_sorted, d.firstSmall_sorted, d.stepSize / 2, d.speed2, d.initSpeed2);
-    // stage IV
+    const int done = phase_;
+    if (!check(status_ ? status_ : D4_OK, "pass")) {  // an earlier failure ends the step
+        job_ = PassJob();
+        return;
+    }
+    switch (done) {
+    case -1:  // the initial speed pass
+        job_ = PassJob();
+        break;
+    case 1:  // stage I, after the Jacobian pass (simulation.cpp:98-109)
+        calculateSparseFormForJacobian();
+        for (unsigned i = 0; i < n; i++)
+            d.g[i] = -d.stepSize * ((1 + d.dVec[i]) * d.speed[i] + (1 - d.dVec[i]) * d.initSpeed[i]) / 2;
+        solveEQSys();
+        for (unsigned i = 0; i < n; i++) d.bigStep_sorted[i] = DislwoB(d.disl_sorted[i].x - d.x[i], d.disl_sorted[i].y);
+        set_speed_job(d.bigStep_sorted, d.speed2, t1_);
+        break;
+    case 2:  // simulation.cpp:111-113, then stage II's rescale
+        calcGSolveAndUpdate(d.bigStep_sorted, d.disl_sorted, d.stepSize, d.speed2, d.initSpeed);
+        set_rescale_job(t0_, th_, t1_);
+        break;
+    case 3:  // stage II (simulation.cpp:118-130)
+        calculateSparseFormForJacobian();
+        for (unsigned i = 0; i < n; i++)
+            d.g[i] = -d.stepSize / 2 * ((1 + d.dVec[i]) * d.speed[i] + (1 - d.dVec[i]) * d.initSpeed[i]) / 2;
+        solveEQSys();
+        for (unsigned i = 0; i < n; i++)
+            d.firstSmall_sorted[i] = DislwoB(d.disl_sorted[i].x - d.x[i], d.disl_sorted[i].y);
+        set_speed_job(d.firstSmall_sorted, d.speed2, th_);
+        break;
+    case 4:  // simulation.cpp:132-134, then stage III's Jacobian pass
+        calcGSolveAndUpdate(d.firstSmall_sorted, d.disl_sorted, d.stepSize / 2, d.speed2, d.initSpeed);
+        set_jacobian_job(d.stepSize / 2, d.firstSmall_sorted, d.initSpeed2, d.speed2, th_, t1_);
+        break;
+    case 5:  // stage III (simulation.cpp:139-150)
+        calculateSparseFormForJacobian();
+        for (unsigned i = 0; i < n; i++)
+            d.g[i] = -d.stepSize / 2 * ((1 + d.dVec[i]) * d.speed2[i] + (1 - d.dVec[i]) * d.initSpeed2[i]) / 2;
+        solveEQSys();
+        for (unsigned i = 0; i < n; i++)
+            d.secondSmall_sorted[i] = DislwoB(d.firstSmall_sorted[i].x - d.x[i], d.firstSmall_sorted[i].y);
+        set_speed_job(d.secondSmall_sorted, d.speed2, t1_);
+        break;
+    case 6:  // simulation.cpp:152-154 and stage IV
+        calcGSolveAndUpdate(d.secondSmall_sorted, d.firstSmall_sorted, d.stepSize / 2, d.speed2, d.initSpeed2);
+        job_ = PassJob();
+        lastAccepted_ = finish_step();
+        break;
+    default: job_ = PassJob(); break;
+    }
+    phase_ = done < 0 ? 0 : done + 1;
+}
+
+bool Simulation::finish_step()  // stage IV, simulation.cpp:160-276
+{
+    SimulationData& d = *sD;
+    const unsigned n = d.dc;
     calculateXError();
     const bool accept = pH->getMaxErrorRatioSqr() < 1;
     if (accept) {
@@ -358,9 +429,21 @@ bool Simulation::step()  // one iteration of simulation.cpp:88-277
     return accept;
 }
 
+bool Simulation::step()  // one iteration of simulation.cpp:88-277
+{
+    if (!begin_step()) return false;
+    while (job_.kind != PassJob::None) {
+        check(default_engine_.run(*this, job_), "pair pass");
+        after_pass();
+    }
+    return status_ ? false : lastAccepted_;
+}
+
 void Simulation::run_prologue()
 {
-    calculateSpeedsAtTime(sD->disl_sorted, sD->initSpeed, sD->simTime);  // simulation.cpp:46
+    begin_prologue();
+    check(default_engine_.run(*this, job_), "speed pass");
+    after_pass();
 }
 
 void Simulation::run()
@@ -381,10 +464,12 @@ struct d4_sim {
     std::string err;
 };
 
-static int sim_setup(d4_ctx* ctx, std::shared_ptr<sdddstCore::SimulationData> sd, const d4_sim_params* p, d4_sim** out)
+namespace sdddstCore {
+// the parameters of a run as project_parser.cpp stores them in SimulationData
+int apply_sim_params(SimulationData& d, const d4_sim_params* p)
 {
-    if (!sd->error.empty()) return D4_ERR_INVALID;
-    sdddstCore::SimulationData& d = *sd;
+    if (p->wfunc != 'c' && p->wfunc != 'p' && p->wfunc != 'm') return D4_ERR_INVALID;
+    if (p->stress_kind < 0 || p->stress_kind > 3) return D4_ERR_INVALID;
     d.initStepSize = d.stepSize = p->init_step_size;
     d.isMaxStepSizeLimit = p->max_step_size > 0;
     d.maxStepSizeLimit = p->max_step_size;
@@ -402,18 +487,24 @@ static int sim_setup(d4_ctx* ctx, std::shared_ptr<sdddstCore::SimulationData> sd
     d.isSpeedThresholdForCutoffChange = p->speed_threshold > 0;
     d.speedThresholdForCutoffChange = p->speed_threshold;
     switch (p->stress_kind) {  // project_parser.cpp:287-326
-    case 1: d.externalStressProtocol.reset(new sdddstCore::FixedRateProtocol(p->stress_init, p->stress_rate)); break;
-    case 2:
-        d.externalStressProtocol.reset(
-            new sdddstCore::CyclicLoadProtocol(p->stress_init, p->stress_rate, p->stress_period));
-        break;
+    case 1: d.externalStressProtocol.reset(new FixedRateProtocol(p->stress_init, p->stress_rate)); break;
+    case 2: d.externalStressProtocol.reset(new CyclicLoadProtocol(p->stress_init, p->stress_rate, p->stress_period)); break;
     case 3:
-        d.externalStressProtocol.reset(new sdddstCore::IncreasingCyclicLoadProtocol(
-            p->stress_init, p->stress_rate, p->stress_period, p->stress_amp));
+        d.externalStressProtocol.reset(
+            new IncreasingCyclicLoadProtocol(p->stress_init, p->stress_rate, p->stress_period, p->stress_amp));
         break;
-    default: d.externalStressProtocol.reset(new sdddstCore::StressProtocol(p->stress_init)); break;
+    default: d.externalStressProtocol.reset(new StressProtocol(p->stress_init)); break;
     }
     d.updateCutOff();
+    return D4_OK;
+}
+}  // namespace sdddstCore
+
+static int sim_setup(d4_ctx* ctx, std::shared_ptr<sdddstCore::SimulationData> sd, const d4_sim_params* p, d4_sim** out)
+{
+    if (!sd->error.empty()) return D4_ERR_INVALID;
+    const int rc = sdddstCore::apply_sim_params(*sd, p);
+    if (rc) return rc;
     d4_sim* s = new d4_sim();
     s->data = sd;
     s->sim.reset(new sdddstCore::Simulation(sd, ctx));
