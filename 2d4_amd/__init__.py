@@ -51,6 +51,7 @@ SIGNATURES = {
     "d4_ctx_create": (C.c_int, [C.c_int, _P(_vp)]),
     "d4_ctx_destroy": (None, [_vp]),
     "d4_last_error": (C.c_char_p, [_vp]),
+    "d4_ctx_device": (C.c_int, [_vp]),
     "d4_ctx_set_stream": (C.c_int, [_vp, _vp]),
     "d4_ctx_synchronize": (C.c_int, [_vp]),
     "d4_ctx_set_profiling": (C.c_int, [_vp, C.c_int]),

@@ -13,7 +13,8 @@ class SimParams(C.Structure):
     _fields_ = [("init_step_size", _d), ("max_step_size", _d), ("prec", _d), ("dipole_prec", _d),
                 ("cutoff_mult", _d), ("heaviside", C.c_int), ("wfunc", C.c_char), ("stress_kind", C.c_int),
                 ("stress_init", _d), ("stress_rate", _d), ("stress_period", _d), ("stress_amp", _d),
-                ("time_limit", _d), ("step_limit", C.c_int64), ("strain_limit", _d), ("speed_threshold", _d)]
+                ("time_limit", _d), ("step_limit", C.c_int64), ("strain_limit", _d), ("speed_threshold", _d),
+                ("solver", C.c_int)]
 
 
 class SimStats(C.Structure):

@@ -20,7 +20,7 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-mcode-obje
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wall", "-Wno-unused-command-line-argument"]
 
-SOURCES = ["d4_kernels.hip", "d4_capi.hip", "d4_batch.hip", "d4_sim.cpp", "d4_solver.cpp"]
+SOURCES = ["d4_kernels.hip", "d4_capi.hip", "d4_batch.hip", "d4_gpulu.hip", "d4_sim.cpp", "d4_solver.cpp"]
 
 
 def _hipcc():
@@ -45,7 +45,10 @@ def build_lib(verbose=False):
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = LIB + ".tmp"
-    cmd = [_hipcc()] + HIPFLAGS + ["-shared", "-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs
+    # rocSOLVER / rocBLAS (dense LU on the GPU): with torch loaded first the
+    # dynamic linker binds these sonames to the copies torch already holds
+    libs = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
+    cmd = [_hipcc()] + HIPFLAGS + ["-shared", "-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs + libs
     _run(cmd, verbose)
     os.replace(tmp, LIB)
     return LIB

@@ -228,6 +228,8 @@ void d4_ctx_destroy(d4_ctx* ctx)
 
 const char* d4_last_error(const d4_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int d4_ctx_device(const d4_ctx* ctx) { return ctx ? ctx->device : -1; }
+
 int d4_ctx_set_stream(d4_ctx* ctx, void* stream)
 {
     if (!ctx) return D4_ERR_INVALID;

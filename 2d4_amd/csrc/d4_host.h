@@ -131,6 +131,7 @@ public:
     std::unique_ptr<StressProtocol> externalStressProtocol;
     double speedThresholdForCutoffChange = 0;
     bool isSpeedThresholdForCutoffChange = false;
+    int solverKind = 0;  // 0 host sparse LU, 1 GPU dense LU
     std::string error;  // set when the input is rejected (the reference prints and exits)
 
 private:
@@ -194,7 +195,7 @@ private:
     std::shared_ptr<SimulationData> sD;
     std::unique_ptr<PrecisionHandler> pH;
     d4_ctx* ctx_;
-    d4host::SparseLU lu_;
+    std::unique_ptr<d4host::LinearSolver> lu_;
     PassEngine default_engine_;
     int status_ = 0;
     double lastRatio_ = 0;

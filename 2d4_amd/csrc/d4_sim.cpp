@@ -181,6 +181,14 @@ Simulation::Simulation(std::shared_ptr<SimulationData> s, d4_ctx* ctx) : sD(std:
     pH->setDipolePrecisity(sD->dipole_prec);
     pH->setSize(sD->dc);
     sD->tau.bind(ctx);
+    if (sD->solverKind == 1) {
+        lu_.reset(d4host::new_gpu_dense_lu(d4_ctx_device(ctx)));
+        if (!lu_) {
+            status_ = D4_ERR_HIP;
+            error_ = "GPU dense LU unavailable";
+        }
+    }
+    if (!lu_) lu_.reset(new d4host::SparseLU());
 }
 
 bool Simulation::check(int rc, const char* where)
@@ -271,13 +279,14 @@ int PassEngine::run(Simulation& sim, PassJob& job)
 void Simulation::calculateSparseFormForJacobian()
 {
     factorizations++;
-    lu_.factor((int)sD->dc, sD->Ap.data(), sD->Ai.data(), sD->Ax.data());
+    if (lu_->factor((int)sD->dc, sD->Ap.data(), sD->Ai.data(), sD->Ax.data()) < 0)
+        check(D4_ERR_HIP, "linear solver factorization");
 }
 
 void Simulation::solveEQSys()  // simulation.cpp:643-659
 {
-    lu_.solve(sD->g.data(), sD->x.data());
-    if (lu_.singular()) {
+    if (lu_->solve(sD->g.data(), sD->x.data()) < 0) check(D4_ERR_HIP, "linear solve");
+    if (lu_->singular()) {
         singularSolves++;
         for (unsigned i = 0; i < sD->dc; ++i)
             if (!((-1 < sD->x[i]) && (sD->x[i] < 1))) sD->x[i] = 0;
@@ -486,6 +495,8 @@ int apply_sim_params(SimulationData& d, const d4_sim_params* p)
     d.totalAccumulatedStrainIncreaseLimit = p->strain_limit;
     d.isSpeedThresholdForCutoffChange = p->speed_threshold > 0;
     d.speedThresholdForCutoffChange = p->speed_threshold;
+    if (p->solver < 0 || p->solver > 1) return D4_ERR_INVALID;
+    d.solverKind = p->solver;
     switch (p->stress_kind) {  // project_parser.cpp:287-326
     case 1: d.externalStressProtocol.reset(new FixedRateProtocol(p->stress_init, p->stress_rate)); break;
     case 2: d.externalStressProtocol.reset(new CyclicLoadProtocol(p->stress_init, p->stress_rate, p->stress_period)); break;

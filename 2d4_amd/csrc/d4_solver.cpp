@@ -220,7 +220,7 @@ int SparseLU::factor(int n, const int* Ap, const int* Ai, const double* Ax)
     return singular_ ? 1 : 0;
 }
 
-void SparseLU::solve(const double* b, double* xout) const
+void SparseLU::solve_const(const double* b, double* xout) const
 {
     const int n = n_;
     if (dense_) {

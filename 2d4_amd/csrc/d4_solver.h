@@ -22,13 +22,29 @@
 
 namespace d4host {
 
-class SparseLU {
+// factor a square CSC matrix once, then solve A x = b any number of times
+class LinearSolver {
+public:
+    virtual ~LinearSolver() = default;
+    // 0 = ok, 1 = singular (some zero pivot), < 0 = failure
+    virtual int factor(int n, const int* Ap, const int* Ai, const double* Ax) = 0;
+    // x = A^{-1} b; 0 = ok, < 0 = failure
+    virtual int solve(const double* b, double* x) = 0;
+    virtual bool singular() const = 0;
+};
+
+class SparseLU : public LinearSolver {
 public:
     // returns 0 = ok, 1 = singular (some zero pivot), <0 = invalid input
-    int factor(int n, const int* Ap, const int* Ai, const double* Ax);
+    int factor(int n, const int* Ap, const int* Ai, const double* Ax) override;
     // x = A^{-1} b (b and x may alias)
-    void solve(const double* b, double* x) const;
-    bool singular() const { return singular_; }
+    int solve(const double* b, double* x) override
+    {
+        solve_const(b, x);
+        return 0;
+    }
+    void solve_const(const double* b, double* x) const;
+    bool singular() const override { return singular_; }
     int64_t fill() const { return (int64_t)Li_.size() + (int64_t)Ui_.size(); }
 
 private:
@@ -48,5 +64,11 @@ private:
 
 // minimum degree order of the symmetric pattern of A + A^T (exposed for tests)
 std::vector<int> min_degree_order(int n, const int* Ap, const int* Ai);
+
+// Dense LU with partial pivoting on the GPU (rocSOLVER getrf / getrs): the CSC
+// matrix is scattered into an n x n device matrix, factored there and solved
+// there; one stream and rocBLAS handle per calling thread.  Defined in
+// d4_gpulu.hip; null if the device cannot be used.
+LinearSolver* new_gpu_dense_lu(int device);
 
 }  // namespace d4host

@@ -77,6 +77,65 @@ def cpu_baseline(n_single, n_omp, min_seconds):
     return out
 
 
+def ensemble_bench(pkg, ctx, steps, R, seed=4242):
+    """Batched ensemble vs the plain one-realization-at-a-time GPU path on the
+    same realizations, same number of lockstep iterations, same solver.
+
+    Workload: R realizations, N_r = 2 U{128..2048} (even N in [256, 4096]),
+    uniform positions, b alternating, heaviside cutoff 0.05 (-u 0.05 sqrt(N) -h),
+    tau = 0.05 (-r 0 -s 0.05), default precision control, linear solves as a
+    dense LU on the GPU.  A step is one loop iteration (stages I-IV) of one
+    realization; the first iteration (with run()'s initial speed pass) is
+    untimed in both paths."""
+    ens = importlib.import_module("2d4_amd.ensemble")
+    simmod = importlib.import_module("2d4_amd.sim")
+    rng = np.random.default_rng(seed)
+    sizes = 2 * rng.integers(128, 2049, R)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    T = int(off[-1])
+    x = rng.uniform(-0.5, 0.5, T)
+    y = rng.uniform(-0.5, 0.5, T)
+    b = np.concatenate([np.where(np.arange(n) % 2 == 0, -1.0, 1.0) for n in sizes])
+    params = [dict(cutoff_mult=0.05 * float(np.sqrt(n)), heaviside=True, stress=("fixed", 0.05, 0.0), solver=1)
+              for n in sizes]
+    log(f"[bench] ensemble R={R} sum N={T}: batched")
+    e = ens.Ensemble(ctx, off, x, y, b, params)
+    e.step(1)
+    cfg, tune_ms = e.autotune()
+    log(f"[bench] ensemble autotuned {cfg} ({tune_ms:.2f} ms per speed+Jacobian pass)")
+    t_before = e.timing()
+    t0 = time.perf_counter()
+    e.step(steps)
+    t_batched = time.perf_counter() - t0
+    t_after = e.timing()
+    pos_b = e.positions()
+    st_b = e.stats()
+    e.close()
+    log("[bench] ensemble: plain path, one realization at a time")
+    t_plain = 0.0
+    equal = True
+    for r in range(R):
+        sl = slice(off[r], off[r + 1])
+        sim = simmod.Simulation.from_arrays(ctx, x[sl], y[sl], b[sl], **params[r])
+        sim.step()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.step()
+        t_plain += time.perf_counter() - t0
+        equal &= bool(np.array_equal(sim.positions(), pos_b[sl]))
+        sim.close()
+    rs = R * steps
+    return {"metric": "ensemble steps/s (one loop iteration of one realization = one step), batched vs plain",
+            "batched_steps_per_s": rs / t_batched, "plain_steps_per_s": rs / t_plain,
+            "ratio": t_plain / t_batched, "R": R, "sum_n": T, "n_min": int(sizes.min()), "n_max": int(sizes.max()),
+            "steps_timed": steps, "batched_s": t_batched, "plain_s": t_plain,
+            "batched_split_ms": {k: t_after[k] - t_before[k] for k in t_after},
+            "accepted": int(sum(s["successful_steps"] for s in st_b)),
+            "failed": int(sum(s["failed_steps"] for s in st_b)),
+            "launch_config": cfg, "solver": "dense LU on the GPU (rocSOLVER getrf/getrs), both paths",
+            "bitwise_equal_to_plain": equal}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -96,6 +155,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=6.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the speed-only and scaled-matrix side measurements")
+    ap.add_argument("--ens-steps", type=int, default=2, help="timed lockstep iterations of the ensemble")
+    ap.add_argument("--ens-r", type=int, default=256, help="realizations in the ensemble")
+    ap.add_argument("--no-ens", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -174,6 +236,9 @@ def main():
                                            pair_evals_per_s=pairs / ((t1 - t0) / args.steps))
     del M
     torch.cuda.empty_cache()
+    ensemble = None
+    if not args.no_ens:
+        ensemble = ensemble_bench(pkg, ctx, args.ens_steps, args.ens_r)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_pair_dense_n16384.json")
@@ -194,7 +259,7 @@ def main():
                "single_thread_value": c["1core"]["pair_evals_per_s"]}
 
     line = {
-        "metric": "FP64 pair evals/s (sigma + d sigma) at N=16384; dense Jacobian + speed pass",
+        "metric": "FP64 pair evals/s (\u03c3+\u2202\u03c3) at N=16384; ensemble steps/s batched vs plain",
         "value": value,
         "unit": "pair evals/s",
         "n_gpus": 1,
@@ -214,6 +279,7 @@ def main():
                      "traffic": traffic, "kernel": "pair_pass_kernel<kModeJacDense>",
                      "kernel_avg_ms": k_avg_ms, "ops_per_launch": OPS_JAC_PAIR * unordered},
         "cpu_baseline": cpu,
+        "ensemble": ensemble,
         "extra": extra,
     }
     print(json.dumps(line), flush=True)

@@ -41,6 +41,8 @@ typedef struct d4_ctx d4_ctx;
 int d4_ctx_create(int device, d4_ctx** out);
 void d4_ctx_destroy(d4_ctx* ctx);
 const char* d4_last_error(const d4_ctx* ctx);
+/* the HIP device the context runs on (no reference counterpart) */
+int d4_ctx_device(const d4_ctx* ctx);
 /* hipStream_t passed as void*; NULL restores the context's own stream */
 int d4_ctx_set_stream(d4_ctx* ctx, void* stream);
 int d4_ctx_synchronize(d4_ctx* ctx);

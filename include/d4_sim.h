@@ -35,6 +35,8 @@ typedef struct {
     int64_t step_limit;      /* -C; < 0: none (stop once successful steps exceed it) */
     double strain_limit;     /* <= 0: none */
     double speed_threshold;  /* cutoff change threshold (simulation.cpp:215-219); <= 0: none */
+    int solver;              /* linear solves (UMFPACK's place, simulation.cpp:635-659):
+                                0 host sparse LU, 1 dense LU on the GPU (rocSOLVER getrf/getrs) */
 } d4_sim_params;
 
 typedef struct {

@@ -8,10 +8,12 @@
 
 #include "../../include/d4.h"
 #include "../../oracle/d4_oracle.h"
+#include "../../2d4_amd/csrc/d4_solver.h"
 
 extern "C" {
 
 const char* d4_last_error(const d4_ctx*) { return "oracle shim"; }
+int d4_ctx_device(const d4_ctx*) { return 0; }
 
 int d4_field_eval(d4_ctx*, int64_t n, const double* dx, const double* dy, const double* e2pix, const double* c2piy,
                   double* sxy, double* dsxy, int flags)
@@ -60,3 +62,6 @@ int d4_jacobian_rescale(d4_ctx*, int n, const int* Ap, const int* Ai, double* Ax
 }
 
 }  // extern "C"
+
+// the GPU dense LU is not part of this CPU build
+d4host::LinearSolver* d4host::new_gpu_dense_lu(int) { return nullptr; }

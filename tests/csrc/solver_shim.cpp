@@ -8,7 +8,7 @@ extern "C" int ts_solve(int n, const int* Ap, const int* Ai, const double* Ax, c
     d4host::SparseLU lu;
     int rc = lu.factor(n, Ap, Ai, Ax);
     if (rc < 0) return rc;
-    lu.solve(b, x);
+    lu.solve_const(b, x);
     if (fill) *fill = lu.fill();
     return rc;
 }

@@ -122,6 +122,16 @@ def test_ensemble_equals_plain_path(ctx):
     e.close()
 
 
+def test_ensemble_gpu_dense_solver_equals_plain_path(ctx):
+    sizes = [64, 300, 130, 96]
+    off, x, y, b = _ens_inputs(sizes, 7)
+    params = [dict(p, solver=1) for p in _params(sizes)]
+    e = ens.Ensemble(ctx, off, x, y, b, params)
+    e.step(4)
+    _check_equal(e, off, _plain(ctx, off, x, y, b, params, 4))
+    e.close()
+
+
 def test_ensemble_launch_shapes_do_not_change_results(ctx):
     sizes = [70, 190, 64, 128]
     off, x, y, b = _ens_inputs(sizes, 9)

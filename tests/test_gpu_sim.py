@@ -149,3 +149,36 @@ def test_sim_from_file_matches_arrays(ctx, tmp_path):
     back = dconf.read_dconf(out)
     np.testing.assert_array_equal(back[:, 2], xyb[:, 2])
     assert dconf.conf_compare(back, np.column_stack([a.positions(True), xyb[:, 2]]))["max_diff"] <= 1e-13
+
+
+def test_gpu_dense_solver_each_step_c2(ctx, orc):
+    # the same per-step parity with the linear solves as a dense LU on the GPU
+    x, y, b = _random(1024, 2024)
+    state = _sorted_xy(x, y, b)
+    bs = np.where(np.arange(len(state)) < len(state) // 2, 1.0, -1.0)
+    h = 1e-6
+    worst = 0.0
+    for _ in range(8):
+        one = dict(C2, init_step_size=h, step_limit=0, solver=1)
+        nxt, rst = orc.run(state, **_oracle_kw({k: v for k, v in one.items() if k != "solver"}))
+        s = simmod.Simulation.from_arrays(ctx, state[:, 0], state[:, 1], bs, **one)
+        st = s.run()
+        assert (st["successful_steps"], st["failed_steps"]) == (rst.ok_steps, rst.failed_steps)
+        worst = max(worst, dconf.max_periodic_dx(s.positions(), nxt))
+        s.close()
+        state, h = nxt, rst.step_size
+    print(f"worst single-step deviation (GPU dense LU) {worst:.3e}", flush=True)
+    assert worst <= TRAJ_TOL
+
+
+def test_gpu_dense_solver_adaptive_run(ctx, orc):
+    x, y, b = _random(256, 5)
+    kw = dict(cutoff_mult=0.9, wfunc="p", stress=("cyclic", 0.05, 2.0, 0.01), prec=1e-6, dipole_prec=0.1,
+              step_limit=40, max_step_size=1e-3)
+    s = simmod.Simulation.from_arrays(ctx, x, y, b, solver=1, **kw)
+    st = s.run()
+    ref, rst = orc.run(_sorted_xy(x, y, b), **_oracle_kw(kw))
+    dev = dconf.max_periodic_dx(s.positions(), ref)
+    print(f"steps {st['successful_steps']}+{st['failed_steps']} max periodic dx {dev:.3e}", flush=True)
+    assert (st["successful_steps"], st["failed_steps"]) == (rst.ok_steps, rst.failed_steps)
+    assert dev <= TRAJ_TOL
