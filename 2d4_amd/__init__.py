@@ -43,7 +43,8 @@ class D4Error(RuntimeError):
 class JacParams(C.Structure):
     """d4_jac_params (include/d4.h); fields as SimulationData names them (simulation_data.h:181-197)."""
     _fields_ = [("h", _d), ("tau_a", _d), ("tau_b", _d), ("cutoff", _d), ("cut_sq", _d),
-                ("one_per_cut_sq", _d), ("heaviside", C.c_int), ("wfunc", C.c_char), ("tol_cap2", _d)]
+                ("one_per_cut_sq", _d), ("heaviside", C.c_int), ("wfunc", C.c_char), ("tol_cap2", _d),
+                ("exact", C.c_int)]
 
 
 # symbol -> (restype, argtypes); the set include/d4.h declares
@@ -53,6 +54,7 @@ SIGNATURES = {
     "d4_last_error": (C.c_char_p, [_vp]),
     "d4_ctx_device": (C.c_int, [_vp]),
     "d4_ctx_set_stream": (C.c_int, [_vp, _vp]),
+    "d4_ctx_use_default_stream": (C.c_int, [_vp]),
     "d4_ctx_synchronize": (C.c_int, [_vp]),
     "d4_ctx_set_profiling": (C.c_int, [_vp, C.c_int]),
     "d4_ctx_pair_kernel_time": (C.c_int, [_vp, _pd, _pi]),
@@ -141,8 +143,15 @@ class Context:
         self.check(self.L.d4_ctx_set_stream(self.h, C.c_void_p(stream_handle) if stream_handle else None))
 
     def use_torch_stream(self):
+        """Orders the context's work on torch's current stream (its default stream
+        is the device's legacy NULL stream, which a non-blocking stream would not
+        wait for)."""
         import torch
-        self.set_stream(torch.cuda.current_stream().cuda_stream)
+        h = torch.cuda.current_stream().cuda_stream
+        if h:
+            self.set_stream(h)
+        else:
+            self.check(self.L.d4_ctx_use_default_stream(self.h))
 
     def synchronize(self):
         self.check(self.L.d4_ctx_synchronize(self.h))
@@ -263,7 +272,7 @@ class Context:
 
 
 def jac_params(h=0.0, tau_a=0.0, tau_b=0.0, n=None, cutoff_mult=None, heaviside=False, wfunc="c", tol_cap2=0.0,
-               cutoff=None):
+               cutoff=None, exact=False):
     """Builds d4_jac_params; cutoff from the multiplier as SimulationData::updateCutOff (simulation_data.cpp:253-262)."""
     if cutoff is None:
         if cutoff_mult is None:
@@ -275,7 +284,8 @@ def jac_params(h=0.0, tau_a=0.0, tau_b=0.0, n=None, cutoff_mult=None, heaviside=
             cutoff = m / np.sqrt(n)
     cut_sq = cutoff * cutoff
     return JacParams(h=h, tau_a=tau_a, tau_b=tau_b, cutoff=cutoff, cut_sq=cut_sq, one_per_cut_sq=1.0 / cut_sq,
-                     heaviside=1 if heaviside else 0, wfunc=wfunc.encode(), tol_cap2=tol_cap2)
+                     heaviside=1 if heaviside else 0, wfunc=wfunc.encode(), tol_cap2=tol_cap2,
+                     exact=1 if exact else 0)
 
 
 _default_ctx = None

@@ -237,6 +237,13 @@ int d4_ctx_set_stream(d4_ctx* ctx, void* stream)
     return D4_OK;
 }
 
+int d4_ctx_use_default_stream(d4_ctx* ctx)
+{
+    if (!ctx) return D4_ERR_INVALID;
+    ctx->stream = nullptr;
+    return D4_OK;
+}
+
 int d4_ctx_synchronize(d4_ctx* ctx)
 {
     if (!ctx) return D4_ERR_INVALID;
@@ -372,7 +379,8 @@ int d4_jacobian_dense_dev(d4_ctx* ctx, int n, const double* xy, const d4_jac_par
     if (n == 0) return D4_OK;
     RealParams rp = real_params(n, p, 0.0);
     if (!tol) rp.tol_cap2 = 0.0;
-    return ctx->single_pass(kModeJacDense, n, xy, rp, tol, vA, vB, nullptr, dvec, M, ld, nullptr, 0, 0, 1);
+    return ctx->single_pass(kModeJacDense, n, xy, rp, tol, vA, vB, nullptr, dvec, M, ld, nullptr, p->exact ? 1 : 0, 0,
+                            1);
 }
 
 int d4_jacobian_dense_scale_dev(d4_ctx* ctx, int n, double* M, int64_t ld, const double* dvec)
@@ -450,7 +458,7 @@ int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* 
         if (attempt == 1 && dtol) D4_TRY(ctx->h2d(dtol, tol, b, "tolerance upload"));
         D4_TRY(ctx->single_pass(kModeJacCoo, n, static_cast<double*>(ctx->xy.p), rp, dtol,
                                 static_cast<double*>(ctx->v0.p), static_cast<double*>(ctx->v1.p), nullptr, nullptr,
-                                nullptr, 0, nullptr, 0, 0, 1));
+                                nullptr, 0, nullptr, p->exact ? 1 : 0, 0, 1));
         D4_TRY(ctx->d2h(&ncoo, ctx->coo_count.p, sizeof(int), "pair count"));
         D4_TRY(ctx->sync("jacobian pass"));
         if ((size_t)ncoo <= ctx->coo.bytes / sizeof(CooEntry)) break;

@@ -45,6 +45,9 @@ const char* d4_last_error(const d4_ctx* ctx);
 int d4_ctx_device(const d4_ctx* ctx);
 /* hipStream_t passed as void*; NULL restores the context's own stream */
 int d4_ctx_set_stream(d4_ctx* ctx, void* stream);
+/* order the context's work on the device's legacy default (NULL) stream, the
+ * one a caller's default-stream work (torch's default stream) runs on */
+int d4_ctx_use_default_stream(d4_ctx* ctx);
 int d4_ctx_synchronize(d4_ctx* ctx);
 /* Kernel timing (no reference counterpart): when on, HIP events recorded on
  * the context's stream bracket every pair-kernel launch; d4_ctx_pair_kernel_time
@@ -88,6 +91,8 @@ typedef struct {
     int heaviside;         /* heavisideCutoff */
     char wfunc;            /* weightFunc: 'c', 'p' or 'm' */
     double tol_cap2;       /* PrecisionHandler minPrecisitySq if the dipole criterion is on, else 0 */
+    int exact;             /* 0: the reference's d sigma (Field.cpp:314-331); 1: the true derivative
+                              (no reference counterpart; for the adjoint's forward map) */
 } d4_jac_params;
 
 /* Jacobian pass, replaces Simulation::calcJacobianAndSpeedsAtTimes
