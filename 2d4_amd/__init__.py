@@ -73,6 +73,11 @@ SIGNATURES = {
     "d4_jacobian_dense_scale_dev": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, _vp]),
     "d4_jacobian_dense": (C.c_int, [_vp, C.c_int, _vp, _P(JacParams), _vp, _vp, _vp, _vp, _vp, C.c_int]),
     "d4_vjp_dev": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _pd, C.c_int]),
+    "d4_table_build_dev": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "d4_table_eval_dev": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, _vp, C.c_int, _vp, _vp]),
+    "d4_speed_pass_table_dev": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, _d, _vp]),
+    "d4_jacobian_dense_table_dev": (C.c_int, [_vp, C.c_int, _vp, _P(JacParams), _vp, _vp, C.c_int, _vp, _vp, _vp,
+                                              C.c_int64, _vp]),
 }
 
 _lib = None
@@ -269,6 +274,61 @@ class Context:
         self.check(self.L.d4_vjp_dev(self.h, n, xy.data_ptr(), w.data_ptr(), gx.data_ptr(),
                                      gy.data_ptr() if gy is not None else None, C.byref(gt), 1 if exact else 0))
         return gx, gy, gt.value
+
+
+def _tp(t):
+    return t.data_ptr() if t is not None else None
+
+
+def _table_methods():
+    """Tabulated-field calls on device tensors (added to Context below)."""
+
+    def table_build_t(self, n=1024, device="cuda:0", with_derivative=True):
+        """(tab_s, tab_d): n x n node tables of Field::xy and xy_diff_x over [-1/2, 1/2)^2."""
+        import torch
+        tab_s = torch.empty((n, n), dtype=torch.float64, device=device)
+        tab_d = torch.empty((n, n), dtype=torch.float64, device=device) if with_derivative else None
+        self.use_torch_stream()
+        self.check(self.L.d4_table_build_dev(self.h, n, tab_s.data_ptr(), _tp(tab_d)))
+        return tab_s, tab_d
+
+    def table_eval_t(self, dx, dy, tab_s, tab_d=None):
+        import torch
+        m = dx.numel()
+        s = torch.empty(m, dtype=torch.float64, device=dx.device)
+        d = torch.empty(m, dtype=torch.float64, device=dx.device) if tab_d is not None else None
+        self.use_torch_stream()
+        self.check(self.L.d4_table_eval_dev(self.h, m, dx.data_ptr(), dy.data_ptr(), tab_s.data_ptr(), _tp(tab_d),
+                                            tab_s.shape[0], s.data_ptr(), _tp(d)))
+        return s, d
+
+    def speed_pass_table_t(self, xy, tab_s, tau=0.0, out=None):
+        import torch
+        n = xy.shape[0]
+        v = out if out is not None else torch.empty(n, dtype=torch.float64, device=xy.device)
+        self.use_torch_stream()
+        self.check(self.L.d4_speed_pass_table_dev(self.h, n, xy.data_ptr(), tab_s.data_ptr(), tab_s.shape[0],
+                                                  float(tau), v.data_ptr()))
+        return v
+
+    def jacobian_dense_table_t(self, xy, params, tab_s, tab_d, M=None):
+        import torch
+        n = xy.shape[0]
+        dev = xy.device
+        vA, vB, d = (torch.empty(n, dtype=torch.float64, device=dev) for _ in range(3))
+        if M is None:
+            M = torch.empty((n, n), dtype=torch.float64, device=dev)  # M[col, row]
+        self.use_torch_stream()
+        self.check(self.L.d4_jacobian_dense_table_dev(self.h, n, xy.data_ptr(), C.byref(params), tab_s.data_ptr(),
+                                                      tab_d.data_ptr(), tab_s.shape[0], vA.data_ptr(),
+                                                      vB.data_ptr(), M.data_ptr(), n, d.data_ptr()))
+        return vA, vB, M, d
+
+    return [table_build_t, table_eval_t, speed_pass_table_t, jacobian_dense_table_t]
+
+
+for _f in _table_methods():
+    setattr(Context, _f.__name__, _f)
 
 
 def jac_params(h=0.0, tau_a=0.0, tau_b=0.0, n=None, cutoff_mult=None, heaviside=False, wfunc="c", tol_cap2=0.0,

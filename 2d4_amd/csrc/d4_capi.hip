@@ -133,6 +133,9 @@ int d4_ctx::single_pass(int mode, int n, const double* d_xy, const RealParams& r
     a.coo_count = static_cast<int*>(coo_count.p);
     a.coo_cap = (int)(coo.bytes / sizeof(CooEntry));
     a.w = d_w;
+    a.tab_s = tab_s;
+    a.tab_d = tab_d;
+    a.tab_n = tab_n;
     a.exact = exact;
     a.want_dy = want_dy;
     hipEvent_t ev = nullptr;
@@ -562,6 +565,59 @@ int d4_vjp_dev(d4_ctx* ctx, int n, const double* xy, const double* w, double* gx
         *gtau = s;
     }
     return D4_OK;
+}
+
+static bool table_size_ok(int n) { return n >= 2 && n <= (1 << 14) && (n & (n - 1)) == 0; }
+
+int d4_table_build_dev(d4_ctx* ctx, int n, double* tab_s, double* tab_d)
+{
+    if (!ctx || !tab_s || !table_size_ok(n)) return D4_ERR_INVALID;
+    hipError_t e = launch_table_build(n, tab_s, tab_d, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "table build");
+    return D4_OK;
+}
+
+int d4_table_eval_dev(d4_ctx* ctx, int64_t m, const double* dx, const double* dy, const double* tab_s,
+                      const double* tab_d, int n, double* s, double* d)
+{
+    if (!ctx || m < 0 || !table_size_ok(n) || (m > 0 && (!dx || !dy)) || (s && !tab_s) || (d && !tab_d))
+        return D4_ERR_INVALID;
+    hipError_t e = launch_table_eval(m, dx, dy, tab_s, tab_d, n, s, d, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "table eval");
+    return D4_OK;
+}
+
+int d4_speed_pass_table_dev(d4_ctx* ctx, int n, const double* xy, const double* tab_s, int tab_n, double tau,
+                            double* v)
+{
+    if (!ctx || n < 0 || !table_size_ok(tab_n) || (n > 0 && (!xy || !v || !tab_s))) return D4_ERR_INVALID;
+    if (n == 0) return D4_OK;
+    ctx->tab_s = tab_s;
+    ctx->tab_d = nullptr;
+    ctx->tab_n = tab_n;
+    RealParams rp = real_params(n, nullptr, tau);
+    const int rc = ctx->single_pass(kModeSpeedTab, n, xy, rp, nullptr, v, nullptr, nullptr, nullptr, nullptr, 0,
+                                    nullptr, 0, 0, 1);
+    ctx->tab_s = nullptr;
+    return rc;
+}
+
+int d4_jacobian_dense_table_dev(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, const double* tab_s,
+                                const double* tab_d, int tab_n, double* vA, double* vB, double* M, int64_t ld,
+                                double* dvec)
+{
+    if (!ctx || !p || n < 0 || !table_size_ok(tab_n) || (n > 0 && (!xy || !vA || !M || ld < n || !tab_s || !tab_d)))
+        return D4_ERR_INVALID;
+    if (n == 0) return D4_OK;
+    ctx->tab_s = tab_s;
+    ctx->tab_d = tab_d;
+    ctx->tab_n = tab_n;
+    RealParams rp = real_params(n, p, 0.0);
+    rp.tol_cap2 = 0.0;
+    const int rc = ctx->single_pass(kModeJacDenseTab, n, xy, rp, nullptr, vA, vB, nullptr, dvec, M, ld, nullptr, 0,
+                                    0, 1);
+    ctx->tab_s = ctx->tab_d = nullptr;
+    return rc;
 }
 
 }  // extern "C"

@@ -65,6 +65,10 @@ struct d4_ctx {
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_marks;
     BatchPasses* batch = nullptr;  // tables and buffers of d4_batch_* (d4_batch.hip)
+    // tabulated field for the table pass modes (set around single_pass)
+    const double* tab_s = nullptr;
+    const double* tab_d = nullptr;
+    int tab_n = 0;
     hipEvent_t take_event();
     int mark_begin(hipEvent_t* e);
     int mark_end(hipEvent_t b);

@@ -424,4 +424,23 @@ D4_HD double weight_inv(double w, char kind)
     return 0;
 }
 
+// Bilinear interpolation in an n x n table of node values over the periodic
+// cell [-1/2, 1/2)^2 (node (kx, ky) at (-1/2 + kx/n, -1/2 + ky/n), row ky,
+// column kx; n a power of two).  dx, dy are wrapped differences; the same
+// operations in the same order on host and device, without contraction, so a
+// host build of this function gives bit-identical values.
+D4_HD inline double table_bilinear(const double* tab, int n, double dx, double dy)
+{
+#pragma clang fp contract(off)
+    const double u = (dx + 0.5) * n;
+    const double w = (dy + 0.5) * n;
+    const double fu = floor(u), fw = floor(w);
+    const double ax = u - fu, ay = w - fw;
+    const int i0 = ((int)fu) & (n - 1), j0 = ((int)fw) & (n - 1);
+    const int i1 = (i0 + 1) & (n - 1), j1 = (j0 + 1) & (n - 1);
+    const double t00 = tab[(size_t)j0 * n + i0], t10 = tab[(size_t)j0 * n + i1];
+    const double t01 = tab[(size_t)j1 * n + i0], t11 = tab[(size_t)j1 * n + i1];
+    return (1.0 - ax) * ((1.0 - ay) * t00 + ay * t01) + ax * ((1.0 - ay) * t10 + ay * t11);
+}
+
 }  // namespace d4

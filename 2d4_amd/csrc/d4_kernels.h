@@ -48,6 +48,8 @@ enum PassMode : int {
     kModeVjp = 3,        // d sigma/dx (+ d sigma/dy): w-weighted pair sums
     kModeJacDenseCut = 4,  // as kModeJacDense with the storage radius of kModeJacCoo:
                            // V = 0 outside it (the reference's cutoff, dense storage)
+    kModeSpeedTab = 5,     // kModeSpeed with sigma interpolated in a table
+    kModeJacDenseTab = 6,  // kModeJacDense with sigma and d sigma interpolated in tables
 };
 
 struct PassArgs {
@@ -71,16 +73,30 @@ struct PassArgs {
     int coo_cap;
     // VJP input (mode 3)
     const double* w;
+    // tabulated field (modes 5, 6): tab_n x tab_n node values over [-1/2, 1/2)^2
+    const double* tab_s;
+    const double* tab_d;
+    int tab_n;
     int exact;                // mode 1/2/3: 1 = true derivative instead of the reference's
     int want_dy;              // mode 3: also d/dy
 };
 
 // number of accumulated quantities per mode
-inline int pass_nq(int mode) { return mode == kModeSpeed || mode == kModeJacCoo ? 1 : 2; }
+inline int pass_nq(int mode)
+{
+    return mode == kModeSpeed || mode == kModeJacCoo || mode == kModeSpeedTab ? 1 : 2;
+}
 
 // grid = 0: one workgroup per tile; grid > 0: `grid` persistent workgroups
 // that stride over the tile table
 hipError_t launch_pair_pass(int mode, const PassArgs& a, hipStream_t s, int grid = 0);
+
+// Node values of the tabulated field: tab[ky * n + kx] = sigma (and d sigma/dx)
+// at (dx, dy) = (-1/2 + kx/n, -1/2 + ky/n), from the analytic field (0 at the origin).
+hipError_t launch_table_build(int n, double* tab_s, double* tab_d, hipStream_t s);
+// bilinear values of the tables for m independent pairs
+hipError_t launch_table_eval(int64_t m, const double* dx, const double* dy, const double* tab_s, const double* tab_d,
+                             int tab_n, double* s_out, double* d_out, hipStream_t s);
 
 // Field values for n independent pairs (E, c may be null -> computed on device).
 hipError_t launch_field_eval(int64_t n, const double* dx, const double* dy, const double* E, const double* c,

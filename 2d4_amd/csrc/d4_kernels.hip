@@ -32,11 +32,12 @@ __device__ __forceinline__ void atomic_min_nonneg(double* p, double v)
 template <int MODE, bool EXACT>
 __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
 {
-    constexpr bool kDense = MODE == kModeJacDense || MODE == kModeJacDenseCut;
+    constexpr bool kTab = MODE == kModeSpeedTab || MODE == kModeJacDenseTab;
+    constexpr bool kDense = MODE == kModeJacDense || MODE == kModeJacDenseCut || MODE == kModeJacDenseTab;
     constexpr bool kCut = MODE == kModeJacCoo || MODE == kModeJacDenseCut;  // storage radius applies
     constexpr bool kCoo = MODE == kModeJacCoo;
     constexpr bool kVjp = MODE == kModeVjp;
-    constexpr int NQ = (MODE == kModeSpeed || MODE == kModeJacCoo) ? 1 : 2;
+    constexpr int NQ = (MODE == kModeSpeed || MODE == kModeJacCoo || MODE == kModeSpeedTab) ? 1 : 2;
 
     __shared__ double sx[kTile], sy[kTile], sw[kVjp ? kTile : 1];
     __shared__ double jq0[kTile], jq1[NQ == 2 ? kTile : 1];
@@ -85,10 +86,24 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                     atomic_min_nonneg(a.tol + base + j, d2);
                 }
                 const double bj = j < half ? 1.0 : -1.0;
-                double E, R, c;
-                field_inputs(dx, dy, E, R, c);
+                double E = 0.0, R = 0.0, c = 0.0;
+                if (!kTab) field_inputs(dx, dy, E, R, c);
                 double sig = 0.0, dsig = 0.0;
-                if (MODE == kModeSpeed) {
+                if (MODE == kModeSpeedTab) {
+                    sig = table_bilinear(a.tab_s, a.tab_n, dx, dy);
+                    const double f = sig * bi * bj;
+                    q0 += f;
+                    atomicAdd(&jq0[jl], -f);
+                } else if (MODE == kModeJacDenseTab) {
+                    sig = table_bilinear(a.tab_s, a.tab_n, dx, dy);
+                    dsig = table_bilinear(a.tab_d, a.tab_n, dx, dy);
+                    const double f = sig * bi * bj;
+                    V = rp.h * bj * bi * dsig;
+                    q0 += f;
+                    atomicAdd(&jq0[jl], -f);
+                    q1 -= V;
+                    atomicAdd(&jq1[jl], -V);
+                } else if (MODE == kModeSpeed) {
                     pair_field<false, false>(dx, dy, E, R, c, sig, dsig);
                     const double f = sig * bi * bj;
                     q0 += f;
@@ -203,7 +218,9 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
 // the pair-list, dense-with-cutoff and adjoint passes (their extra live values
 // would spill at 168).  PERSIST: the grid strides over the tile table.
 template <int MODE, bool EXACT, bool PERSIST>
-__global__ __launch_bounds__(64, MODE == kModeSpeed ? 4 : (MODE == kModeJacDense ? 3 : 2)) void pair_pass_kernel(
+__global__ __launch_bounds__(64, (MODE == kModeSpeed || MODE == kModeSpeedTab)
+                                     ? 4
+                                     : ((MODE == kModeJacDense || MODE == kModeJacDenseTab) ? 3 : 2)) void pair_pass_kernel(
     PassArgs a)
 {
     if constexpr (PERSIST) {
@@ -240,8 +257,54 @@ hipError_t launch_pair_pass(int mode, const PassArgs& a, hipStream_t s, int grid
     case kModeJacCoo: return launch_mode<kModeJacCoo>(a, s, grid);
     case kModeVjp: return launch_mode<kModeVjp>(a, s, grid);
     case kModeJacDenseCut: return launch_mode<kModeJacDenseCut>(a, s, grid);
+    case kModeSpeedTab: return launch_mode<kModeSpeedTab>(a, s, grid);
+    case kModeJacDenseTab: return launch_mode<kModeJacDenseTab>(a, s, grid);
     default: return hipErrorInvalidValue;
     }
+}
+
+// ------------------------------------------------------- tabulated field
+
+__global__ void table_build_kernel(int n, double* tab_s, double* tab_d)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)n * n) return;
+    const int kx = (int)(k % n), ky = (int)(k / n);
+    const double dx = -0.5 + (double)kx / n, dy = -0.5 + (double)ky / n;  // exact for n a power of two
+    double s = 0.0, d = 0.0;
+    if (dx != 0.0 || dy != 0.0) {  // the field is singular at the origin; its node holds 0
+        double E, R, c;
+        field_inputs(dx, dy, E, R, c);
+        pair_field<true, false>(dx, dy, E, R, c, s, d);
+    }
+    tab_s[k] = s;
+    if (tab_d) tab_d[k] = d;
+}
+
+hipError_t launch_table_build(int n, double* tab_s, double* tab_d, hipStream_t s)
+{
+    const int64_t nn = (int64_t)n * n;
+    hipLaunchKernelGGL(table_build_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, n, tab_s, tab_d);
+    return hipGetLastError();
+}
+
+__global__ void table_eval_kernel(int64_t m, const double* dx, const double* dy, const double* tab_s,
+                                  const double* tab_d, int tab_n, double* s_out, double* d_out)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const double x = wrap(dx[k]), y = wrap(dy[k]);
+    if (s_out) s_out[k] = table_bilinear(tab_s, tab_n, x, y);
+    if (d_out) d_out[k] = table_bilinear(tab_d, tab_n, x, y);
+}
+
+hipError_t launch_table_eval(int64_t m, const double* dx, const double* dy, const double* tab_s, const double* tab_d,
+                             int tab_n, double* s_out, double* d_out, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(table_eval_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, dx, dy, tab_s, tab_d,
+                       tab_n, s_out, d_out);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------ field eval

@@ -234,6 +234,24 @@ def main():
         t1 = time.perf_counter()
         extra["scaled_matrix_pass"] = dict(ms_per_pass=(t1 - t0) * 1e3 / args.steps,
                                            pair_evals_per_s=pairs / ((t1 - t0) / args.steps))
+        # the tabulated-field variant: 1024 x 1024 FP64 tables of sigma and d sigma, bilinear
+        tab_s, tab_d = ctx.table_build_t(1024)
+        for name, fn in (("table_speed_pass", lambda: ctx.speed_pass_table_t(xy, tab_s, 0.1, out=v)),
+                         ("table_jacobian_pass", lambda: ctx.jacobian_dense_table_t(xy, p, tab_s, tab_d, M=M))):
+            fn()
+            torch.cuda.synchronize()
+            ctx.set_profiling(True)
+            ctx.pair_kernel_time()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                fn()
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            k_ms2, k_l2 = ctx.pair_kernel_time()
+            ctx.set_profiling(False)
+            extra[name] = dict(ms_per_pass=(t1 - t0) * 1e3 / args.steps, kernel_ms=k_ms2 / max(k_l2, 1),
+                               pair_evals_per_s=pairs / ((t1 - t0) / args.steps), table="1024 x 1024 FP64")
+        del tab_s, tab_d
     del M
     torch.cuda.empty_cache()
     ensemble = None

@@ -136,6 +136,25 @@ int d4_jacobian_dense(d4_ctx* ctx, int n, const double* xy, const d4_jac_params*
 int d4_vjp_dev(d4_ctx* ctx, int n, const double* xy, const double* w, double* gx, double* gy, double* gtau,
                int flags);
 
+/* Tabulated field (no reference counterpart: the reference evaluates the
+ * analytic Field::xy per pair).  A table is n x n doubles (n a power of two),
+ * row-major in dy: tab[ky*n + kx] holds the value at (dx, dy) =
+ * (-1/2 + kx/n, -1/2 + ky/n); between nodes the passes interpolate bilinearly
+ * and periodically.  d4_table_build_dev fills tab_s with Field::xy and tab_d
+ * (may be NULL) with Field::xy_diff_x (Field.cpp:295-346) at the nodes, 0 at
+ * the origin.  Device pointers. */
+int d4_table_build_dev(d4_ctx* ctx, int n, double* tab_s, double* tab_d);
+/* bilinear table values for m pairs (dx, dy wrapped first); s or d may be NULL */
+int d4_table_eval_dev(d4_ctx* ctx, int64_t m, const double* dx, const double* dy, const double* tab_s,
+                      const double* tab_d, int n, double* s, double* d);
+/* d4_speed_pass_dev with sigma from the table */
+int d4_speed_pass_table_dev(d4_ctx* ctx, int n, const double* xy, const double* tab_s, int tab_n, double tau,
+                            double* v);
+/* d4_jacobian_dense_dev with sigma and d sigma from the tables (no tolerance tracking) */
+int d4_jacobian_dense_table_dev(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, const double* tab_s,
+                                const double* tab_d, int tab_n, double* vA, double* vB, double* M, int64_t ld,
+                                double* dvec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,8 @@ class Oracle:
             "orc_normalize": (d, [d]), "orc_weight": (d, [d, ch]), "orc_weight_inv": (d, [d, ch]),
             "orc_xy4": (d, [d, d, d, d]), "orc_dxy4": (d, [d, d, d, d, i]), "orc_xy": (d, [d, d]),
             "orc_dxy": (d, [d, d]), "orc_dyxy4": (d, [d, d, d, d]),
+            "orc_table_bilinear": (d, [vp, i, d, d]), "orc_table_build": (None, [i, vp, vp]),
+            "orc_speed_pass_table": (None, [i, vp, vp, i, d, vp]),
             "orc_field_inputs": (None, [d, d, pd, pd]),
             "orc_field_batch": (None, [C.c_int64, vp, vp, vp, vp, vp, vp, i]),
             "orc_inputs_batch": (None, [C.c_int64, vp, vp, vp, vp]),

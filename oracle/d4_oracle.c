@@ -20,6 +20,10 @@
  *       invariants (antisymmetric pair sums, V symmetric, row sums, A*1 = 1).
  *   orc_read_dconf / orc_cutoff             the reference SimulationData under
  *       oracle/_ref on init_config_gen/README_files/1000_64.dconf.
+ *   orc_table_bilinear / orc_table_build / orc_speed_pass_table   the build's own
+ *       table format: exact at the nodes (the node value itself), exact on
+ *       bilinear functions (a table of a + b x + c y + d x y reproduces it),
+ *       periodic wrap; the nodes themselves are orc_xy / orc_dxy.
  *   orc_run                                 parity unpinned against the
  *       reference itself (simulation.cpp needs UMFPACK, absent from the image);
  *       checked by invariants (strain = sum b dx, accepted-step counts) only.
@@ -288,6 +292,51 @@ void orc_speed_pass(int n, const double* xy, double tau, double* v, double* tol,
             upd_tol(tol, cap2, d2, i);
             upd_tol(tol, cap2, d2, j);
             double f = orc_xy(dx, dy) * bsign(i, n) * bsign(j, n);
+            v[i] += f;
+            v[j] -= f;
+        }
+        v[i] += tau * bsign(i, n);
+    }
+}
+
+/* ---------------------------------------------------- tabulated field
+ * The build's own format (no reference counterpart): n x n node values over
+ * the periodic cell, tab[ky*n + kx] at (-1/2 + kx/n, -1/2 + ky/n), bilinear
+ * in between.  Written out independently of the kernel's table_bilinear, with
+ * the same order of operations (the library is built without contraction). */
+double orc_table_bilinear(const double* tab, int n, double dx, double dy)
+{
+    double u = (dx + 0.5) * n, w = (dy + 0.5) * n;
+    double fu = floor(u), fw = floor(w);
+    double ax = u - fu, ay = w - fw;
+    int i0 = ((int)fu) & (n - 1), j0 = ((int)fw) & (n - 1);
+    int i1 = (i0 + 1) & (n - 1), j1 = (j0 + 1) & (n - 1);
+    double t00 = tab[(size_t)j0 * n + i0], t10 = tab[(size_t)j0 * n + i1];
+    double t01 = tab[(size_t)j1 * n + i0], t11 = tab[(size_t)j1 * n + i1];
+    return (1.0 - ax) * ((1.0 - ay) * t00 + ay * t01) + ax * ((1.0 - ay) * t10 + ay * t11);
+}
+
+void orc_table_build(int n, double* tab_s, double* tab_d)
+{
+    for (int ky = 0; ky < n; ++ky)
+        for (int kx = 0; kx < n; ++kx) {
+            double dx = -0.5 + (double)kx / n, dy = -0.5 + (double)ky / n;
+            size_t k = (size_t)ky * n + kx;
+            int origin = dx == 0.0 && dy == 0.0;
+            tab_s[k] = origin ? 0.0 : orc_xy(dx, dy);
+            if (tab_d) tab_d[k] = origin ? 0.0 : orc_dxy(dx, dy);
+        }
+}
+
+/* orc_speed_pass with sigma from the table */
+void orc_speed_pass_table(int n, const double* xy, const double* tab, int tn, double tau, double* v)
+{
+    for (int i = 0; i < n; ++i) v[i] = 0;
+    for (int i = 0; i < n; ++i) {
+        for (int j = i + 1; j < n; ++j) {
+            double dx = orc_normalize(xy[2 * i] - xy[2 * j]);
+            double dy = orc_normalize(xy[2 * i + 1] - xy[2 * j + 1]);
+            double f = orc_table_bilinear(tab, tn, dx, dy) * bsign(i, n) * bsign(j, n);
             v[i] += f;
             v[j] -= f;
         }

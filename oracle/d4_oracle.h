@@ -64,6 +64,14 @@ void orc_jacobian_from_prev(int n, int* Ap, int* Ai, double* Ax, double* dvec, c
 void orc_jacobian_dense(int n, const double* xy, double h, double tauA, double tauB, char wfunc,
                         double* vA, double* vB, double* M, double* dvec, double* tol, double tol_cap2);
 
+/* Tabulated field (the build's format, no reference counterpart): n x n node
+ * values, tab[ky*n + kx] at (-1/2 + kx/n, -1/2 + ky/n), n a power of two,
+ * bilinear and periodic in between; the build fills nodes with orc_xy/orc_dxy
+ * (0 at the origin). */
+double orc_table_bilinear(const double* tab, int n, double dx, double dy);
+void orc_table_build(int n, double* tab_s, double* tab_d);
+void orc_speed_pass_table(int n, const double* xy, const double* tab, int tn, double tau, double* v);
+
 /* simulation_data.cpp:253-262 */
 void orc_cutoff(int n, double mult, int heaviside, double* cutoff, double* cut_sq, double* one_per);
 
