@@ -101,7 +101,8 @@ class ImplicitStep(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, y, tau, dt, d4, tol=1e-12, max_iter=30):
         t = float(tau)
-        floor = ImplicitStep.RESIDUAL_FLOOR * dt
+        # ... and below that, the rounding of the positions themselves
+        floor = ImplicitStep.RESIDUAL_FLOOR * dt + 64 * 2.0 ** -52 * (1.0 + float(x.abs().max()))
         stats["solves"] += 1
         xn = x.clone()
         v, A = _newton_matrix(d4, xn, y, t, dt)
@@ -123,7 +124,7 @@ class ImplicitStep(torch.autograd.Function):
             stalled = False
             while True:
                 trial = xn - lam * delta
-                vt, At = _newton_matrix(d4, trial, y, t, dt)
+                vt = d4.speed_pass_t(_xy(trial, y), t)  # speeds only; the matrix once a trial is taken
                 Gt = trial - x - dt * vt
                 if float(torch.linalg.vector_norm(Gt)) < g0:
                     break
@@ -136,7 +137,8 @@ class ImplicitStep(torch.autograd.Function):
             if stalled:
                 stats["floor_stops"] += 1
                 break
-            xn, G, A = trial, Gt, At
+            xn, G = trial, Gt
+            _, A = _newton_matrix(d4, xn, y, t, dt)
         else:
             raise NewtonFailure(f"Newton did not converge in {max_iter} iterations "
                                 f"(last correction {float(delta.abs().max()):.3e})")

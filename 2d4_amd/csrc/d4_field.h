@@ -30,6 +30,22 @@
 
 #define D4_HD __host__ __device__ __forceinline__
 
+// a * b + c, fused, for a constant c (a polynomial coefficient).  On the
+// device as an explicit three-operand v_fma_f64 reading c from a scalar
+// register: the compiler's accumulate form (v_fmac_f64) would first copy the
+// constant into the destination, one extra VALU slot per step of a
+// polynomial.  Arithmetic only.
+D4_HD double fma3(double a, double b, double c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double d;
+    asm volatile("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
+    return d;
+#else
+    return fma(a, b, c);
+#endif
+}
+
 namespace d4 {
 
 constexpr double kPi = 3.14159265358979323846;
@@ -317,19 +333,19 @@ D4_HD void exp2_pair(double y, double& E, double& R)
 {
     const double k = rint(y);
     const double f = y - k;
-    double p = fma(f, tc::e13, tc::e12);
-    p = fma(f, p, tc::e11);
-    p = fma(f, p, tc::e10);
-    p = fma(f, p, tc::e9);
-    p = fma(f, p, tc::e8);
-    p = fma(f, p, tc::e7);
-    p = fma(f, p, tc::e6);
-    p = fma(f, p, tc::e5);
-    p = fma(f, p, tc::e4);
-    p = fma(f, p, tc::e3);
-    p = fma(f, p, tc::e2);
-    p = fma(f, p, tc::e1);
-    p = fma(f, p, 1.0);
+    double p = fma3(f, tc::e13, tc::e12);
+    p = fma3(f, p, tc::e11);
+    p = fma3(f, p, tc::e10);
+    p = fma3(f, p, tc::e9);
+    p = fma3(f, p, tc::e8);
+    p = fma3(f, p, tc::e7);
+    p = fma3(f, p, tc::e6);
+    p = fma3(f, p, tc::e5);
+    p = fma3(f, p, tc::e4);
+    p = fma3(f, p, tc::e3);
+    p = fma3(f, p, tc::e2);
+    p = fma3(f, p, tc::e1);
+    p = fma3(f, p, 1.0);
     E = ldexp(p, (int)k);
     // 1/E correctly rounded, as the reference's identities take it (constants.h:63-79):
     // the last Newton step of rcp() runs from a faithful estimate, which rounds correctly
@@ -347,13 +363,13 @@ D4_HD double cos2pi_reduced(double dy)
     const bool hi = at > 0.5 * tc::pio2_1;
     const double rho = hi ? (tc::pio2_1 - at) + tc::pio2_2 : th;
     const double r2 = rho * rho;
-    double sp = fma(r2, tc::s17, tc::s15);
-    sp = fma(r2, sp, tc::s13);
-    sp = fma(r2, sp, tc::s11);
-    sp = fma(r2, sp, tc::s9);
-    sp = fma(r2, sp, tc::s7);
-    sp = fma(r2, sp, tc::s5);
-    sp = fma(r2, sp, tc::s3);
+    double sp = fma3(r2, tc::s17, tc::s15);
+    sp = fma3(r2, sp, tc::s13);
+    sp = fma3(r2, sp, tc::s11);
+    sp = fma3(r2, sp, tc::s9);
+    sp = fma3(r2, sp, tc::s7);
+    sp = fma3(r2, sp, tc::s5);
+    sp = fma3(r2, sp, tc::s3);
     const double sn = fma(rho * r2, sp, rho);
     const double v = fma(-2.0 * sn, sn, 1.0);
     return hi ? -v : v;
