@@ -136,6 +136,40 @@ def ensemble_bench(pkg, ctx, steps, R, seed=4242):
             "bitwise_equal_to_plain": equal}
 
 
+def adjoint_bench(n, T, dt, every):
+    """Strain gradient through T backward-Euler steps (2d4_amd/adjoint.py) at
+    N = n: times the forward pass, the checkpointed forward and the backward."""
+    import torch
+    adj = importlib.import_module("2d4_amd.adjoint")
+    pkg = importlib.import_module("2d4_amd")
+    ctx = pkg.Context(0)
+    x, y = adj.separated_config(n, n, 0.5 / np.sqrt(n), device="cuda:0")
+    tau = torch.tensor(0.2, dtype=torch.float64, device="cuda:0")
+    adj.reset_stats()
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        xs = x
+        for k in range(0, T, every):
+            xs = adj._segment(xs, y, tau, min(every, T - k), dt, ctx, 1e-12, 30)
+            log(f"[bench] adjoint forward {min(k + every, T)} steps: {time.perf_counter() - t0:.1f} s {adj.stats}")
+    t1 = time.perf_counter()
+    fwd_stats = dict(adj.stats)
+    x0 = x.clone().requires_grad_()
+    yy = y.clone().requires_grad_()
+    tt = tau.clone().requires_grad_()
+    L = adj.accumulated_strain(x0, yy, tt, T, dt, ctx, checkpoint_every=every)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    L.backward()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    return {"n": n, "T": T, "dt": dt, "checkpoint_every": every, "tau": 0.2,
+            "forward_s": t1 - t0, "forward_checkpointed_s": t2 - t1, "backward_s": t3 - t2,
+            "forward_stats": fwd_stats, "L": L.item(), "max_abs_dL_dx0": x0.grad.abs().max().item(),
+            "max_abs_dL_dy": yy.grad.abs().max().item(), "dL_dtau": tt.grad.item(),
+            "config": "uniform positions with a minimum separation of 0.5/sqrt(N), sorted"}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -158,6 +192,8 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=2, help="timed lockstep iterations of the ensemble")
     ap.add_argument("--ens-r", type=int, default=256, help="realizations in the ensemble")
     ap.add_argument("--no-ens", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--adjoint", type=str, default="",
+                    help="only the adjoint measurement: N,T,dt (e.g. 2048,64,1e-5), checkpoints every 16 steps")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -175,6 +211,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calib_copy_bytes": nel * 8}), flush=True)
+        return
+    if args.adjoint:
+        an, aT, adt = args.adjoint.split(",")
+        print(json.dumps({"adjoint": adjoint_bench(int(an), int(aT), float(adt), 16)}), flush=True)
         return
     pkg = importlib.import_module("2d4_amd")
     dev = torch.device("cuda:0")
