@@ -21,7 +21,8 @@ class SimStats(C.Structure):
     """d4_sim_stats (include/d4_sim.h)."""
     _fields_ = [("sim_time", _d), ("step_size", _d), ("strain", _d), ("last_max_error_ratio_sq", _d),
                 ("successful_steps", C.c_int64), ("failed_steps", C.c_int64), ("pair_passes", C.c_int64),
-                ("factorizations", C.c_int64), ("singular_solves", C.c_int64)]
+                ("factorizations", C.c_int64), ("singular_solves", C.c_int64),
+                ("unpivoted_factorizations", C.c_int64)]
 
 
 class EnsConfig(C.Structure):
@@ -40,6 +41,7 @@ SIGNATURES = {
     "d4_sim_positions": (C.c_int, [_vp, _vp, C.c_int]),
     "d4_sim_get_stats": (C.c_int, [_vp, _P(SimStats)]),
     "d4_sim_write_dconf": (C.c_int, [_vp, C.c_char_p]),
+    "d4_lu_solve_batch": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     # include/d4_batch.h
     "d4_batch_speed_pass": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "d4_batch_jacobian_pass": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _pi,

@@ -10,6 +10,7 @@
 #include "../../include/d4_batch.h"
 #include "d4_ctx.h"
 #include "d4_host.h"
+#include "d4_solver.h"
 
 #include <algorithm>
 #include <atomic>
@@ -541,10 +542,11 @@ struct d4_ens {
     std::vector<int> off;
     std::unique_ptr<BatchPasses> bp;
     std::unique_ptr<ThreadPool> pool;
+    std::unique_ptr<d4host::BatchDenseLU> blu;  // the solver-1 realizations' LUs, batched
     d4_ens_config cfg{0, 0, 0, 0};
     bool started = false;
     std::string err;
-    double t_pass = 0, t_host = 0, t_total = 0;
+    double t_pass = 0, t_solve = 0, t_host = 0, t_total = 0;
 
     int fail(int code, const std::string& m)
     {
@@ -563,10 +565,70 @@ namespace {
 using sdddstCore::PassJob;
 using sdddstCore::Simulation;
 
+void pool_for(void* user, int count, void (*fn)(void*, int), void* arg)
+{
+    static_cast<ThreadPool*>(user)->parallel_for(count, [&](int k) { fn(arg, k); });
+}
+
+// Factor / Solve jobs: the realizations with the GPU dense solver through one
+// BatchDenseLU call (bitwise what each one's own solver gives), the others
+// each with its own solver on the pool
+int run_linear(d4_ens* e, const std::vector<int>& act, PassJob::Kind kind)
+{
+    const double t0 = now_ms();
+    std::vector<int> gpu, own;
+    for (int r : act) (e->data[r]->solverKind == 1 ? gpu : own).push_back(r);
+    std::vector<int> rcs(own.size(), D4_OK);
+    e->pool->parallel_for((int)own.size(), [&](int i) {
+        Simulation& s = *e->sims[own[i]];
+        sdddstCore::PassEngine eng;
+        rcs[i] = eng.run(s, s.job());
+    });
+    for (int rc : rcs)
+        if (rc) return e->fail(rc, kind == PassJob::Factor ? "linear solver factorization failed" : "linear solve failed");
+    if (!gpu.empty()) {
+        if (!e->blu) {
+            e->blu.reset(d4host::new_batch_dense_lu(d4_ctx_device(e->ctx)));
+            if (!e->blu) return e->fail(D4_ERR_HIP, "batched GPU LU unavailable");
+            e->blu->set_parallel(pool_for, e->pool.get());
+        }
+        const int K = (int)gpu.size();
+        if (kind == PassJob::Factor) {
+            std::vector<d4host::BatchDenseLU::Matrix> m(K);
+            for (int k = 0; k < K; ++k) {
+                const sdddstCore::SimulationData& d = *e->data[gpu[k]];
+                m[k] = {(int)d.dc, d.Ap.data(), d.Ai.data(), d.Ax.data()};
+            }
+            std::vector<unsigned char> sing(K, 0), unpiv(K, 0);
+            if (e->blu->factor(K, m.data(), sing.data(), unpiv.data()) < 0)
+                return e->fail(D4_ERR_HIP, "batched GPU LU factorization failed");
+            for (int k = 0; k < K; ++k) {
+                PassJob& j = e->sims[gpu[k]]->job();
+                j.singular = sing[k] != 0;
+                j.unpivoted = unpiv[k] != 0;
+            }
+        } else {
+            std::vector<const double*> b(K);
+            std::vector<double*> x(K);
+            for (int k = 0; k < K; ++k) {
+                PassJob& j = e->sims[gpu[k]]->job();
+                b[k] = j.rhs->data();
+                x[k] = j.sol->data();
+            }
+            if (e->blu->solve(K, b.data(), x.data()) < 0) return e->fail(D4_ERR_HIP, "batched GPU LU solve failed");
+        }
+    }
+    e->t_solve += now_ms() - t0;
+    return D4_OK;
+}
+
 // stages the jobs of the simulations `act` (all of one kind), runs them as one
 // batched pass, and hands each simulation its results
 int run_batched(d4_ens* e, const std::vector<int>& act)
 {
+    if (e->sims[act[0]]->job().kind == PassJob::Factor || e->sims[act[0]]->job().kind == PassJob::Solve)
+        return run_linear(e, act, e->sims[act[0]]->job().kind);
+
     BatchPasses& bp = *e->bp;
     const int R = (int)act.size();
     std::vector<int> off(R + 1, 0);
@@ -846,6 +908,7 @@ int d4_ens_stats(const d4_ens* e, d4_sim_stats* st)
         st[r].pair_passes = s.pairPasses;
         st[r].factorizations = s.factorizations;
         st[r].singular_solves = s.singularSolves;
+        st[r].unpivoted_factorizations = s.unpivotedFactorizations;
     }
     return D4_OK;
 }
@@ -934,12 +997,13 @@ int d4_ens_autotune(d4_ens* e, d4_ens_config* best, double* best_ms)
     return D4_OK;
 }
 
-int d4_ens_timing(const d4_ens* e, double* ms3)
+int d4_ens_timing(const d4_ens* e, double* ms4)
 {
-    if (!e || !ms3) return D4_ERR_INVALID;
-    ms3[0] = e->t_pass;
-    ms3[1] = e->t_host;
-    ms3[2] = e->t_total;
+    if (!e || !ms4) return D4_ERR_INVALID;
+    ms4[0] = e->t_pass;
+    ms4[1] = e->t_solve;
+    ms4[2] = e->t_host;
+    ms4[3] = e->t_total;
     return D4_OK;
 }
 

@@ -14,6 +14,7 @@ using namespace d4;
 d4_ctx::~d4_ctx()
 {
     if (batch) batch_passes_free(batch);
+    if (lu) batch_lu_free(lu);
     for (auto* b : {&part, &tiles, &xy, &v0, &v1, &tol, &coo, &coo_count, &cnt, &cursor, &Ap, &Ai, &Ax, &idx, &dvec,
                     &nw, &aux0, &aux1, &aux2})
         b->release();

@@ -41,6 +41,10 @@ inline RealParams real_params(int n, const d4_jac_params* p, double tau)
 
 struct BatchPasses;
 void batch_passes_free(BatchPasses* b);
+namespace d4host {
+class BatchDenseLU;
+}
+void batch_lu_free(d4host::BatchDenseLU* lu);  // d4_gpulu.hip
 
 struct DevBuf {
     void* p = nullptr;
@@ -65,6 +69,7 @@ struct d4_ctx {
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_marks;
     BatchPasses* batch = nullptr;  // tables and buffers of d4_batch_* (d4_batch.hip)
+    d4host::BatchDenseLU* lu = nullptr;  // d4_lu_solve_batch's solver (d4_gpulu.hip)
     // tabulated field for the table pass modes (set around single_pass)
     const double* tab_s = nullptr;
     const double* tab_d = nullptr;

@@ -1,22 +1,58 @@
-// d4_gpulu.hip -- the host simulation's linear solves as a dense LU on the GPU
-// (rocSOLVER dgetrf / dgetrs, partial pivoting), in place of UMFPACK's
-// umfpack_di_symbolic / _numeric / _solve (simulation.cpp:635-659).
+// d4_gpulu.hip -- the host simulation's linear solves as a dense LU on the GPU,
+// in place of UMFPACK's umfpack_di_symbolic / _numeric / _solve
+// (simulation.cpp:635-659), for one matrix (GpuDenseLU) or for the matrices of
+// many realizations at once (BatchLU).
 //
 // At the sizes of a realization (N <= a few thousand) one dense LU on the GPU
 // takes milliseconds where a sparse LU of the cutoff pattern on a host core
 // takes up to seconds (the 2D cutoff graph fills in heavily), so this is the
 // solver the batched ensemble and large single runs use.
+//
+// Layout: matrix k is column-major of order npad_k = gpu_lu_padded(n_k) (n_k
+// rounded up to 256), the CSC entries scattered into its leading n_k x n_k
+// block and the identity beyond it.  The padding changes nothing in the
+// solution: its rows and columns are decoupled from the leading block and
+// never enter a pivot choice.  It lets matrices of similar order share one
+// batched rocSOLVER call (pointer arrays), and rocSOLVER's batched routines
+// treat each matrix of a batch alone, so a matrix's factors do not depend on
+// the batch it is in (bitwise; the ensemble tests check it).  GpuDenseLU is
+// BatchLU with K = 1.
+//
+// Pivoting: threshold partial pivoting with u = 0.1 and a preference for the
+// diagonal (0.1 is UMFPACK's default relative pivot tolerance; UMFPACK is the
+// reference's solver, and the host SparseLU here uses the same threshold).  Every matrix is first
+// factored without row interchanges (getrf_npvt_batched, 2-4x faster than
+// getrf on these sizes); if every pivot is finite and nonzero and every
+// multiplier |l_rc| <= 1/u = 10, the diagonal was an acceptable pivot at every
+// stage and the factorization stands.  Otherwise the matrix is scattered again
+// and factored with partial pivoting (getrf_batched).  The choice depends on
+// the matrix values only.
+//
+// Solves: a triangular-solve kernel of this file (lu_solve_kernel), one
+// workgroup per matrix for all matrices of the batch in one launch, in place
+// of getrs (whose single-right-hand-side solves are launch bound: 1-5 ms each
+// at N = 256..4096).
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <algorithm>
+#include <climits>
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <thread>
+#include <type_traits>
+#include <vector>
 
+#include "../../include/d4_sim.h"
+#include "d4_ctx.h"
 #include "d4_solver.h"
 
 namespace d4host {
+
+int gpu_lu_padded(int n) { return n <= 0 ? 0 : (n + 255) / 256 * 256; }
+
 namespace {
 
 // one stream + rocBLAS handle per (thread, device); kept until the process ends
@@ -45,121 +81,567 @@ ThreadRes* thread_res(int device)
     return r;
 }
 
-__global__ void zero_kernel(double* A, size_t count)
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    bool ensure(size_t count)
+    {
+        if (count <= cap) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        cap = count;
+        return true;
+    }
+};
+
+template <class T>
+struct PinBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    ~PinBuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    bool ensure(size_t count)
+    {
+        if (count <= cap) return true;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        if (hipHostMalloc(&p, sizeof(T) * std::max<size_t>(count, 1), hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            return false;
+        }
+        cap = count;
+        return true;
+    }
+};
+
+struct MatDesc {
+    long long a_off;  // first entry of the matrix in the dense store
+    int n, npad;
+    int col_off;   // its n + 1 column pointers in the packed CSC
+    int ipiv_off;  // its npad pivots (matrices refactored with pivoting)
+};
+
+struct SolveDesc {
+    long long a_off;
+    int n, npad;
+    int b_off;     // its right-hand side in the packed vector (n entries)
+    int perm_off;  // its row permutation, or -1 when rows stayed in place
+};
+
+constexpr double kMaxMultiplier = 10.0;  // threshold partial pivoting with u = 0.1
+constexpr int kSB = 64;                  // block of the triangular solves
+constexpr int kSolveThreads = 1024;
+
+// identity beyond n, zero in the leading block: column blockIdx.x of matrix blockIdx.y
+__global__ void fill_identity_kernel(const MatDesc* __restrict__ desc, double* __restrict__ A)
 {
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < count) A[k] = 0.0;
+    const MatDesc m = desc[blockIdx.y];
+    const int c = blockIdx.x;
+    if (c >= m.npad) return;
+    double* col = A + m.a_off + (size_t)c * m.npad;
+    const double diag = c >= m.n ? 1.0 : 0.0;
+    for (int r = threadIdx.x; r < m.npad; r += blockDim.x) col[r] = r == c ? diag : 0.0;
 }
 
-// column c of the CSC matrix into column c of the dense column-major A
-__global__ void scatter_kernel(const int* Ap, const int* Ai, const double* Ax, double* A, int n)
+// the CSC entries of column c (one thread) into the dense matrix
+__global__ void scatter_kernel(const MatDesc* __restrict__ desc, const int* __restrict__ colp,
+                               const int* __restrict__ rowi, const double* __restrict__ vals, double* __restrict__ A)
 {
+    const MatDesc m = desc[blockIdx.y];
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    double* col = A + (size_t)c * n;
-    for (int k = Ap[c]; k < Ap[c + 1]; ++k) col[Ai[k]] = Ax[k];
+    if (c >= m.n) return;
+    double* col = A + m.a_off + (size_t)c * m.npad;
+    for (int q = colp[m.col_off + c]; q < colp[m.col_off + c + 1]; ++q) col[rowi[q]] = vals[q];
 }
+
+// after the no-pivot factorization: flag[k] = 1 unless every pivot is finite
+// and nonzero and every multiplier |l_rc| <= kMaxMultiplier, i.e. unless
+// threshold partial pivoting with u = 0.1 would keep the diagonal at every stage
+__global__ void multiplier_check_kernel(const MatDesc* __restrict__ desc, const double* __restrict__ A,
+                                        int* __restrict__ flag)
+{
+    const MatDesc m = desc[blockIdx.y];
+    const int c = blockIdx.x;
+    if (c >= m.n) return;
+    const double* col = A + m.a_off + (size_t)c * m.npad;
+    bool bad = false;
+    if (threadIdx.x == 0) {
+        const double p = std::fabs(col[c]);
+        bad = !(p > 0.0) || !(p <= 1.7976931348623157e308);
+    }
+    for (int r = c + 1 + threadIdx.x; r < m.n; r += blockDim.x) bad |= !(std::fabs(col[r]) <= kMaxMultiplier);
+    if (bad) atomicOr(flag + blockIdx.y, 1);
+}
+
+// LAPACK pivots (row i swapped with row ipiv[i] - 1, in order) into the
+// permutation perm (solution row i comes from right-hand-side row perm[i]),
+// in place; one workgroup per matrix, the swaps by one thread in LDS (kLds) or
+// in global memory
+template <bool kLds>
+__global__ void pivots_to_perm_kernel(const MatDesc* __restrict__ desc, int* __restrict__ ipiv,
+                                      int* __restrict__ scratch)
+{
+    extern __shared__ int sp[];
+    const MatDesc m = desc[blockIdx.x];
+    int* iv = ipiv + m.ipiv_off;
+    int* p = kLds ? sp : scratch + 2 * (size_t)m.ipiv_off;
+    int* q = p + m.npad;
+    for (int i = threadIdx.x; i < m.npad; i += blockDim.x) {
+        p[i] = i;
+        const int j = iv[i] - 1;
+        q[i] = (j >= i && j < m.npad) ? j : i;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int i = 0; i < m.npad; ++i) {
+            const int j = q[i];
+            if (j != i) {
+                const int t = p[i];
+                p[i] = p[j];
+                p[j] = t;
+            }
+        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < m.npad; i += blockDim.x) iv[i] = p[i];
+}
+
+__device__ inline double lane_value(double v, int lane)  // v of `lane` (uniform) in every lane
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// x := U^{-1} L^{-1} P b for the factors in A (getrf's layout: unit lower L
+// below the diagonal, U on and above), one workgroup per matrix.  Blocked by
+// kSB rows: the diagonal block is staged in LDS and solved by wave 0 (lane l
+// owns row l, the solved value of lane c broadcast at step c), then every
+// thread updates rows outside the block with the block's columns
+// (x_r -= sum_c a_rc x_c, accumulated in ascending c).  The order of
+// operations is fixed by the matrix alone.  x lives in LDS (kLds) or in
+// global scratch; the arithmetic is the same either way.
+template <bool kLds>
+__global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc* __restrict__ desc,
+                                                                  const double* __restrict__ A,
+                                                                  const int* __restrict__ perm, double* __restrict__ b,
+                                                                  double* __restrict__ scratch)
+{
+    extern __shared__ double lds[];
+    const SolveDesc m = desc[blockIdx.x];
+    const int n = m.n, tid = threadIdx.x;
+    const size_t lda = (size_t)m.npad;
+    const double* Am = A + m.a_off;
+    double* bm = b + m.b_off;
+    double* D = lds;  // kSB x kSB diagonal block, column-major
+    double* x = kLds ? lds + kSB * kSB : scratch + m.b_off;
+    for (int i = tid; i < n; i += kSolveThreads) {
+        const int p = m.perm_off >= 0 ? perm[m.perm_off + i] : i;  // padded rows never move into [0, n)
+        x[i] = bm[(p >= 0 && p < n) ? p : i];
+    }
+    __syncthreads();
+
+    const int nblk = (n + kSB - 1) / kSB;
+    for (int pass = 0; pass < 2; ++pass) {  // 0: L y = P b top down, 1: U x = y bottom up
+        for (int s = 0; s < nblk; ++s) {
+            const int bi = pass == 0 ? s : nblk - 1 - s;
+            const int jb = bi * kSB, B = min(kSB, n - jb);
+            for (int e = tid; e < kSB * kSB; e += kSolveThreads) {
+                const int r = e % kSB, c = e / kSB;
+                D[e] = (r < B && c < B) ? Am[(size_t)(jb + c) * lda + jb + r] : 0.0;
+            }
+            __syncthreads();
+            if (tid < 64) {
+                const int l = tid;
+                double v = l < B ? x[jb + l] : 0.0;
+                if (pass == 0) {
+#pragma unroll 8
+                    for (int c = 0; c < kSB - 1; ++c) {
+                        const double yc = lane_value(v, c);
+                        if (l > c) v = fma(-D[c * kSB + l], yc, v);
+                    }
+                } else {
+#pragma unroll 8
+                    for (int c = kSB - 1; c >= 0; --c) {
+                        if (c >= B) continue;
+                        if (l == c) v = v / D[c * kSB + c];
+                        const double xc = lane_value(v, c);
+                        if (l < c) v = fma(-D[c * kSB + l], xc, v);
+                    }
+                }
+                if (l < B) x[jb + l] = v;
+            }
+            __syncthreads();
+            const int r0 = pass == 0 ? jb + B : 0, r1 = pass == 0 ? n : jb;
+            for (int r = r0 + tid; r < r1; r += kSolveThreads) {
+                const double* a = Am + (size_t)jb * lda + r;
+                double acc = 0.0;
+#pragma unroll 8
+                for (int c = 0; c < B; ++c) acc = fma(a[c * lda], x[jb + c], acc);
+                x[r] -= acc;
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = tid; i < n; i += kSolveThreads) bm[i] = x[i];
+}
+
+// a valid CSC matrix: Ap[0] = 0, nondecreasing column pointers, rows in [0, n)
+bool valid_csc(const BatchDenseLU::Matrix& m)
+{
+    if (m.Ap[0] != 0) return false;
+    for (int c = 0; c < m.n; ++c) {
+        if (m.Ap[c + 1] < m.Ap[c]) return false;
+        for (int k = m.Ap[c]; k < m.Ap[c + 1]; ++k)
+            if (m.Ai[k] < 0 || m.Ai[k] >= m.n) return false;
+    }
+    return true;
+}
+
+class BatchLU : public BatchDenseLU {
+public:
+    explicit BatchLU(int device) : device_(device)
+    {
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lds = 65536;
+        lds_bytes_ = lds;
+        lds_order_ = std::max(0, (lds / 8 - kSB * kSB) / 256 * 256);
+    }
+
+    void set_parallel(ParallelFor pf, void* user) override
+    {
+        pf_ = pf;
+        user_ = user;
+    }
+
+    int factor(int K, const Matrix* m, unsigned char* singular, unsigned char* unpivoted) override
+    {
+        K_ = 0;
+        if (K < 0 || (K > 0 && !m)) return -1;
+        if (K == 0) return 0;
+        if (K > kMaxMatrices) return -1;
+        for (int k = 0; k < K; ++k)
+            if (m[k].n < 0 || (m[k].n > 0 && (!m[k].Ap || !m[k].Ai || !m[k].Ax)) || m[k].n > kMaxOrder) return -1;
+        std::vector<unsigned char> ok(K, 1);
+        par(K, [&](int k) { ok[k] = m[k].n == 0 || valid_csc(m[k]); });
+        for (int k = 0; k < K; ++k)
+            if (!ok[k]) return -1;
+        ThreadRes* r = thread_res(device_);
+        if (!r || hipSetDevice(device_) != hipSuccess) return -2;
+        hipStream_t st = r->stream;
+
+        // layout in the caller's order
+        desc_h_.resize(K);
+        std::vector<long long> nz_off(K + 1, 0);
+        b_off_.assign(K + 1, 0);
+        size_t a_total = 0;
+        long long col_total = 0, piv_total = 0;
+        int npad_max = 0;
+        for (int k = 0; k < K; ++k) {
+            const int n = m[k].n, npad = gpu_lu_padded(n);
+            MatDesc& d = desc_h_[k];
+            d.a_off = (long long)a_total;
+            d.n = n;
+            d.npad = npad;
+            d.col_off = (int)col_total;
+            d.ipiv_off = -1;
+            a_total += (size_t)npad * npad;
+            col_total += n + 1;
+            piv_total += npad;
+            nz_off[k + 1] = nz_off[k] + (n > 0 ? m[k].Ap[n] : 0);
+            b_off_[k + 1] = b_off_[k] + n;
+            npad_max = std::max(npad_max, npad);
+        }
+        if (nz_off[K] > INT_MAX || col_total > INT_MAX || piv_total > INT_MAX / 2) return -1;
+        const long long nnz = nz_off[K];
+
+        // every matrix first without pivoting, in groups of one padded order
+        // (largest first: that is also the solve's launch order)
+        std::vector<int> order = by_order(K, [](int k) { return k; });
+        std::vector<Group> groups = make_groups(order);
+        std::vector<int> pos(K);
+        for (int i = 0; i < K; ++i) pos[order[i]] = i;
+
+        if (!A_.ensure(a_total) || !ipiv_.ensure((size_t)piv_total) || !iscr_.ensure(2 * (size_t)piv_total) ||
+            !info_.ensure(3 * (size_t)K) || !ptrA_.ensure(2 * (size_t)K) || !desc_.ensure(2 * (size_t)K) ||
+            !sdesc_.ensure(K) || !colp_.ensure((size_t)col_total) || !rowi_.ensure((size_t)nnz) ||
+            !vals_.ensure((size_t)nnz) || !b_.ensure(2 * (size_t)b_off_[K]))
+            return -2;
+        if (!colp_h_.ensure((size_t)col_total) || !rowi_h_.ensure((size_t)nnz) || !vals_h_.ensure((size_t)nnz) ||
+            !ptr_h_.ensure(2 * (size_t)K) || !info_h_.ensure(3 * (size_t)K))
+            return -2;
+        int* info = info_.p;          // [0, K): no-pivot pass (group order)
+        int* flag = info_.p + K;      // [K, 2K): check (caller's order)
+        int* info2 = info_.p + 2 * K; // [2K, 3K): pivoted pass (its group order)
+
+        par(K, [&](int k) {
+            const int n = m[k].n;
+            if (n == 0) return;
+            const long long base = nz_off[k];
+            int* cp = colp_h_.p + desc_h_[k].col_off;
+            for (int c = 0; c <= n; ++c) cp[c] = (int)(base + m[k].Ap[c]);
+            const int cnt = m[k].Ap[n];
+            std::copy(m[k].Ai, m[k].Ai + cnt, rowi_h_.p + base);
+            std::copy(m[k].Ax, m[k].Ax + cnt, vals_h_.p + base);
+        });
+        for (int i = 0; i < K; ++i) ptr_h_.p[i] = A_.p + desc_h_[order[i]].a_off;
+        if (hipMemcpyAsync(desc_.p, desc_h_.data(), sizeof(MatDesc) * K, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(colp_.p, colp_h_.p, sizeof(int) * col_total, hipMemcpyHostToDevice, st) != hipSuccess ||
+            (nnz > 0 &&
+             (hipMemcpyAsync(rowi_.p, rowi_h_.p, sizeof(int) * nnz, hipMemcpyHostToDevice, st) != hipSuccess ||
+              hipMemcpyAsync(vals_.p, vals_h_.p, sizeof(double) * nnz, hipMemcpyHostToDevice, st) != hipSuccess)) ||
+            hipMemcpyAsync(ptrA_.p, ptr_h_.p, sizeof(double*) * K, hipMemcpyHostToDevice, st) != hipSuccess)
+            return -2;
+        if (assemble(desc_.p, K, npad_max, st)) return -2;
+        if (hipMemsetAsync(info, 0, sizeof(int) * 2 * K, st) != hipSuccess) return -2;
+        for (const Group& g : groups)
+            if (rocsolver_dgetrf_npvt_batched(r->handle, g.npad, g.npad, ptrA_.p + g.first, g.npad, info + g.first,
+                                              g.count) != rocblas_status_success)
+                return -2;
+        for (int k0 = 0; k0 < K; k0 += kGridY) {
+            const int kc = std::min(kGridY, K - k0);
+            if (npad_max > 0)
+                hipLaunchKernelGGL(multiplier_check_kernel, dim3(npad_max, kc), dim3(256), 0, st, desc_.p + k0, A_.p,
+                                   flag + k0);
+        }
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(info_h_.p, info, sizeof(int) * 2 * K, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return -2;
+
+        // the rest again from the CSC, with partial pivoting
+        std::vector<int> redo;
+        for (int k = 0; k < K; ++k)
+            if (desc_h_[k].n > 0 && (info_h_.p[pos[k]] > 0 || info_h_.p[K + k] != 0)) redo.push_back(k);
+        const int F = (int)redo.size();
+        std::vector<int> redo_pos(K, -1);
+        if (F > 0) {
+            std::vector<int> order2 = by_order(F, [&](int i) { return redo[i]; });
+            for (int& i : order2) i = redo[i];
+            std::vector<Group> groups2 = make_groups(order2);
+            std::vector<MatDesc> sub(F);
+            int npad2 = 0;
+            long long poff = 0;
+            for (int i = 0; i < F; ++i) {
+                const int k = order2[i];
+                desc_h_[k].ipiv_off = (int)poff;
+                poff += desc_h_[k].npad;
+                sub[i] = desc_h_[k];
+                redo_pos[k] = i;
+                ptr_h_.p[K + i] = A_.p + desc_h_[k].a_off;
+                npad2 = std::max(npad2, desc_h_[k].npad);
+            }
+            if (hipMemcpyAsync(desc_.p + K, sub.data(), sizeof(MatDesc) * F, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(ptrA_.p + K, ptr_h_.p + K, sizeof(double*) * F, hipMemcpyHostToDevice, st) !=
+                    hipSuccess)
+                return -2;
+            if (assemble(desc_.p + K, F, npad2, st)) return -2;
+            if (hipMemsetAsync(info2, 0, sizeof(int) * F, st) != hipSuccess) return -2;
+            for (const Group& g : groups2)
+                if (rocsolver_dgetrf_batched(r->handle, g.npad, g.npad, ptrA_.p + K + g.first, g.npad,
+                                             ipiv_.p + desc_h_[order2[g.first]].ipiv_off, g.npad, info2 + g.first,
+                                             g.count) != rocblas_status_success)
+                    return -2;
+            if (npad2 <= lds_bytes_ / 8)
+                hipLaunchKernelGGL(pivots_to_perm_kernel<true>, dim3(F), dim3(256), sizeof(int) * 2 * npad2, st,
+                                   desc_.p + K, ipiv_.p, iscr_.p);
+            else
+                hipLaunchKernelGGL(pivots_to_perm_kernel<false>, dim3(F), dim3(256), 0, st, desc_.p + K, ipiv_.p,
+                                   iscr_.p);
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(info_h_.p + 2 * K, info2, sizeof(int) * F, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                return -2;
+        }
+        for (int k = 0; k < K; ++k) {
+            const bool pivoted = redo_pos[k] >= 0;
+            // U(info, info) == 0 after partial pivoting: UMFPACK_WARNING_singular_matrix's case
+            if (singular) singular[k] = pivoted && info_h_.p[2 * K + redo_pos[k]] > 0;
+            if (unpivoted) unpivoted[k] = !pivoted;
+        }
+
+        // the solves' descriptors, largest matrices first
+        std::vector<SolveDesc> sd;
+        sd.reserve(K);
+        for (int i = 0; i < K; ++i) {
+            const int k = order[i];
+            if (desc_h_[k].n == 0) continue;
+            sd.push_back({desc_h_[k].a_off, desc_h_[k].n, desc_h_[k].npad, (int)b_off_[k],
+                          redo_pos[k] >= 0 ? desc_h_[k].ipiv_off : -1});
+        }
+        nsolve_ = (int)sd.size();
+        npad_max_ = npad_max;
+        if (nsolve_ > 0 &&
+            (hipMemcpyAsync(sdesc_.p, sd.data(), sizeof(SolveDesc) * nsolve_, hipMemcpyHostToDevice, st) != hipSuccess ||
+             hipStreamSynchronize(st) != hipSuccess))
+            return -2;
+        ndesc_.resize(K);
+        for (int k = 0; k < K; ++k) ndesc_[k] = desc_h_[k].n;
+        K_ = K;
+        return 0;
+    }
+
+    int solve(int K, const double* const* b, double* const* x) override
+    {
+        if (K != K_ || (K > 0 && (!b || !x))) return -1;
+        if (K == 0 || nsolve_ == 0) return 0;
+        ThreadRes* r = thread_res(device_);
+        if (!r || hipSetDevice(device_) != hipSuccess) return -2;
+        const long long total = b_off_[K];
+        if (!bh_.ensure((size_t)total)) return -2;
+        par(K, [&](int k) { std::copy(b[k], b[k] + ndesc_[k], bh_.p + b_off_[k]); });
+        hipStream_t st = r->stream;
+        if (hipMemcpyAsync(b_.p, bh_.p, sizeof(double) * total, hipMemcpyHostToDevice, st) != hipSuccess) return -2;
+        if (npad_max_ <= lds_order_)
+            hipLaunchKernelGGL(lu_solve_kernel<true>, dim3(nsolve_), dim3(kSolveThreads),
+                               sizeof(double) * (kSB * kSB + npad_max_), st, sdesc_.p, A_.p, ipiv_.p, b_.p,
+                               b_.p + total);
+        else
+            hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(nsolve_), dim3(kSolveThreads), sizeof(double) * kSB * kSB,
+                               st, sdesc_.p, A_.p, ipiv_.p, b_.p, b_.p + total);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(bh_.p, b_.p, sizeof(double) * total, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return -2;
+        par(K, [&](int k) { std::copy(bh_.p + b_off_[k], bh_.p + b_off_[k] + ndesc_[k], x[k]); });
+        return 0;
+    }
+
+private:
+    static constexpr int kGridY = 65535;  // grid.y limit: matrices per launch
+    static constexpr int kMaxMatrices = 1 << 24;
+    static constexpr int kMaxOrder = 46080;  // npad^2 stays below 2^31 (rocSOLVER's int lda * n)
+
+    struct Group {
+        int first, count, npad;
+    };
+
+    int device_;
+    int lds_bytes_ = 65536, lds_order_ = 0;
+    ParallelFor pf_ = nullptr;
+    void* user_ = nullptr;
+    int K_ = 0, nsolve_ = 0, npad_max_ = 0;
+    std::vector<MatDesc> desc_h_;
+    std::vector<long long> b_off_;
+    std::vector<int> ndesc_;
+    DevBuf<double> A_, vals_, b_;
+    DevBuf<int> ipiv_, iscr_, info_, colp_, rowi_;
+    DevBuf<double*> ptrA_;
+    DevBuf<MatDesc> desc_;
+    DevBuf<SolveDesc> sdesc_;
+    PinBuf<int> colp_h_, rowi_h_, info_h_;
+    PinBuf<double> vals_h_, bh_;
+    PinBuf<double*> ptr_h_;
+
+    // indices 0..count-1 (through key to the caller's index) by padded order, largest first, stable
+    template <class Key>
+    std::vector<int> by_order(int count, Key key) const
+    {
+        std::vector<int> o(count);
+        for (int i = 0; i < count; ++i) o[i] = i;
+        std::stable_sort(o.begin(), o.end(),
+                         [&](int a, int b) { return desc_h_[key(a)].npad > desc_h_[key(b)].npad; });
+        return o;
+    }
+
+    std::vector<Group> make_groups(const std::vector<int>& order) const
+    {
+        std::vector<Group> g;
+        for (int i = 0; i < (int)order.size(); ++i) {
+            const int npad = desc_h_[order[i]].npad;
+            if (npad == 0) continue;
+            if (g.empty() || g.back().npad != npad) g.push_back({i, 0, npad});
+            g.back().count++;
+        }
+        return g;
+    }
+
+    int assemble(const MatDesc* desc, int K, int npad_max, hipStream_t st)
+    {
+        if (npad_max == 0) return 0;
+        for (int k0 = 0; k0 < K; k0 += kGridY) {
+            const int kc = std::min(kGridY, K - k0);
+            hipLaunchKernelGGL(fill_identity_kernel, dim3(npad_max, kc), dim3(256), 0, st, desc + k0, A_.p);
+            hipLaunchKernelGGL(scatter_kernel, dim3((npad_max + 255) / 256, kc), dim3(256), 0, st, desc + k0, colp_.p,
+                               rowi_.p, vals_.p, A_.p);
+        }
+        return hipGetLastError() != hipSuccess ? -1 : 0;
+    }
+
+    template <class F>
+    void par(int count, F&& f)
+    {
+        using Fn = std::remove_reference_t<F>;
+        if (!pf_ || count < 2) {
+            for (int k = 0; k < count; ++k) f(k);
+            return;
+        }
+        pf_(user_, count, [](void* a, int k) { (*static_cast<Fn*>(a))(k); }, &f);
+    }
+};
 
 class GpuDenseLU : public LinearSolver {
 public:
-    explicit GpuDenseLU(int device) : device_(device) {}
-    ~GpuDenseLU() override
-    {
-        for (void* p : {(void*)A_, (void*)ipiv_, (void*)info_, (void*)b_, (void*)Ap_, (void*)Ai_, (void*)Ax_})
-            if (p) (void)hipFree(p);
-    }
+    explicit GpuDenseLU(int device) : lu_(device) {}
 
     int factor(int n, const int* Ap, const int* Ai, const double* Ax) override
     {
-        singular_ = false;
-        if (n <= 0 || !Ap) return n == 0 ? 0 : -1;
-        ThreadRes* r = thread_res(device_);
-        if (!r) return -2;
-        const int nnz = Ap[n];
-        if (grow(n, nnz)) return -2;
-        n_ = n;
-        const size_t nn = (size_t)n * n;
-        hipError_t e = hipMemcpyAsync(Ap_, Ap, sizeof(int) * (n + 1), hipMemcpyHostToDevice, r->stream);
-        if (e == hipSuccess && nnz > 0)
-            e = hipMemcpyAsync(Ai_, Ai, sizeof(int) * nnz, hipMemcpyHostToDevice, r->stream);
-        if (e == hipSuccess && nnz > 0)
-            e = hipMemcpyAsync(Ax_, Ax, sizeof(double) * nnz, hipMemcpyHostToDevice, r->stream);
-        if (e != hipSuccess) return -2;
-        hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, r->stream, A_, nn);
-        hipLaunchKernelGGL(scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, r->stream, Ap_, Ai_, Ax_, A_, n);
-        if (hipGetLastError() != hipSuccess) return -2;
-        if (rocsolver_dgetrf(r->handle, n, n, A_, n, ipiv_, info_) != rocblas_status_success) return -2;
-        int info = 0;
-        if (hipMemcpyAsync(&info, info_, sizeof(int), hipMemcpyDeviceToHost, r->stream) != hipSuccess ||
-            hipStreamSynchronize(r->stream) != hipSuccess)
-            return -2;
-        singular_ = info > 0;  // U(info, info) == 0: UMFPACK_WARNING_singular_matrix's case
+        singular_ = unpivoted_ = false;
+        if (n < 0 || (n > 0 && !Ap)) return -1;
+        BatchDenseLU::Matrix m{n, Ap, Ai, Ax};
+        unsigned char s = 0, u = 0;
+        const int rc = lu_.factor(1, &m, &s, &u);
+        if (rc < 0) return rc;
+        singular_ = s != 0;
+        unpivoted_ = u != 0;
         return singular_ ? 1 : 0;
     }
 
-    int solve(const double* b, double* x) override
-    {
-        ThreadRes* r = thread_res(device_);
-        if (!r || !A_) return -2;
-        const int n = n_;
-        if (hipMemcpyAsync(b_, b, sizeof(double) * n, hipMemcpyHostToDevice, r->stream) != hipSuccess) return -2;
-        if (rocsolver_dgetrs(r->handle, rocblas_operation_none, n, 1, A_, n, ipiv_, b_, n) != rocblas_status_success)
-            return -2;
-        if (hipMemcpyAsync(x, b_, sizeof(double) * n, hipMemcpyDeviceToHost, r->stream) != hipSuccess ||
-            hipStreamSynchronize(r->stream) != hipSuccess)
-            return -2;
-        return 0;
-    }
-
+    int solve(const double* b, double* x) override { return lu_.solve(1, &b, &x); }
     bool singular() const override { return singular_; }
+    bool unpivoted() const override { return unpivoted_; }
 
 private:
-    int device_;
-    int n_ = 0, cap_n_ = 0, cap_nnz_ = 0;
-    bool singular_ = false;
-    double* A_ = nullptr;
-    int* ipiv_ = nullptr;
-    int* info_ = nullptr;
-    double* b_ = nullptr;
-    int* Ap_ = nullptr;
-    int* Ai_ = nullptr;
-    double* Ax_ = nullptr;
-
-    int grow(int n, int nnz)
-    {
-        if (hipSetDevice(device_) != hipSuccess) return -1;
-        if (n > cap_n_) {
-            for (void* p : {(void*)A_, (void*)ipiv_, (void*)info_, (void*)b_, (void*)Ap_})
-                if (p) (void)hipFree(p);
-            A_ = nullptr;
-            ipiv_ = info_ = Ap_ = nullptr;
-            b_ = nullptr;
-            if (hipMalloc(&A_, sizeof(double) * (size_t)n * n) != hipSuccess ||
-                hipMalloc(&ipiv_, sizeof(int) * n) != hipSuccess || hipMalloc(&info_, sizeof(int)) != hipSuccess ||
-                hipMalloc(&b_, sizeof(double) * n) != hipSuccess ||
-                hipMalloc(&Ap_, sizeof(int) * (n + 1)) != hipSuccess) {
-                cap_n_ = 0;
-                return -1;
-            }
-            cap_n_ = n;
-        }
-        if (nnz > cap_nnz_) {
-            if (Ai_) (void)hipFree(Ai_);
-            if (Ax_) (void)hipFree(Ax_);
-            Ai_ = nullptr;
-            Ax_ = nullptr;
-            if (hipMalloc(&Ai_, sizeof(int) * nnz) != hipSuccess || hipMalloc(&Ax_, sizeof(double) * nnz) != hipSuccess) {
-                cap_nnz_ = 0;
-                return -1;
-            }
-            cap_nnz_ = nnz;
-        }
-        return 0;
-    }
+    BatchLU lu_;
+    bool singular_ = false, unpivoted_ = false;
 };
 
 }  // namespace
 
 LinearSolver* new_gpu_dense_lu(int device) { return new GpuDenseLU(device); }
+BatchDenseLU* new_batch_dense_lu(int device) { return new BatchLU(device); }
 
 }  // namespace d4host
+
+void batch_lu_free(d4host::BatchDenseLU* lu) { delete lu; }
+
+extern "C" int d4_lu_solve_batch(d4_ctx* ctx, int K, const int* n, const int* const* Ap, const int* const* Ai,
+                                 const double* const* Ax, const double* const* b, double* const* x,
+                                 unsigned char* singular, unsigned char* unpivoted)
+{
+    if (!ctx) return D4_ERR_INVALID;
+    if (K < 0 || (K > 0 && (!n || !Ap || !Ai || !Ax || !b || !x)))
+        return ctx->fail(D4_ERR_INVALID, "d4_lu_solve_batch: null argument or K < 0");
+    if (K == 0) return D4_OK;
+    if (!ctx->lu) ctx->lu = d4host::new_batch_dense_lu(ctx->device);
+    std::vector<d4host::BatchDenseLU::Matrix> m(K);
+    for (int k = 0; k < K; ++k) m[k] = {n[k], Ap[k], Ai[k], Ax[k]};
+    int rc = ctx->lu->factor(K, m.data(), singular, unpivoted);
+    if (rc == -1) return ctx->fail(D4_ERR_INVALID, "d4_lu_solve_batch: invalid matrix (order or CSC structure)");
+    if (rc < 0) return ctx->fail(D4_ERR_HIP, "d4_lu_solve_batch: factorization failed on the GPU");
+    rc = ctx->lu->solve(K, b, x);
+    if (rc < 0) return ctx->fail(rc == -1 ? D4_ERR_INVALID : D4_ERR_HIP, "d4_lu_solve_batch: solve failed");
+    return D4_OK;
+}

@@ -144,7 +144,7 @@ class Simulation;
 // calls per step: two Jacobian passes, three speed passes and the stage-II
 // rescale; run() adds one speed pass before the loop).
 struct PassJob {
-    enum Kind { None = 0, Speed, Jacobian, Rescale };
+    enum Kind { None = 0, Speed, Jacobian, Rescale, Factor, Solve };
     Kind kind = None;
     const std::vector<DislwoB>* xy = nullptr;  // Speed, Jacobian
     double tau = 0;                            // Speed: extStress(t)
@@ -155,11 +155,18 @@ struct PassJob {
     double factor = 0, dtau = 0;               // Rescale (simulation.cpp:546-608)
     std::vector<double>* speed = nullptr;      // Rescale: speeds moved by dtau b
     int nnz = 0;                               // Jacobian: entries written to the step matrix
+    // Factor: LU of the step matrix (the simulation's Ap, Ai, Ax; simulation.cpp:635-641);
+    // the engine reports a zero pivot and whether rows were left in place
+    bool singular = false, unpivoted = false;
+    // Solve: sol = A^{-1} rhs with the last Factor's matrix (simulation.cpp:643-659)
+    const std::vector<double>* rhs = nullptr;
+    std::vector<double>* sol = nullptr;
 };
 
-// Runs PassJobs.  The default engine runs each job at once through include/d4.h
-// on the simulation's context; a batching engine collects the same job from
-// many simulations and runs them in one launch.
+// Runs PassJobs.  The default engine runs each job at once: pair passes through
+// include/d4.h on the simulation's context, Factor / Solve with the simulation's
+// own linear solver; a batching engine collects the same job from many
+// simulations and runs them together.
 class PassEngine {
 public:
     virtual ~PassEngine() = default;
@@ -176,11 +183,12 @@ public:
     bool interrupt() const;  // simulation.cpp:722-750 (limits that exist here)
     int status() const { return status_; }
     const std::string& error() const { return error_; }
-    int64_t pairPasses = 0, factorizations = 0, singularSolves = 0;
+    int64_t pairPasses = 0, factorizations = 0, singularSolves = 0, unpivotedFactorizations = 0;
 
-    // the step split at its pair passes: begin_step() (false when interrupt()
-    // holds) sets the first job, then each after_pass() consumes job() and sets
-    // the next; job().kind == None when the step is complete
+    // the step split at its pair passes and linear solves: begin_step() (false
+    // when interrupt() holds) sets the first job, then each after_pass()
+    // consumes job() and sets the next; job().kind == None when the step is
+    // complete
     void begin_prologue();
     bool begin_step();
     void after_pass();
@@ -189,6 +197,7 @@ public:
     SimulationData& data() { return *sD; }
     PrecisionHandler& precision() { return *pH; }
     d4_ctx* ctx() const { return ctx_; }
+    d4host::LinearSolver& solver() { return *lu_; }
     bool check(int rc, const char* where);
 
 private:
@@ -208,10 +217,14 @@ private:
     void set_jacobian_job(double stepsize, const std::vector<DislwoB>& dislocs, std::vector<double>& forces_A,
                           std::vector<double>& forces_B, double simTime_A, double simTime_B);
     void set_rescale_job(double baseTime, double newTime, double oldTime);
-    void calcGSolveAndUpdate(std::vector<DislwoB>& new_disloc, const std::vector<DislwoB>& old_config, double stepSize,
-                             const std::vector<double>& endSpeed, const std::vector<double>& initSpeed);
-    void calculateSparseFormForJacobian();
-    void solveEQSys();
+    bool singular_ = false;  // the last factorization met a zero pivot
+    void set_factor_job();
+    void after_factor();
+    void set_solve_job();
+    void after_solve();
+    void calcG(const std::vector<DislwoB>& new_disloc, const std::vector<DislwoB>& old_config, double stepSize,
+               const std::vector<double>& endSpeed, const std::vector<double>& initSpeed);
+    void calcGFromSpeeds(double stepSize, const std::vector<double>& endSpeed, const std::vector<double>& initSpeed);
     void calculateXError();
     double calculateStrainIncrement(const std::vector<DislwoB>& old, const std::vector<DislwoB>& newD) const;
     bool finish_step();

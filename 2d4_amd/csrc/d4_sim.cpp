@@ -272,36 +272,60 @@ int PassEngine::run(Simulation& sim, PassJob& job)
     case PassJob::Rescale:
         return d4_jacobian_rescale(ctx, n, d.Ap.data(), d.Ai.data(), d.Ax.data(), d.dVec.data(), d.weightFunc,
                                    job.factor, job.dtau, job.speed->data());
+    case PassJob::Factor: {
+        d4host::LinearSolver& lu = sim.solver();
+        if (lu.factor(n, d.Ap.data(), d.Ai.data(), d.Ax.data()) < 0) return D4_ERR_HIP;
+        job.singular = lu.singular();
+        job.unpivoted = lu.unpivoted();
+        return D4_OK;
+    }
+    case PassJob::Solve: return sim.solver().solve(job.rhs->data(), job.sol->data()) < 0 ? D4_ERR_HIP : D4_OK;
     default: return D4_OK;
     }
 }
 
-void Simulation::calculateSparseFormForJacobian()
+void Simulation::set_factor_job()  // simulation.cpp:635-641
 {
+    job_ = PassJob();
+    job_.kind = PassJob::Factor;
     factorizations++;
-    if (lu_->factor((int)sD->dc, sD->Ap.data(), sD->Ai.data(), sD->Ax.data()) < 0)
-        check(D4_ERR_HIP, "linear solver factorization");
 }
 
-void Simulation::solveEQSys()  // simulation.cpp:643-659
+void Simulation::after_factor()
 {
-    if (lu_->solve(sD->g.data(), sD->x.data()) < 0) check(D4_ERR_HIP, "linear solve");
-    if (lu_->singular()) {
-        singularSolves++;
-        for (unsigned i = 0; i < sD->dc; ++i)
-            if (!((-1 < sD->x[i]) && (sD->x[i] < 1))) sD->x[i] = 0;
-    }
+    singular_ = job_.singular;
+    if (job_.unpivoted) unpivotedFactorizations++;
 }
 
-void Simulation::calcGSolveAndUpdate(std::vector<DislwoB>& new_disloc, const std::vector<DislwoB>& old_config,
-                                     double stepSize, const std::vector<double>& endSpeed,
-                                     const std::vector<double>& initSpeed)
+void Simulation::set_solve_job()  // solveEQSys, simulation.cpp:643-659
+{
+    job_ = PassJob();
+    job_.kind = PassJob::Solve;
+    job_.rhs = &sD->g;
+    job_.sol = &sD->x;
+}
+
+void Simulation::after_solve()  // simulation.cpp:651-657: the singular factorization's clamp
+{
+    if (!singular_) return;
+    singularSolves++;
+    for (unsigned i = 0; i < sD->dc; ++i)
+        if (!((-1 < sD->x[i]) && (sD->x[i] < 1))) sD->x[i] = 0;
+}
+
+void Simulation::calcG(const std::vector<DislwoB>& new_disloc, const std::vector<DislwoB>& old_config,
+                       double stepSize, const std::vector<double>& endSpeed, const std::vector<double>& initSpeed)
 {
     for (unsigned i = 0; i < sD->dc; i++)  // simulation.cpp:620-624
         sD->g[i] = new_disloc[i].x - old_config[i].x -
                    stepSize * ((1 + sD->dVec[i]) * endSpeed[i] + (1 - sD->dVec[i]) * initSpeed[i]) / 2;
-    solveEQSys();
-    for (unsigned i = 0; i < sD->dc; i++) new_disloc[i].x -= sD->x[i];
+}
+
+void Simulation::calcGFromSpeeds(double stepSize, const std::vector<double>& endSpeed,
+                                 const std::vector<double>& initSpeed)
+{
+    for (unsigned i = 0; i < sD->dc; i++)  // simulation.cpp:102-105, 121-124, 142-145
+        sD->g[i] = -stepSize * ((1 + sD->dVec[i]) * endSpeed[i] + (1 - sD->dVec[i]) * initSpeed[i]) / 2;
 }
 
 void Simulation::calculateXError()  // simulation.cpp:692-700
@@ -356,46 +380,74 @@ void Simulation::after_pass()
         job_ = PassJob();
         return;
     }
+    // each stage: pass, Factor, Solve (first Newton iterate), speed pass, Solve
+    // (second iterate, simulation.cpp:611-633), then the next stage's pass
     switch (done) {
     case -1:  // the initial speed pass
         job_ = PassJob();
         break;
-    case 1:  // stage I, after the Jacobian pass (simulation.cpp:98-109)
-        calculateSparseFormForJacobian();
-        for (unsigned i = 0; i < n; i++)
-            d.g[i] = -d.stepSize * ((1 + d.dVec[i]) * d.speed[i] + (1 - d.dVec[i]) * d.initSpeed[i]) / 2;
-        solveEQSys();
+    case 1:  // stage I, after the Jacobian pass (simulation.cpp:98-100)
+    case 6:  // stage II, after the rescale (simulation.cpp:118-120)
+    case 11:  // stage III, after the Jacobian pass (simulation.cpp:139-141)
+        set_factor_job();
+        break;
+    case 2:  // simulation.cpp:101-105
+        after_factor();
+        calcGFromSpeeds(d.stepSize, d.speed, d.initSpeed);
+        set_solve_job();
+        break;
+    case 3:  // simulation.cpp:106-109
+        after_solve();
         for (unsigned i = 0; i < n; i++) d.bigStep_sorted[i] = DislwoB(d.disl_sorted[i].x - d.x[i], d.disl_sorted[i].y);
         set_speed_job(d.bigStep_sorted, d.speed2, t1_);
         break;
-    case 2:  // simulation.cpp:111-113, then stage II's rescale
-        calcGSolveAndUpdate(d.bigStep_sorted, d.disl_sorted, d.stepSize, d.speed2, d.initSpeed);
+    case 4:  // simulation.cpp:111-113
+        calcG(d.bigStep_sorted, d.disl_sorted, d.stepSize, d.speed2, d.initSpeed);
+        set_solve_job();
+        break;
+    case 5:  // then stage II's rescale
+        after_solve();
+        for (unsigned i = 0; i < n; i++) d.bigStep_sorted[i].x -= d.x[i];
         set_rescale_job(t0_, th_, t1_);
         break;
-    case 3:  // stage II (simulation.cpp:118-130)
-        calculateSparseFormForJacobian();
-        for (unsigned i = 0; i < n; i++)
-            d.g[i] = -d.stepSize / 2 * ((1 + d.dVec[i]) * d.speed[i] + (1 - d.dVec[i]) * d.initSpeed[i]) / 2;
-        solveEQSys();
+    case 7:  // simulation.cpp:121-124
+        after_factor();
+        calcGFromSpeeds(d.stepSize / 2, d.speed, d.initSpeed);
+        set_solve_job();
+        break;
+    case 8:  // simulation.cpp:125-130
+        after_solve();
         for (unsigned i = 0; i < n; i++)
             d.firstSmall_sorted[i] = DislwoB(d.disl_sorted[i].x - d.x[i], d.disl_sorted[i].y);
         set_speed_job(d.firstSmall_sorted, d.speed2, th_);
         break;
-    case 4:  // simulation.cpp:132-134, then stage III's Jacobian pass
-        calcGSolveAndUpdate(d.firstSmall_sorted, d.disl_sorted, d.stepSize / 2, d.speed2, d.initSpeed);
+    case 9:  // simulation.cpp:132-134
+        calcG(d.firstSmall_sorted, d.disl_sorted, d.stepSize / 2, d.speed2, d.initSpeed);
+        set_solve_job();
+        break;
+    case 10:  // then stage III's Jacobian pass
+        after_solve();
+        for (unsigned i = 0; i < n; i++) d.firstSmall_sorted[i].x -= d.x[i];
         set_jacobian_job(d.stepSize / 2, d.firstSmall_sorted, d.initSpeed2, d.speed2, th_, t1_);
         break;
-    case 5:  // stage III (simulation.cpp:139-150)
-        calculateSparseFormForJacobian();
-        for (unsigned i = 0; i < n; i++)
-            d.g[i] = -d.stepSize / 2 * ((1 + d.dVec[i]) * d.speed2[i] + (1 - d.dVec[i]) * d.initSpeed2[i]) / 2;
-        solveEQSys();
+    case 12:  // simulation.cpp:142-145
+        after_factor();
+        calcGFromSpeeds(d.stepSize / 2, d.speed2, d.initSpeed2);
+        set_solve_job();
+        break;
+    case 13:  // simulation.cpp:146-150
+        after_solve();
         for (unsigned i = 0; i < n; i++)
             d.secondSmall_sorted[i] = DislwoB(d.firstSmall_sorted[i].x - d.x[i], d.firstSmall_sorted[i].y);
         set_speed_job(d.secondSmall_sorted, d.speed2, t1_);
         break;
-    case 6:  // simulation.cpp:152-154 and stage IV
-        calcGSolveAndUpdate(d.secondSmall_sorted, d.firstSmall_sorted, d.stepSize / 2, d.speed2, d.initSpeed2);
+    case 14:  // simulation.cpp:152-154
+        calcG(d.secondSmall_sorted, d.firstSmall_sorted, d.stepSize / 2, d.speed2, d.initSpeed2);
+        set_solve_job();
+        break;
+    case 15:  // stage IV
+        after_solve();
+        for (unsigned i = 0; i < n; i++) d.secondSmall_sorted[i].x -= d.x[i];
         job_ = PassJob();
         lastAccepted_ = finish_step();
         break;
@@ -442,7 +494,9 @@ bool Simulation::step()  // one iteration of simulation.cpp:88-277
 {
     if (!begin_step()) return false;
     while (job_.kind != PassJob::None) {
-        check(default_engine_.run(*this, job_), "pair pass");
+        check(default_engine_.run(*this, job_), job_.kind == PassJob::Factor  ? "linear solver factorization"
+                                                : job_.kind == PassJob::Solve ? "linear solve"
+                                                                              : "pair pass");
         after_pass();
     }
     return status_ ? false : lastAccepted_;
@@ -614,6 +668,7 @@ int d4_sim_get_stats(const d4_sim* s, d4_sim_stats* st)
     st->pair_passes = s->sim->pairPasses;
     st->factorizations = s->sim->factorizations;
     st->singular_solves = s->sim->singularSolves;
+    st->unpivoted_factorizations = s->sim->unpivotedFactorizations;
     return D4_OK;
 }
 

@@ -31,6 +31,9 @@ public:
     // x = A^{-1} b; 0 = ok, < 0 = failure
     virtual int solve(const double* b, double* x) = 0;
     virtual bool singular() const = 0;
+    // true when the last factor() took no row interchanges by construction
+    // (a solver that may skip pivoting on matrices where it is not needed)
+    virtual bool unpivoted() const { return false; }
 };
 
 class SparseLU : public LinearSolver {
@@ -65,10 +68,42 @@ private:
 // minimum degree order of the symmetric pattern of A + A^T (exposed for tests)
 std::vector<int> min_degree_order(int n, const int* Ap, const int* Ai);
 
-// Dense LU with partial pivoting on the GPU (rocSOLVER getrf / getrs): the CSC
-// matrix is scattered into an n x n device matrix, factored there and solved
-// there; one stream and rocBLAS handle per calling thread.  Defined in
-// d4_gpulu.hip; null if the device cannot be used.
+// Dense LU on the GPU: the CSC matrix is scattered into a device matrix of
+// order gpu_lu_padded(n) (identity beyond n) and factored there with
+// threshold partial pivoting (u = 0.1, diagonal preferred: rocSOLVER
+// getrf_npvt, kept when every multiplier is at most 10 in magnitude, else
+// getrf), then solved there by a triangular-solve kernel; one stream and
+// rocBLAS handle per calling thread.  It is BatchDenseLU with one matrix, so a
+// matrix gets the same bits from either.  Defined in d4_gpulu.hip; null if the
+// device cannot be used.
 LinearSolver* new_gpu_dense_lu(int device);
+
+// order of the padded device matrix for an n x n system: n rounded up to a
+// multiple of 256, so that matrices of similar order share a batched call
+int gpu_lu_padded(int n);
+
+// The dense LU of new_gpu_dense_lu for K matrices at once: matrices of one
+// padded order go through one batched rocSOLVER call (getrf_npvt_batched, then
+// getrf_batched for those that need row interchanges), and all solves through
+// one launch of the triangular-solve kernel.  A matrix's factors and
+// solutions do not depend on which other matrices share the call.
+class BatchDenseLU {
+public:
+    struct Matrix {
+        int n;
+        const int* Ap;
+        const int* Ai;
+        const double* Ax;
+    };
+    // runs fn(k) for k in [0, count), possibly on several threads
+    using ParallelFor = void (*)(void* user, int count, void (*fn)(void*, int), void* arg);
+    virtual ~BatchDenseLU() = default;
+    virtual void set_parallel(ParallelFor pf, void* user) = 0;
+    // 0 = ok, < 0 = failure; singular[k] and unpivoted[k] as LinearSolver reports them
+    virtual int factor(int K, const Matrix* m, unsigned char* singular, unsigned char* unpivoted) = 0;
+    // x[k] = A_k^{-1} b[k] for the K matrices of the last factor(); 0 = ok, < 0 = failure
+    virtual int solve(int K, const double* const* b, double* const* x) = 0;
+};
+BatchDenseLU* new_batch_dense_lu(int device);
 
 }  // namespace d4host

@@ -139,6 +139,6 @@ class Ensemble:
         return {k: getattr(c, k) for k, _ in EnsConfig._fields_}, ms.value
 
     def timing(self):
-        t = np.zeros(3)
+        t = np.zeros(4)
         self._check(self.L.d4_ens_timing(self.h, t.ctypes.data))
-        return {"pass_ms": t[0], "host_ms": t[1], "total_ms": t[2]}
+        return {"pass_ms": t[0], "solve_ms": t[1], "host_ms": t[2], "total_ms": t[3]}

@@ -122,3 +122,36 @@ class Simulation:
 
     def write_dconf(self, path):
         self._check(self.L.d4_sim_write_dconf(self.h, str(path).encode()))
+
+
+def lu_solve_batch(ctx, systems):
+    """d4_lu_solve_batch (include/d4_sim.h): the GPU dense LU of solver 1 on K
+    independent systems.  systems: [(Ap, Ai, Ax, b), ...] with Ap the n + 1
+    column pointers from 0, Ai the rows and Ax the values of a CSC matrix.
+    Returns (xs, singular, unpivoted): the K solutions and two boolean arrays."""
+    K = len(systems)
+    keep = []
+
+    def arr(a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        keep.append(a)
+        return a.ctypes.data
+
+    n = np.array([len(s[0]) - 1 for s in systems], dtype=np.int32)
+    xs = [np.zeros(k) for k in n]
+    ptrs = {name: (C.c_void_p * max(K, 1))() for name in ("Ap", "Ai", "Ax", "b", "x")}
+    for k, (Ap, Ai, Ax, b) in enumerate(systems):
+        ptrs["Ap"][k] = arr(Ap, np.int32)
+        ptrs["Ai"][k] = arr(Ai, np.int32)
+        ptrs["Ax"][k] = arr(Ax, np.float64)
+        ptrs["b"][k] = arr(b, np.float64)
+        ptrs["x"][k] = xs[k].ctypes.data
+    singular = np.zeros(max(K, 1), dtype=np.uint8)
+    unpivoted = np.zeros(max(K, 1), dtype=np.uint8)
+    rc = ctx.L.d4_lu_solve_batch(ctx.h, K, n.ctypes.data, C.cast(ptrs["Ap"], C.c_void_p),
+                                 C.cast(ptrs["Ai"], C.c_void_p), C.cast(ptrs["Ax"], C.c_void_p),
+                                 C.cast(ptrs["b"], C.c_void_p), C.cast(ptrs["x"], C.c_void_p),
+                                 singular.ctypes.data, unpivoted.ctypes.data)
+    if rc:
+        raise RuntimeError(f"d4_lu_solve_batch: {ctx.L.d4_last_error(ctx.h).decode()} (code {rc})")
+    return xs, singular[:K].astype(bool), unpivoted[:K].astype(bool)

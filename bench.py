@@ -132,7 +132,11 @@ def ensemble_bench(pkg, ctx, steps, R, seed=4242):
             "batched_split_ms": {k: t_after[k] - t_before[k] for k in t_after},
             "accepted": int(sum(s["successful_steps"] for s in st_b)),
             "failed": int(sum(s["failed_steps"] for s in st_b)),
-            "launch_config": cfg, "solver": "dense LU on the GPU (rocSOLVER getrf/getrs), both paths",
+            "factorizations": int(sum(s["factorizations"] for s in st_b)),
+            "unpivoted_factorizations": int(sum(s["unpivoted_factorizations"] for s in st_b)),
+            "launch_config": cfg,
+            "solver": "dense LU on the GPU (rocSOLVER getrf_npvt on column diagonally dominant matrices, "
+                      "else getrf; getrs), both paths",
             "bitwise_equal_to_plain": equal}
 
 

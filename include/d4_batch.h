@@ -67,9 +67,11 @@ int d4_ens_get_config(const d4_ens* e, d4_ens_config* c);
 /* Times the candidate launch shapes on the batched passes of the current
  * configuration (positions are not changed) and keeps the fastest. */
 int d4_ens_autotune(d4_ens* e, d4_ens_config* best, double* best_ms);
-/* accumulated wall time (ms) since creation: [0] batched passes incl. transfers,
- * [1] per-realization host work, [2] all of d4_ens_step */
-int d4_ens_timing(const d4_ens* e, double* ms3);
+/* accumulated wall time (ms) since creation: [0] batched pair passes incl.
+ * transfers, [1] linear solves (factorizations and solves, batched on the GPU
+ * for realizations with solver 1), [2] per-realization host work, [3] all of
+ * d4_ens_step */
+int d4_ens_timing(const d4_ens* e, double* ms4);
 
 #ifdef __cplusplus
 }

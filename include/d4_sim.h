@@ -43,6 +43,8 @@ typedef struct {
     double sim_time, step_size, strain, last_max_error_ratio_sq;
     int64_t successful_steps, failed_steps;
     int64_t pair_passes, factorizations, singular_solves;
+    int64_t unpivoted_factorizations; /* of factorizations: those the GPU dense LU finished
+                                         without row interchanges (every multiplier <= 10) */
 } d4_sim_stats;
 
 /* Fills p with the reference's defaults (project_parser.cpp:44-50, 64). */
@@ -68,6 +70,19 @@ int d4_sim_positions(const d4_sim* s, double* xy, int original_order);
 int d4_sim_get_stats(const d4_sim* s, d4_sim_stats* st);
 /* writeDislocationDataToFile (simulation_data.cpp:177-191) */
 int d4_sim_write_dconf(const d4_sim* s, const char* path);
+
+/* The linear solves of solver 1 (UMFPACK's place, simulation.cpp:635-659) on
+ * K independent systems at once: matrix k is n[k] x n[k] in CSC form (Ap[k]:
+ * n[k] + 1 column pointers from 0, Ai[k] rows in [0, n[k]), Ax[k] values,
+ * duplicates not allowed), b[k] its right-hand side, x[k] its solution (may be
+ * b[k]).  Dense LU on the GPU with threshold partial pivoting (u = 0.1, the
+ * diagonal preferred); singular[k] = 1 when the factorization met a zero
+ * pivot (x[k] then holds what the solve gives, as with UMFPACK's warning),
+ * unpivoted[k] = 1 when no rows were interchanged; either may be null.  A
+ * system's solution does not depend on the other systems of the call. */
+int d4_lu_solve_batch(d4_ctx* ctx, int K, const int* n, const int* const* Ap, const int* const* Ai,
+                      const double* const* Ax, const double* const* b, double* const* x, unsigned char* singular,
+                      unsigned char* unpivoted);
 
 #ifdef __cplusplus
 }

@@ -108,7 +108,7 @@ def _check_equal(e, off, plain):
     for r, (p, s) in enumerate(plain):
         np.testing.assert_array_equal(pos[off[r]:off[r + 1]], p)
         for k in ("sim_time", "step_size", "strain", "successful_steps", "failed_steps", "pair_passes",
-                  "factorizations"):
+                  "factorizations", "unpivoted_factorizations"):
             assert st[r][k] == s[k], (r, k, st[r][k], s[k])
 
 
@@ -129,6 +129,24 @@ def test_ensemble_gpu_dense_solver_equals_plain_path(ctx):
     e = ens.Ensemble(ctx, off, x, y, b, params)
     e.step(4)
     _check_equal(e, off, _plain(ctx, off, x, y, b, params, 4))
+    e.close()
+
+
+def test_ensemble_gpu_dense_solver_padded_orders_equal_plain_path(ctx):
+    # realizations over several padded orders (256 .. 1536) so that batched LU
+    # calls hold different matrices than the plain path's one-matrix calls
+    sizes = [700, 1300, 260, 520, 1030, 264, 1290]
+    off, x, y, b = _ens_inputs(sizes, 21)
+    params = [dict(cutoff_mult=0.05 * np.sqrt(n), heaviside=True, stress=("fixed", 0.05, 0.0), solver=1)
+              for n in sizes]
+    e = ens.Ensemble(ctx, off, x, y, b, params)
+    e.step(4)
+    _check_equal(e, off, _plain(ctx, off, x, y, b, params, 4))
+    st = e.stats()
+    fact = sum(s["factorizations"] for s in st)
+    unpiv = sum(s["unpivoted_factorizations"] for s in st)
+    print(f"factorizations {fact}, unpivoted {unpiv}", flush=True)
+    assert 0 < unpiv <= fact
     e.close()
 
 

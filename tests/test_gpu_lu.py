@@ -1,0 +1,120 @@
+"""The GPU dense LU of the simulation's linear solves (d4_lu_solve_batch,
+include/d4_sim.h; UMFPACK's place, simulation.cpp:635-659) against numpy's
+LAPACK solve, its threshold pivoting rule (u = 0.1: the diagonal is kept while
+every multiplier is at most 10 in magnitude), singular matrices, invalid
+input, and the independence of a system's bits from the rest of its batch."""
+from importlib import import_module
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+simmod = import_module("2d4_amd.sim")
+
+
+def _csc(A):
+    Ap, Ai, Ax = [0], [], []
+    for c in range(A.shape[1]):
+        rows = np.nonzero(A[:, c])[0]
+        Ai.extend(rows.tolist())
+        Ax.extend(A[rows, c].tolist())
+        Ap.append(len(Ai))
+    return np.array(Ap, np.int32), np.array(Ai, np.int32), np.array(Ax, np.float64)
+
+
+def _sys(A, b):
+    return (*_csc(A), b)
+
+
+def _dominant(n, rng, density=0.2):
+    A = rng.uniform(-1, 1, (n, n)) * (rng.uniform(0, 1, (n, n)) < density) * (0.5 / n)
+    return A + np.eye(n)
+
+
+def _rel_err(x, ref):
+    return float(np.max(np.abs(x - ref)) / max(1.0, float(np.max(np.abs(ref)))))
+
+
+def _backward(A, x, b):
+    r = np.linalg.norm(A @ x - b, np.inf)
+    return float(r / (np.linalg.norm(A, np.inf) * np.linalg.norm(x, np.inf) + np.linalg.norm(b, np.inf)))
+
+
+def test_dominant_systems_match_lapack(ctx):
+    rng = np.random.default_rng(1)
+    sizes = [1, 2, 63, 64, 65, 255, 256, 257, 300, 700, 1100]
+    mats = [_dominant(n, rng) for n in sizes]
+    bs = [rng.uniform(-1, 1, n) for n in sizes]
+    xs, sing, unpiv = simmod.lu_solve_batch(ctx, [_sys(A, b) for A, b in zip(mats, bs)])
+    assert not sing.any() and unpiv.all()
+    for A, b, x in zip(mats, bs, xs):
+        err = _rel_err(x, np.linalg.solve(A, b))
+        assert err <= 1e-12, (A.shape, err)
+
+
+def test_threshold_rule_at_the_boundary(ctx):
+    # the multiplier of the first stage is exactly 10 (kept) or 10.5 (pivoted);
+    # in the 3x3 case the second stage's pivot 1e-3 gives a multiplier of 1000
+    keep = np.array([[1.0, 0.5], [10.0, 1.0]])
+    swap = np.array([[1.0, 0.5], [10.5, 1.0]])
+    late = np.array([[1.0, 0.0, 0.0], [0.0, 1e-3, 1.0], [0.0, 1.0, 1.0]])
+    mats = [keep, swap, late]
+    bs = [np.array([1.0, 2.0]), np.array([1.0, 2.0]), np.array([1.0, 2.0, 3.0])]
+    xs, sing, unpiv = simmod.lu_solve_batch(ctx, [_sys(A, b) for A, b in zip(mats, bs)])
+    assert unpiv.tolist() == [True, False, False]
+    assert not sing.any()
+    for A, b, x in zip(mats, bs, xs):
+        assert _rel_err(x, np.linalg.solve(A, b)) <= 1e-15
+
+
+def test_pivoting_systems_match_lapack(ctx):
+    rng = np.random.default_rng(2)
+    mats = [rng.standard_normal((n, n)) for n in (50, 300, 600, 1300)]
+    z = rng.standard_normal((257, 257))
+    np.fill_diagonal(z, 0.0)  # zero pivots without row interchanges
+    mats += [z, np.array([[0.0, 1.0], [1.0, 0.0]])]
+    bs = [rng.standard_normal(A.shape[0]) for A in mats]
+    xs, sing, unpiv = simmod.lu_solve_batch(ctx, [_sys(A, b) for A, b in zip(mats, bs)])
+    assert not sing.any() and not unpiv.any()
+    for A, b, x in zip(mats, bs, xs):
+        bw = _backward(A, x, b)
+        assert bw <= 1e-14, (A.shape, bw)
+        fw = _rel_err(x, np.linalg.solve(A, b))
+        assert fw <= 1e-15 * np.linalg.cond(A, np.inf) * A.shape[0], (A.shape, fw)
+
+
+def test_singular_matrix_is_reported(ctx):
+    A = _dominant(40, np.random.default_rng(3))
+    A[:, 17] = 0.0
+    _, sing, unpiv = simmod.lu_solve_batch(ctx, [_sys(A, np.ones(40)), _sys(np.eye(3), np.ones(3))])
+    assert sing.tolist() == [True, False]
+    assert unpiv.tolist() == [False, True]
+
+
+def test_batch_composition_does_not_change_bits(ctx):
+    rng = np.random.default_rng(4)
+    mats = [_dominant(n, rng) for n in (300, 1030, 64)] + [rng.standard_normal((n, n)) for n in (520, 70)]
+    systems = [_sys(A, rng.uniform(-1, 1, A.shape[0])) for A in mats]
+    together, _, unpiv = simmod.lu_solve_batch(ctx, systems)
+    assert unpiv.tolist() == [True, True, True, False, False]
+    alone = [simmod.lu_solve_batch(ctx, [s])[0][0] for s in systems]
+    extra = [_sys(_dominant(n, rng), np.ones(n)) for n in (290, 1100)]
+    mixed, _, _ = simmod.lu_solve_batch(ctx, extra[:1] + systems[::-1] + extra[1:])
+    mixed = mixed[1:-1][::-1]
+    for a, t, m in zip(alone, together, mixed):
+        np.testing.assert_array_equal(a, t)
+        np.testing.assert_array_equal(a, m)
+
+
+def test_invalid_matrices_are_rejected(ctx):
+    Ap, Ai, Ax = _csc(np.eye(4))
+    bad_row = Ai.copy()
+    bad_row[2] = 4
+    with pytest.raises(RuntimeError):
+        simmod.lu_solve_batch(ctx, [(Ap, bad_row, Ax, np.ones(4))])
+    bad_start = Ap.copy() + 1
+    with pytest.raises(RuntimeError):
+        simmod.lu_solve_batch(ctx, [(bad_start, Ai, Ax, np.ones(4))])
+    # the context stays usable
+    xs, _, _ = simmod.lu_solve_batch(ctx, [(Ap, Ai, Ax, np.arange(4.0))])
+    np.testing.assert_array_equal(xs[0], np.arange(4.0))

@@ -158,6 +158,7 @@ def test_gpu_dense_solver_each_step_c2(ctx, orc):
     bs = np.where(np.arange(len(state)) < len(state) // 2, 1.0, -1.0)
     h = 1e-6
     worst = 0.0
+    unpiv = fact = 0
     for _ in range(8):
         one = dict(C2, init_step_size=h, step_limit=0, solver=1)
         nxt, rst = orc.run(state, **_oracle_kw({k: v for k, v in one.items() if k != "solver"}))
@@ -165,9 +166,11 @@ def test_gpu_dense_solver_each_step_c2(ctx, orc):
         st = s.run()
         assert (st["successful_steps"], st["failed_steps"]) == (rst.ok_steps, rst.failed_steps)
         worst = max(worst, dconf.max_periodic_dx(s.positions(), nxt))
+        unpiv += st["unpivoted_factorizations"]
+        fact += st["factorizations"]
         s.close()
         state, h = nxt, rst.step_size
-    print(f"worst single-step deviation (GPU dense LU) {worst:.3e}", flush=True)
+    print(f"worst single-step deviation (GPU dense LU) {worst:.3e}, unpivoted {unpiv} of {fact}", flush=True)
     assert worst <= TRAJ_TOL
 
 
@@ -179,6 +182,7 @@ def test_gpu_dense_solver_adaptive_run(ctx, orc):
     st = s.run()
     ref, rst = orc.run(_sorted_xy(x, y, b), **_oracle_kw(kw))
     dev = dconf.max_periodic_dx(s.positions(), ref)
-    print(f"steps {st['successful_steps']}+{st['failed_steps']} max periodic dx {dev:.3e}", flush=True)
+    print(f"steps {st['successful_steps']}+{st['failed_steps']} max periodic dx {dev:.3e} "
+          f"unpivoted {st['unpivoted_factorizations']} of {st['factorizations']}", flush=True)
     assert (st["successful_steps"], st["failed_steps"]) == (rst.ok_steps, rst.failed_steps)
     assert dev <= TRAJ_TOL
