@@ -100,25 +100,6 @@ D4_HD double dist2(double dx, double dy)
     return dx * dx + dy * dy;
 }
 
-// One image wall from C = cosh, S = "sinh", c = cos(2 pi dy): adds w*f to sig
-// and w*f' to dsig (both before the common factor 2 pi^2); returns f.
-template <bool NEED_D>
-D4_HD double wall(double u, double C, double S, double c, double w, double& sig, double& dsig)
-{
-    double D = C - c;
-    double r = rcp(D);
-    double N = fma(C, c, -1.0);
-    double t = N * r;
-    double f = u * t * r;
-    sig = fma(w, f, sig);
-    if (NEED_D) {
-        double q = fma(-2.0, t, c);
-        double g = fma(kTwoPi * u, S * q, N);
-        dsig = fma(w, g * r * r, dsig);
-    }
-    return f;
-}
-
 // Central wall inside the series radius (Field.cpp:182-187, 224-249): the
 // same wall algebra with N, D, S*c and S replaced by their expansions.
 template <bool NEED_D>
@@ -200,29 +181,24 @@ D4_HD void wall0(double dx, double C, double S, double c, double& sig, double& d
     }
 }
 
-// Ten reciprocals from one: Montgomery's batch inversion over a balanced
-// product tree (9 products, one rcp(), 18 products back).  Each result is within
-// a few ulp of 1/x.  The products stay far inside binary64's range here
-// (wall denominators lie between ~2e-5 and ~1e15).
-D4_HD void batch_rcp10(const double* x, double* r)
+// Seven reciprocals from one: Montgomery's batch inversion over a product tree
+// (6 products, one rcp(), 12 products back).  Each result is within a few ulp
+// of 1/x.  The products stay far inside binary64's range here (the seven wall
+// denominators lie between ~2e-5 and ~2e9, their product below ~1e34).
+D4_HD void batch_rcp7(const double* x, double* r)
 {
-    const double p01 = x[0] * x[1], p23 = x[2] * x[3], p45 = x[4] * x[5], p67 = x[6] * x[7], p89 = x[8] * x[9];
-    const double p03 = p01 * p23, p47 = p45 * p67;
-    const double p07 = p03 * p47;
-    const double iall = rcp(p07 * p89);
-    const double i07 = iall * p89, i89 = iall * p07;
-    const double i03 = i07 * p47, i47 = i07 * p03;
-    const double i01 = i03 * p23, i23 = i03 * p01, i45 = i47 * p67, i67 = i47 * p45;
+    const double p01 = x[0] * x[1], p23 = x[2] * x[3], p45 = x[4] * x[5];
+    const double p03 = p01 * p23, p46 = p45 * x[6];
+    const double iall = rcp(p03 * p46);
+    const double i03 = iall * p46, i46 = iall * p03;
+    const double i01 = i03 * p23, i23 = i03 * p01, i45 = i46 * x[6];
+    r[6] = i46 * p45;
     r[0] = i01 * x[1];
     r[1] = i01 * x[0];
     r[2] = i23 * x[3];
     r[3] = i23 * x[2];
     r[4] = i45 * x[5];
     r[5] = i45 * x[4];
-    r[6] = i67 * x[7];
-    r[7] = i67 * x[6];
-    r[8] = i89 * x[9];
-    r[9] = i89 * x[8];
 }
 
 // sigma (and d sigma/dx if NEED_D) of one pair from (dx, dy) already wrapped
@@ -230,6 +206,15 @@ D4_HD void batch_rcp10(const double* x, double* r)
 // Walls (index: offset u, weight w): 0: dx + 3; 1: dx - 3; 2: dx - 4s (1);
 // 3: dx + 4s (1 - |dx|); 4: dx - 5s (|dx|); 5, 6: dx +- 2; 7, 8: dx +- 1;
 // 9: dx (the central wall, or the series when dx^2 + dy^2 <= 1e-6).
+//
+// The walls at +-4 and +-5 have C = cosh(2 pi k) E^(+-1) >= kCh4 e^-pi ~ 1.8e9,
+// so with rho = 1/C <= 5.6e-10 their terms are, exactly,
+//   f  = u rho (c - rho) / (1 - c rho)^2                      = u rho c + O(u rho^2)
+//   f' = rho [(c - rho) + 2 pi u s (c - 2t)] / (1 - c rho)^2  = rho c (1 - 2 pi s u) + O(36 rho^2)
+// (t = (c - rho)/(1 - c rho), s the sign of the "sinh", +1 for the reference's
+// cosh-as-sinh).  The remainders are below 1e-17 absolute, four orders under
+// the rounding of the near walls, so these three walls take the leading terms
+// and need no reciprocal; the other seven share one batch inversion.
 template <bool NEED_D, bool EXACT>
 D4_HD void pair_field(double dx, double dy, double E, double R, double c, double& sig_out, double& dsig_out)
 {
@@ -239,23 +224,17 @@ D4_HD void pair_field(double dx, double dy, double E, double R, double c, double
     const double ad = pos ? dx : -dx;
     const bool series = !(dist2(dx, dy) > 1e-6);
 
-    // cosh of the ten walls; offsets u and "sinh" values are formed again when used
-    const double Cp4 = kCh4 * E, Cm4 = kCh4 * R, Cp5 = kCh5 * E, Cm5 = kCh5 * R;
-    double C[10];
+    // the seven reciprocal walls, in batch order: +3, -3, +2, -2, +1, -1, 0
+    double C[7], D[7], N[7], r[7];
     C[0] = kCh3 * E;
     C[1] = kCh3 * R;
-    C[2] = pos ? Cm4 : Cp4;
-    C[3] = pos ? Cp4 : Cm4;
-    C[4] = pos ? Cm5 : Cp5;
-    C[5] = h.Cp2;
-    C[6] = h.Cm2;
-    C[7] = h.Cp1;
-    C[8] = h.Cm1;
-    C[9] = 0.5 * h.P;
-
-    double D[10], N[10], r[10];
+    C[2] = h.Cp2;
+    C[3] = h.Cm2;
+    C[4] = h.Cp1;
+    C[5] = h.Cm1;
+    C[6] = 0.5 * h.P;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
+    for (int k = 0; k < 6; ++k) {
         D[k] = C[k] - c;
         N[k] = fma(C[k], c, -1.0);
     }
@@ -263,46 +242,59 @@ D4_HD void pair_field(double dx, double dy, double E, double R, double c, double
         // central wall: C - c and C c - 1 rounded as the reference rounds them
         // (Field.cpp:178-179), since for near pairs both cancel
 #pragma clang fp contract(off)
-        D[9] = C[9] - c;
-        N[9] = C[9] * c - 1.0;
+        D[6] = C[6] - c;
+        N[6] = C[6] * c - 1.0;
     }
-    if (series) D[9] = 1.0;  // the series branch brings its own denominator
-    batch_rcp10(D, r);
+    if (series) D[6] = 1.0;  // the series branch brings its own denominator
+    batch_rcp7(D, r);
 
-    double sig = 0.0, dsig = 0.0, f3 = 0.0, f4 = 0.0;
+    double sig = 0.0, dsig = 0.0;
 #pragma unroll
-    for (int k = 0; k < 10; ++k) {
-        if (k == 9 && series) break;
-        const double uk = k == 0 ? dx + 3.0 : k == 1 ? dx - 3.0 : k == 2 ? dx - 4.0 * sg : k == 3 ? dx + 4.0 * sg
-                        : k == 4 ? dx - 5.0 * sg : k == 5 ? dx + 2.0 : k == 6 ? dx - 2.0 : k == 7 ? dx + 1.0
-                        : k == 8 ? dx - 1.0 : dx;
+    for (int k = 0; k < 7; ++k) {
+        if (k == 6 && series) break;
+        const double uk = k == 0 ? dx + 3.0 : k == 1 ? dx - 3.0 : k == 2 ? dx + 2.0 : k == 3 ? dx - 2.0
+                        : k == 4 ? dx + 1.0 : k == 5 ? dx - 1.0 : dx;
         const double r2 = r[k] * r[k];
-        const double Nr2 = N[k] * r2;
-        const double f = uk * Nr2;
-        const double w = k == 3 ? 1.0 - ad : (k == 4 ? ad : 1.0);
-        sig = fma(w, f, sig);
-        if (k == 3) f3 = f;
-        if (k == 4) f4 = f;
+        sig = fma(uk, N[k] * r2, sig);
         if (NEED_D) {
-            // sinh of the wall: identities for |k| <= 2, the cosh value itself (reference,
-            // Field.cpp:269-283) or its shift-signed copy (EXACT) beyond
+            // sinh of the wall: the cosh value itself at +-3 (the reference's
+            // cosh-as-sinh, Field.cpp:269-283) or its shift-signed copy (EXACT);
+            // the identities for |k| <= 2
             double Sk;
-            if (k == 9) Sk = 0.5 * h.M;
-            else if (k == 5) Sk = h.Sp2;
-            else if (k == 6) Sk = h.Sm2;
-            else if (k == 7) Sk = h.Sp1;
-            else if (k == 8) Sk = h.Sm1;
-            else if (!EXACT) Sk = C[k];
-            else Sk = (k == 0 ? 1.0 : k == 1 ? -1.0 : k == 3 ? sg : -sg) * C[k];
+            if (k == 0) Sk = C[0];
+            else if (k == 1) Sk = EXACT ? -C[1] : C[1];
+            else if (k == 2) Sk = h.Sp2;
+            else if (k == 3) Sk = h.Sm2;
+            else if (k == 4) Sk = h.Sp1;
+            else if (k == 5) Sk = h.Sm1;
+            else Sk = 0.5 * h.M;
             const double q = fma(-2.0, N[k] * r[k], c);
             const double g = fma(kTwoPi * uk, Sk * q, N[k]);
-            dsig = fma(w * g, r2, dsig);
+            dsig = fma(g, r2, dsig);
         }
     }
     if (series) wall0_series<NEED_D>(dx, dy, sig, dsig);
 
+    // the walls at -4s (weight 1), +4s (1 - |dx|) and -5s (|dx|)
+    constexpr double kIch4 = 1.0 / kCh4, kIch5 = 1.0 / kCh5;
+    const double rho_p4 = kIch4 * R, rho_m4 = kIch4 * E, rho_p5 = kIch5 * R, rho_m5 = kIch5 * E;
+    const double u2 = dx - 4.0 * sg, u3 = dx + 4.0 * sg, u4 = dx - 5.0 * sg;
+    const double rc2 = (pos ? rho_m4 : rho_p4) * c;
+    const double rc3 = (pos ? rho_p4 : rho_m4) * c;
+    const double rc4 = (pos ? rho_m5 : rho_p5) * c;
+    const double f3 = u3 * rc3, f4 = u4 * rc4;
+    sig = fma(u2, rc2, sig);
+    sig = fma(1.0 - ad, f3, sig);
+    sig = fma(ad, f4, sig);
     sig_out = kTwoPiSq * sig;
-    if (NEED_D) dsig_out = kTwoPiSq * fma(sg, f4 - f3, dsig);
+    if (NEED_D) {
+        // s = +1 (reference); EXACT: the sign of the shift
+        const double s2 = EXACT ? -sg : 1.0, s3 = EXACT ? sg : 1.0, s4 = EXACT ? -sg : 1.0;
+        dsig = fma(rc2, fma(-kTwoPi * s2, u2, 1.0), dsig);
+        dsig = fma((1.0 - ad) * rc3, fma(-kTwoPi * s3, u3, 1.0), dsig);
+        dsig = fma(ad * rc4, fma(-kTwoPi * s4, u4, 1.0), dsig);
+        dsig_out = kTwoPiSq * fma(sg, f4 - f3, dsig);
+    }
 }
 
 // The reference's transcendental inputs of a pair (simulation.cpp:456-457,
