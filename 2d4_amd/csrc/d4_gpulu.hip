@@ -320,6 +320,14 @@ public:
         lds_order_ = std::max(0, (lds / 8 - kSB * kSB) / 256 * 256);
     }
 
+    ~BatchLU() override
+    {
+        for (rocblas_handle h : side_h_) (void)rocblas_destroy_handle(h);
+        for (hipStream_t st : side_) (void)hipStreamDestroy(st);
+        for (hipEvent_t e : ev_)
+            if (e) (void)hipEventDestroy(e);
+    }
+
     void set_parallel(ParallelFor pf, void* user) override
     {
         pf_ = pf;
@@ -406,10 +414,11 @@ public:
             return -2;
         if (assemble(desc_.p, K, npad_max, st)) return -2;
         if (hipMemsetAsync(info, 0, sizeof(int) * 2 * K, st) != hipSuccess) return -2;
-        for (const Group& g : groups)
-            if (rocsolver_dgetrf_npvt_batched(r->handle, g.npad, g.npad, ptrA_.p + g.first, g.npad, info + g.first,
-                                              g.count) != rocblas_status_success)
-                return -2;
+        if (run_groups(groups, r, [&](rocblas_handle h, const Group& g) {
+                return rocsolver_dgetrf_npvt_batched(h, g.npad, g.npad, ptrA_.p + g.first, g.npad, info + g.first,
+                                                     g.count);
+            }))
+            return -2;
         for (int k0 = 0; k0 < K; k0 += kGridY) {
             const int kc = std::min(kGridY, K - k0);
             if (npad_max > 0)
@@ -449,11 +458,12 @@ public:
                 return -2;
             if (assemble(desc_.p + K, F, npad2, st)) return -2;
             if (hipMemsetAsync(info2, 0, sizeof(int) * F, st) != hipSuccess) return -2;
-            for (const Group& g : groups2)
-                if (rocsolver_dgetrf_batched(r->handle, g.npad, g.npad, ptrA_.p + K + g.first, g.npad,
-                                             ipiv_.p + desc_h_[order2[g.first]].ipiv_off, g.npad, info2 + g.first,
-                                             g.count) != rocblas_status_success)
-                    return -2;
+            if (run_groups(groups2, r, [&](rocblas_handle h, const Group& g) {
+                    return rocsolver_dgetrf_batched(h, g.npad, g.npad, ptrA_.p + K + g.first, g.npad,
+                                                    ipiv_.p + desc_h_[order2[g.first]].ipiv_off, g.npad,
+                                                    info2 + g.first, g.count);
+                }))
+                return -2;
             if (npad2 <= lds_bytes_ / 8)
                 hipLaunchKernelGGL(pivots_to_perm_kernel<true>, dim3(F), dim3(256), sizeof(int) * 2 * npad2, st,
                                    desc_.p + K, ipiv_.p, iscr_.p);
@@ -528,8 +538,14 @@ private:
         int first, count, npad;
     };
 
+    static constexpr int kSide = 3;  // side streams for the group calls of one factorization
+
     int device_;
     int lds_bytes_ = 65536, lds_order_ = 0;
+    std::vector<hipStream_t> side_;
+    std::vector<rocblas_handle> side_h_;
+    std::vector<hipEvent_t> ev_;  // [0]: inputs ready on the caller's stream, [1 + s]: side stream s done
+    bool side_failed_ = false;
     ParallelFor pf_ = nullptr;
     void* user_ = nullptr;
     int K_ = 0, nsolve_ = 0, npad_max_ = 0;
@@ -566,6 +582,62 @@ private:
             g.back().count++;
         }
         return g;
+    }
+
+    bool ensure_side()
+    {
+        if (!side_.empty()) return true;
+        if (side_failed_) return false;
+        std::vector<hipStream_t> st(kSide, nullptr);
+        std::vector<rocblas_handle> h(kSide, nullptr);
+        std::vector<hipEvent_t> ev(kSide + 1, nullptr);
+        bool ok = hipSetDevice(device_) == hipSuccess;
+        for (int i = 0; ok && i < kSide; ++i)
+            ok = hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking) == hipSuccess &&
+                 rocblas_create_handle(&h[i]) == rocblas_status_success &&
+                 rocblas_set_stream(h[i], st[i]) == rocblas_status_success;
+        for (int i = 0; ok && i <= kSide; ++i) ok = hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) == hipSuccess;
+        if (!ok) {
+            for (rocblas_handle x : h)
+                if (x) (void)rocblas_destroy_handle(x);
+            for (hipStream_t x : st)
+                if (x) (void)hipStreamDestroy(x);
+            for (hipEvent_t x : ev)
+                if (x) (void)hipEventDestroy(x);
+            side_failed_ = true;
+            return false;
+        }
+        side_ = st;
+        side_h_ = h;
+        ev_ = ev;
+        return true;
+    }
+
+    // call(handle, group) for every group: one group runs on the caller's
+    // stream; several are dealt round-robin (largest first) over that stream
+    // and kSide side streams, which start after the work already queued on the
+    // caller's stream and finish before anything queued on it afterwards.
+    // Each matrix goes through the same call whichever stream runs it.
+    template <class Call>
+    int run_groups(const std::vector<Group>& groups, ThreadRes* r, Call call)
+    {
+        if (groups.size() <= 1 || !ensure_side()) {
+            for (const Group& g : groups)
+                if (call(r->handle, g) != rocblas_status_success) return -2;
+            return 0;
+        }
+        if (hipEventRecord(ev_[0], r->stream) != hipSuccess) return -2;
+        for (int i = 0; i < kSide; ++i)
+            if (hipStreamWaitEvent(side_[i], ev_[0], 0) != hipSuccess) return -2;
+        for (size_t i = 0; i < groups.size(); ++i) {
+            const int lane = (int)(i % (kSide + 1));
+            if (call(lane == 0 ? r->handle : side_h_[lane - 1], groups[i]) != rocblas_status_success) return -2;
+        }
+        for (int i = 0; i < kSide; ++i)
+            if (hipEventRecord(ev_[1 + i], side_[i]) != hipSuccess ||
+                hipStreamWaitEvent(r->stream, ev_[1 + i], 0) != hipSuccess)
+                return -2;
+        return 0;
     }
 
     int assemble(const MatDesc* desc, int K, int npad_max, hipStream_t st)
