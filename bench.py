@@ -104,9 +104,16 @@ def ensemble_bench(pkg, ctx, steps, R, seed=4242, plain=True):
     cfg, tune_ms = e.autotune()
     log(f"[bench] ensemble autotuned {cfg} ({tune_ms:.2f} ms per speed+Jacobian pass)")
     t_before = e.timing()
-    t0 = time.perf_counter()
-    e.step(steps)
-    t_batched = time.perf_counter() - t0
+    t_batched = 0.0
+    done = 0
+    while done < steps:  # chunks of at most 20 iterations, one progress line each
+        chunk = min(20, steps - done)
+        t0 = time.perf_counter()
+        e.step(chunk)
+        t_batched += time.perf_counter() - t0
+        done += chunk
+        if steps > 20:
+            log(f"[bench] ensemble: {done}/{steps} lockstep iterations, {t_batched:.1f} s")
     t_after = e.timing()
     pos_b = e.positions()
     st_b = e.stats()
