@@ -12,21 +12,24 @@
 // rounded up to 256), the CSC entries scattered into its leading n_k x n_k
 // block and the identity beyond it.  The padding changes nothing in the
 // solution: its rows and columns are decoupled from the leading block and
-// never enter a pivot choice.  It lets matrices of similar order share one
-// batched rocSOLVER call (pointer arrays), and rocSOLVER's batched routines
-// treat each matrix of a batch alone, so a matrix's factors do not depend on
-// the batch it is in (bitwise; the ensemble tests check it).  GpuDenseLU is
-// BatchLU with K = 1.
+// never enter a pivot choice.  It gives every matrix an order that is a
+// multiple of the factorization's block and lets matrices of one order share
+// a batched rocSOLVER call (pointer arrays) in the pivoted fallback, whose
+// batched routines treat each matrix of a batch alone.  A matrix's factors
+// do not depend on the batch it is in (bitwise; the ensemble tests check it).
+// GpuDenseLU is BatchLU with K = 1.
 //
 // Pivoting: threshold partial pivoting with u = 0.1 and a preference for the
 // diagonal (0.1 is UMFPACK's default relative pivot tolerance; UMFPACK is the
-// reference's solver, and the host SparseLU here uses the same threshold).  Every matrix is first
-// factored without row interchanges (getrf_npvt_batched, 2-4x faster than
-// getrf on these sizes); if every pivot is finite and nonzero and every
+// reference's solver, and the host SparseLU here uses the same threshold).
+// Every matrix is first factored without row interchanges by this file's
+// blocked LU (lu_diag_kernel, lu_panel_kernel, lu_update_kernel: three
+// launches per 64 columns for the whole ragged batch, the trailing update on
+// the FP64 matrix cores); if every pivot is finite and nonzero and every
 // multiplier |l_rc| <= 1/u = 10, the diagonal was an acceptable pivot at every
 // stage and the factorization stands.  Otherwise the matrix is scattered again
-// and factored with partial pivoting (getrf_batched).  The choice depends on
-// the matrix values only.
+// and factored with partial pivoting (rocSOLVER getrf_batched).  The choice
+// depends on the matrix values only.
 //
 // Solves: a triangular-solve kernel of this file (lu_solve_kernel), one
 // workgroup per matrix for all matrices of the batch in one launch, in place
@@ -298,6 +301,178 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
     for (int i = tid; i < n; i += kSolveThreads) bm[i] = x[i];
 }
 
+// ---------------------------------------------------------------------------
+// Blocked right-looking LU without row interchanges for a ragged batch (every
+// matrix of order npad, a multiple of kFB).  Step p (j0 = kFB p) covers the
+// matrices with npad > j0, a prefix of the batch sorted by npad, largest first:
+//   lu_diag_kernel    factors the diagonal block A11 = L11 U11 (one workgroup per matrix, LDS);
+//   lu_panel_kernel   L21 = A21 U11^-1 and U12 = L11^-1 A12, one workgroup per kFB x kFB block;
+//   lu_update_kernel  A22 -= L21 U12, one workgroup per kFB x kFB tile, on the matrix cores.
+// The order of operations for an entry depends on its matrix alone (the
+// workgroups of other matrices only add parallelism), so a matrix gets the
+// same factors in any batch.
+constexpr int kFB = 64;       // block order
+constexpr int kFLd = kFB + 1;  // LDS column stride (odd: conflict-free column and row walks)
+
+struct FacDesc {
+    long long a_off;
+    int npad;
+    int pad_;
+};
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+// Work item of this workgroup such that each of the 8 XCDs (which receive the
+// workgroups of a launch round-robin) runs a contiguous range of items: the
+// tiles of one block row or column then share that XCD's L2.  A bijection on
+// [0, gridDim.x) whatever the dispatch order.
+__device__ inline int xcd_item()
+{
+    const int n = gridDim.x, x = blockIdx.x, xcd = x & 7, idx = x >> 3;
+    const int q = n >> 3, r = n & 7;
+    return xcd < r ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+}
+
+// index of the matrix owning work item g: pre[i] <= g < pre[i + 1], pre[0] = 0
+__device__ inline int owner(const int* __restrict__ pre, int count, int g)
+{
+    int lo = 0, hi = count;  // pre[lo] <= g < pre[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pre[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void lu_diag_kernel(const FacDesc* __restrict__ fd, double* __restrict__ A, int j0)
+{
+    __shared__ double T[kFB * kFLd];
+    const FacDesc m = fd[blockIdx.x];
+    const size_t lda = (size_t)m.npad;
+    double* blk = A + m.a_off + (size_t)j0 * lda + j0;
+    const int t = threadIdx.x, rr = t & 63, cg = t >> 6;
+    for (int c = cg; c < kFB; c += 4) T[c * kFLd + rr] = blk[(size_t)c * lda + rr];
+    __syncthreads();
+    for (int j = 0; j < kFB - 1; ++j) {
+        // every thread of row rr > j forms the same multiplier; the first writes it
+        const double l = rr > j ? T[j * kFLd + rr] / T[j * kFLd + j] : 0.0;
+        __syncthreads();
+        if (rr > j) {
+            if (cg == 0) T[j * kFLd + rr] = l;
+            for (int c = j + 1 + cg; c < kFB; c += 4) T[c * kFLd + rr] = fma(-l, T[c * kFLd + j], T[c * kFLd + rr]);
+        }
+        __syncthreads();
+    }
+    for (int c = cg; c < kFB; c += 4) blk[(size_t)c * lda + rr] = T[c * kFLd + rr];
+}
+
+__global__ __launch_bounds__(256) void lu_panel_kernel(const FacDesc* __restrict__ fd, const int* __restrict__ pre,
+                                                       int count, double* __restrict__ A, int j0)
+{
+    __shared__ double Dg[kFB * kFLd], X[kFB * kFLd];
+    const int g = xcd_item();
+    const int i = owner(pre, count, g);
+    const FacDesc m = fd[i];
+    const size_t lda = (size_t)m.npad;
+    const int T = m.npad / kFB - j0 / kFB - 1;  // trailing blocks
+    const int item = g - pre[i];
+    const bool rowblk = item < T;                      // L21 block (below) or U12 block (right)
+    const int b = (rowblk ? item : item - T) + 1;      // block index past the diagonal
+    const double* dg = A + m.a_off + (size_t)j0 * lda + j0;
+    double* blk = rowblk ? A + m.a_off + (size_t)j0 * lda + (j0 + kFB * b)
+                         : A + m.a_off + (size_t)(j0 + kFB * b) * lda + j0;
+    const int t = threadIdx.x, lo = t & 63, q = t >> 6;
+    for (int c = q; c < kFB; c += 4) {
+        Dg[c * kFLd + lo] = dg[(size_t)c * lda + lo];
+        X[c * kFLd + lo] = blk[(size_t)c * lda + lo];
+    }
+    __syncthreads();
+    if (rowblk) {
+        // X U11 = B: row lo of X, columns left to right.  The four threads of a
+        // row all divide column c; its result is stored a step later (after the
+        // barrier), when no thread reads that column any more.
+        double xprev = 0.0;
+        for (int c = 0; c < kFB; ++c) {
+            const double xc = X[c * kFLd + lo] / Dg[c * kFLd + c];
+            if (q == 0 && c > 0) X[(c - 1) * kFLd + lo] = xprev;
+            for (int c2 = c + 1 + q; c2 < kFB; c2 += 4) X[c2 * kFLd + lo] = fma(-xc, Dg[c2 * kFLd + c], X[c2 * kFLd + lo]);
+            xprev = xc;
+            __syncthreads();
+        }
+        if (q == 0) X[(kFB - 1) * kFLd + lo] = xprev;
+        __syncthreads();
+    } else {
+        // L11 X = B (unit lower): column lo of X, rows top to bottom
+        for (int r = 0; r < kFB - 1; ++r) {
+            const double xr = X[lo * kFLd + r];
+            for (int r2 = r + 1 + q; r2 < kFB; r2 += 4) X[lo * kFLd + r2] = fma(-Dg[r * kFLd + r2], xr, X[lo * kFLd + r2]);
+            __syncthreads();
+        }
+    }
+    for (int c = q; c < kFB; c += 4) blk[(size_t)c * lda + lo] = X[c * kFLd + lo];
+}
+
+// C -= L21 U12 for one kFB x kFB tile, computed transposed on the matrix cores:
+// C^T -= U12^T L21^T with v_mfma_f64_16x16x4f64 (lane l: A[l % 16][l / 16],
+// B[l / 16][l % 16], D[4 e + l / 16][l % 16] for e = 0..3), so that C and the
+// L21 operand are read down columns (coalesced) and only U12 goes through LDS.
+// Wave w owns the 32 x 32 quadrant rows 32 (w >> 1), columns 32 (w & 1) of C.
+__global__ __launch_bounds__(256) void lu_update_kernel(const FacDesc* __restrict__ fd, const int* __restrict__ pre,
+                                                        int count, double* __restrict__ A, int j0)
+{
+    __shared__ double U[kFB * kFLd];  // U12 tile, U[k * kFLd + c]
+    const int g = xcd_item();
+    const int i = owner(pre, count, g);
+    const FacDesc m = fd[i];
+    const size_t lda = (size_t)m.npad;
+    const int T = m.npad / kFB - j0 / kFB - 1;
+    const int item = g - pre[i];
+    const int bi = item / T + 1, bj = item % T + 1;
+    const size_t r0 = (size_t)j0 + kFB * bi, c0 = (size_t)j0 + kFB * bj;
+    const double* Ab = A + m.a_off;
+    const int t = threadIdx.x, lo = t & 63, q = t >> 6;
+    for (int c = q; c < kFB; c += 4) U[lo * kFLd + c] = Ab[(c0 + c) * lda + j0 + lo];  // U12[k = lo][c]
+    const int w = t >> 6, l = t & 63, lr = l & 15, lk = l >> 4;
+    const int rw = 32 * (w >> 1), cw = 32 * (w & 1);
+    double* Cb = A + m.a_off;
+    f64x4 acc[2][2];  // [tc][tr]: C^T tile (c = cw + 16 tc + 4 e + lk, r = rw + 16 tr + lr)
+#pragma unroll
+    for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+        for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                acc[tc][tr][e] = Cb[(c0 + cw + 16 * tc + 4 * e + lk) * lda + r0 + rw + 16 * tr + lr];
+    __syncthreads();
+    // the L21 operands of all 16 k-steps first (independent loads in flight together)
+    const double* L = Ab + (size_t)j0 * lda + r0 + rw + lr;  // L21[r][k] at L[k * lda + 16 tr]
+    double bl[kFB / 4][2];
+#pragma unroll
+    for (int s = 0; s < kFB / 4; ++s) {
+        const size_t k = 4 * s + lk;
+        bl[s][0] = L[k * lda];
+        bl[s][1] = L[k * lda + 16];
+    }
+#pragma unroll
+    for (int s = 0; s < kFB / 4; ++s) {
+        const int k = 4 * s + lk;
+        const double a0 = U[k * kFLd + cw + lr], a1 = U[k * kFLd + cw + 16 + lr];  // U12^T[c][k]
+        const double b0 = -bl[s][0], b1 = -bl[s][1];                                 // -L21^T[k][r]
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+#pragma unroll
+    for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+        for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                Cb[(c0 + cw + 16 * tc + 4 * e + lk) * lda + r0 + rw + 16 * tr + lr] = acc[tc][tr][e];
+}
+
 // a valid CSC matrix: Ap[0] = 0, nondecreasing column pointers, rows in [0, n)
 bool valid_csc(const BatchDenseLU::Matrix& m)
 {
@@ -375,10 +550,9 @@ public:
         if (nz_off[K] > INT_MAX || col_total > INT_MAX || piv_total > INT_MAX / 2) return -1;
         const long long nnz = nz_off[K];
 
-        // every matrix first without pivoting, in groups of one padded order
-        // (largest first: that is also the solve's launch order)
+        // every matrix first without pivoting (largest first: that is also the
+        // solve's launch order)
         std::vector<int> order = by_order(K, [](int k) { return k; });
-        std::vector<Group> groups = make_groups(order);
         std::vector<int> pos(K);
         for (int i = 0; i < K; ++i) pos[order[i]] = i;
 
@@ -414,11 +588,7 @@ public:
             return -2;
         if (assemble(desc_.p, K, npad_max, st)) return -2;
         if (hipMemsetAsync(info, 0, sizeof(int) * 2 * K, st) != hipSuccess) return -2;
-        if (run_groups(groups, r, [&](rocblas_handle h, const Group& g) {
-                return rocsolver_dgetrf_npvt_batched(h, g.npad, g.npad, ptrA_.p + g.first, g.npad, info + g.first,
-                                                     g.count);
-            }))
-            return -2;
+        if (factor_unpivoted(order, st)) return -2;
         for (int k0 = 0; k0 < K; k0 += kGridY) {
             const int kc = std::min(kGridY, K - k0);
             if (npad_max > 0)
@@ -556,6 +726,8 @@ private:
     DevBuf<int> ipiv_, iscr_, info_, colp_, rowi_;
     DevBuf<double*> ptrA_;
     DevBuf<MatDesc> desc_;
+    DevBuf<FacDesc> fdesc_;
+    DevBuf<int> fpre_;
     DevBuf<SolveDesc> sdesc_;
     PinBuf<int> colp_h_, rowi_h_, info_h_;
     PinBuf<double> vals_h_, bh_;
@@ -582,6 +754,57 @@ private:
             g.back().count++;
         }
         return g;
+    }
+
+    // the blocked no-pivot LU of every matrix, in `order` (largest first):
+    // three launches per step of kFB columns, each over all matrices still active
+    int factor_unpivoted(const std::vector<int>& order, hipStream_t st)
+    {
+        std::vector<FacDesc> fd;
+        for (int k : order)
+            if (desc_h_[k].npad > 0) fd.push_back({desc_h_[k].a_off, desc_h_[k].npad, 0});
+        const int Ka = (int)fd.size();
+        if (Ka == 0) return 0;
+        const int P = fd[0].npad / kFB;
+        struct Step {
+            int act, pre2, n2, pre3, n3;
+        };
+        std::vector<Step> steps(P);
+        std::vector<int> pre;
+        for (int p = 0; p < P; ++p) {
+            int act = 0;
+            while (act < Ka && fd[act].npad > kFB * p) ++act;
+            Step& sp = steps[p];
+            sp.act = act;
+            sp.pre2 = (int)pre.size();
+            pre.push_back(0);
+            for (int i = 0; i < act; ++i) pre.push_back(pre.back() + 2 * (fd[i].npad / kFB - p - 1));
+            sp.n2 = pre.back();
+            sp.pre3 = (int)pre.size();
+            pre.push_back(0);
+            for (int i = 0; i < act; ++i) {
+                const long long T = fd[i].npad / kFB - p - 1;
+                if (pre.back() + T * T > INT_MAX) return -1;
+                pre.push_back(pre.back() + (int)(T * T));
+            }
+            sp.n3 = pre.back();
+        }
+        if (!fdesc_.ensure(Ka) || !fpre_.ensure(pre.size())) return -1;
+        if (hipMemcpyAsync(fdesc_.p, fd.data(), sizeof(FacDesc) * Ka, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(fpre_.p, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+            return -1;
+        for (int p = 0; p < P; ++p) {
+            const Step& sp = steps[p];
+            const int j0 = kFB * p;
+            hipLaunchKernelGGL(lu_diag_kernel, dim3(sp.act), dim3(256), 0, st, fdesc_.p, A_.p, j0);
+            if (sp.n2 > 0)
+                hipLaunchKernelGGL(lu_panel_kernel, dim3(sp.n2), dim3(256), 0, st, fdesc_.p, fpre_.p + sp.pre2, sp.act,
+                                   A_.p, j0);
+            if (sp.n3 > 0)
+                hipLaunchKernelGGL(lu_update_kernel, dim3(sp.n3), dim3(256), 0, st, fdesc_.p, fpre_.p + sp.pre3,
+                                   sp.act, A_.p, j0);
+        }
+        return hipGetLastError() != hipSuccess ? -1 : 0;
     }
 
     bool ensure_side()

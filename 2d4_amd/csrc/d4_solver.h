@@ -70,10 +70,10 @@ std::vector<int> min_degree_order(int n, const int* Ap, const int* Ai);
 
 // Dense LU on the GPU: the CSC matrix is scattered into a device matrix of
 // order gpu_lu_padded(n) (identity beyond n) and factored there with
-// threshold partial pivoting (u = 0.1, diagonal preferred: rocSOLVER
-// getrf_npvt, kept when every multiplier is at most 10 in magnitude, else
-// getrf), then solved there by a triangular-solve kernel; one stream and
-// rocBLAS handle per calling thread.  It is BatchDenseLU with one matrix, so a
+// threshold partial pivoting (u = 0.1, diagonal preferred: a blocked LU
+// without interchanges, kept when every multiplier is at most 10 in
+// magnitude, else rocSOLVER getrf), then solved there by a triangular-solve
+// kernel; one stream and rocBLAS handle per calling thread.  It is BatchDenseLU with one matrix, so a
 // matrix gets the same bits from either.  Defined in d4_gpulu.hip; null if the
 // device cannot be used.
 LinearSolver* new_gpu_dense_lu(int device);
@@ -82,10 +82,11 @@ LinearSolver* new_gpu_dense_lu(int device);
 // multiple of 256, so that matrices of similar order share a batched call
 int gpu_lu_padded(int n);
 
-// The dense LU of new_gpu_dense_lu for K matrices at once: matrices of one
-// padded order go through one batched rocSOLVER call (getrf_npvt_batched, then
-// getrf_batched for those that need row interchanges), and all solves through
-// one launch of the triangular-solve kernel.  A matrix's factors and
+// The dense LU of new_gpu_dense_lu for K matrices at once: the unpivoted
+// factorization covers every matrix in the same launches (three per 64
+// columns), the matrices that need row interchanges go through rocSOLVER
+// getrf_batched per padded order, and all solves through one launch of the
+// triangular-solve kernel.  A matrix's factors and
 // solutions do not depend on which other matrices share the call.
 class BatchDenseLU {
 public:
