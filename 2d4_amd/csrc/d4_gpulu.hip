@@ -19,15 +19,17 @@
 // do not depend on the batch it is in (bitwise; the ensemble tests check it).
 // GpuDenseLU is BatchLU with K = 1.
 //
-// Pivoting: threshold partial pivoting with u = 0.1 and a preference for the
-// diagonal (0.1 is UMFPACK's default relative pivot tolerance; UMFPACK is the
-// reference's solver, and the host SparseLU here uses the same threshold).
-// Every matrix is first factored without row interchanges by this file's
-// blocked LU (lu_diag_kernel, lu_panel_kernel, lu_update_kernel: three
-// launches per 64 columns for the whole ragged batch, the trailing update on
-// the FP64 matrix cores); if every pivot is finite and nonzero and every
-// multiplier |l_rc| <= 1/u = 10, the diagonal was an acceptable pivot at every
-// stage and the factorization stands.  Otherwise the matrix is scattered again
+// Pivoting: threshold partial pivoting with a preference for the diagonal and
+// u = 0.001, the diagonal tolerance of UMFPACK's symmetric strategy
+// (Control[UMFPACK_SYM_PIVOT_TOLERANCE]); UMFPACK, the reference's solver,
+// selects that strategy by default for a symmetric nonzero pattern with a
+// nonzero diagonal, which these step matrices have.  Every matrix is first
+// factored without row interchanges by this file's blocked LU
+// (lu_diag_kernel, lu_panel_kernel, lu_update_kernel: three launches per 64
+// columns for the whole ragged batch, the trailing update on the FP64 matrix
+// cores); if every pivot is finite and nonzero and every multiplier
+// |l_rc| <= 1/u = 1000, the diagonal was an acceptable pivot at every stage
+// and the factorization stands.  Otherwise the matrix is scattered again
 // and factored with partial pivoting (rocSOLVER getrf_batched).  The choice
 // depends on the matrix values only.
 //
@@ -144,7 +146,7 @@ struct SolveDesc {
     int perm_off;  // its row permutation, or -1 when rows stayed in place
 };
 
-constexpr double kMaxMultiplier = 10.0;  // threshold partial pivoting with u = 0.1
+constexpr double kMaxMultiplier = 1000.0;  // threshold partial pivoting with u = 0.001
 constexpr int kSB = 64;                  // block of the triangular solves
 constexpr int kSolveThreads = 1024;
 
@@ -172,7 +174,8 @@ __global__ void scatter_kernel(const MatDesc* __restrict__ desc, const int* __re
 
 // after the no-pivot factorization: flag[k] = 1 unless every pivot is finite
 // and nonzero and every multiplier |l_rc| <= kMaxMultiplier, i.e. unless
-// threshold partial pivoting with u = 0.1 would keep the diagonal at every stage
+// threshold partial pivoting with u = 1/kMaxMultiplier would keep the diagonal
+// at every stage
 __global__ void multiplier_check_kernel(const MatDesc* __restrict__ desc, const double* __restrict__ A,
                                         int* __restrict__ flag)
 {

@@ -70,9 +70,9 @@ std::vector<int> min_degree_order(int n, const int* Ap, const int* Ai);
 
 // Dense LU on the GPU: the CSC matrix is scattered into a device matrix of
 // order gpu_lu_padded(n) (identity beyond n) and factored there with
-// threshold partial pivoting (u = 0.1, diagonal preferred: a blocked LU
-// without interchanges, kept when every multiplier is at most 10 in
-// magnitude, else rocSOLVER getrf), then solved there by a triangular-solve
+// threshold partial pivoting (u = 0.001, diagonal preferred, as UMFPACK's
+// symmetric strategy: a blocked LU without interchanges, kept when every
+// multiplier is at most 1000 in magnitude, else rocSOLVER getrf), then solved there by a triangular-solve
 // kernel; one stream and rocBLAS handle per calling thread.  It is BatchDenseLU with one matrix, so a
 // matrix gets the same bits from either.  Defined in d4_gpulu.hip; null if the
 // device cannot be used.

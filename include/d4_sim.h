@@ -44,7 +44,7 @@ typedef struct {
     int64_t successful_steps, failed_steps;
     int64_t pair_passes, factorizations, singular_solves;
     int64_t unpivoted_factorizations; /* of factorizations: those the GPU dense LU finished
-                                         without row interchanges (every multiplier <= 10) */
+                                         without row interchanges (every multiplier <= 1000) */
 } d4_sim_stats;
 
 /* Fills p with the reference's defaults (project_parser.cpp:44-50, 64). */
@@ -75,8 +75,8 @@ int d4_sim_write_dconf(const d4_sim* s, const char* path);
  * K independent systems at once: matrix k is n[k] x n[k] in CSC form (Ap[k]:
  * n[k] + 1 column pointers from 0, Ai[k] rows in [0, n[k]), Ax[k] values,
  * duplicates not allowed), b[k] its right-hand side, x[k] its solution (may be
- * b[k]).  Dense LU on the GPU with threshold partial pivoting (u = 0.1, the
- * diagonal preferred); singular[k] = 1 when the factorization met a zero
+ * b[k]).  Dense LU on the GPU with threshold partial pivoting (u = 0.001, the
+ * diagonal preferred, as UMFPACK's symmetric strategy); singular[k] = 1 when the factorization met a zero
  * pivot (x[k] then holds what the solve gives, as with UMFPACK's warning),
  * unpivoted[k] = 1 when no rows were interchanged; either may be null.  A
  * system's solution does not depend on the other systems of the call. */

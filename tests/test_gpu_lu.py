@@ -1,8 +1,9 @@
 """The GPU dense LU of the simulation's linear solves (d4_lu_solve_batch,
 include/d4_sim.h; UMFPACK's place, simulation.cpp:635-659) against numpy's
-LAPACK solve, its threshold pivoting rule (u = 0.1: the diagonal is kept while
-every multiplier is at most 10 in magnitude), singular matrices, invalid
-input, and the independence of a system's bits from the rest of its batch."""
+LAPACK solve, its threshold pivoting rule (u = 0.001, UMFPACK's symmetric
+strategy: the diagonal is kept while every multiplier is at most 1000 in
+magnitude), singular matrices, invalid input, and the independence of a
+system's bits from the rest of its batch."""
 from importlib import import_module
 
 import numpy as np
@@ -31,6 +32,13 @@ def _dominant(n, rng, density=0.2):
     return A + np.eye(n)
 
 
+def _needs_pivoting(n, rng):
+    # first-stage multipliers |a_i0 / a_00| far above 1000
+    A = rng.standard_normal((n, n))
+    A[np.diag_indices(n)] *= 1e-7
+    return A
+
+
 def _rel_err(x, ref):
     return float(np.max(np.abs(x - ref)) / max(1.0, float(np.max(np.abs(ref)))))
 
@@ -53,11 +61,11 @@ def test_dominant_systems_match_lapack(ctx):
 
 
 def test_threshold_rule_at_the_boundary(ctx):
-    # the multiplier of the first stage is exactly 10 (kept) or 10.5 (pivoted);
-    # in the 3x3 case the second stage's pivot 1e-3 gives a multiplier of 1000
-    keep = np.array([[1.0, 0.5], [10.0, 1.0]])
-    swap = np.array([[1.0, 0.5], [10.5, 1.0]])
-    late = np.array([[1.0, 0.0, 0.0], [0.0, 1e-3, 1.0], [0.0, 1.0, 1.0]])
+    # the multiplier of the first stage is exactly 1000 (kept) or 1000.5
+    # (pivoted); in the 3x3 case the second stage's pivot 1e-4 gives 1e4
+    keep = np.array([[1.0, 0.5], [1000.0, 1.0]])
+    swap = np.array([[1.0, 0.5], [1000.5, 1.0]])
+    late = np.array([[1.0, 0.0, 0.0], [0.0, 1e-4, 1.0], [0.0, 1.0, 1.0]])
     mats = [keep, swap, late]
     bs = [np.array([1.0, 2.0]), np.array([1.0, 2.0]), np.array([1.0, 2.0, 3.0])]
     xs, sing, unpiv = simmod.lu_solve_batch(ctx, [_sys(A, b) for A, b in zip(mats, bs)])
@@ -69,7 +77,7 @@ def test_threshold_rule_at_the_boundary(ctx):
 
 def test_pivoting_systems_match_lapack(ctx):
     rng = np.random.default_rng(2)
-    mats = [rng.standard_normal((n, n)) for n in (50, 300, 600, 1300)]
+    mats = [_needs_pivoting(n, rng) for n in (50, 300, 600, 1300)]
     z = rng.standard_normal((257, 257))
     np.fill_diagonal(z, 0.0)  # zero pivots without row interchanges
     mats += [z, np.array([[0.0, 1.0], [1.0, 0.0]])]
@@ -83,6 +91,20 @@ def test_pivoting_systems_match_lapack(ctx):
         assert fw <= 1e-15 * np.linalg.cond(A, np.inf) * A.shape[0], (A.shape, fw)
 
 
+def test_general_matrices_within_backward_bound(ctx):
+    # random matrices that the rule may keep without interchanges (multipliers
+    # below 1000, growth up to a few thousand): backward stable to 1e-12
+    rng = np.random.default_rng(6)
+    mats = [rng.standard_normal((n, n)) for n in (50, 50, 100, 100, 300, 300, 600)]
+    bs = [rng.standard_normal(A.shape[0]) for A in mats]
+    xs, sing, unpiv = simmod.lu_solve_batch(ctx, [_sys(A, b) for A, b in zip(mats, bs)])
+    assert not sing.any()
+    print(f"kept without interchanges: {int(unpiv.sum())} of {len(mats)}", flush=True)
+    for A, b, x in zip(mats, bs, xs):
+        bw = _backward(A, x, b)
+        assert bw <= 1e-12, (A.shape, bw)
+
+
 def test_singular_matrix_is_reported(ctx):
     A = _dominant(40, np.random.default_rng(3))
     A[:, 17] = 0.0
@@ -93,7 +115,7 @@ def test_singular_matrix_is_reported(ctx):
 
 def test_batch_composition_does_not_change_bits(ctx):
     rng = np.random.default_rng(4)
-    mats = [_dominant(n, rng) for n in (300, 1030, 64)] + [rng.standard_normal((n, n)) for n in (520, 70)]
+    mats = [_dominant(n, rng) for n in (300, 1030, 64)] + [_needs_pivoting(n, rng) for n in (520, 70)]
     systems = [_sys(A, rng.uniform(-1, 1, A.shape[0])) for A in mats]
     together, _, unpiv = simmod.lu_solve_batch(ctx, systems)
     assert unpiv.tolist() == [True, True, True, False, False]
