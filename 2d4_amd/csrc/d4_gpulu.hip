@@ -373,7 +373,7 @@ __global__ __launch_bounds__(256) void lu_diag_kernel(const FacDesc* __restrict_
 __global__ __launch_bounds__(256) void lu_panel_kernel(const FacDesc* __restrict__ fd, const int* __restrict__ pre,
                                                        int count, double* __restrict__ A, int j0)
 {
-    __shared__ double Dg[kFB * kFLd], X[kFB * kFLd];
+    __shared__ double Dg[kFB * kFLd], X[kFB * kFLd], Rd[kFB];
     const int g = xcd_item();
     const int i = owner(pre, count, g);
     const FacDesc m = fd[i];
@@ -390,14 +390,15 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(const FacDesc* __restrict
         Dg[c * kFLd + lo] = dg[(size_t)c * lda + lo];
         X[c * kFLd + lo] = blk[(size_t)c * lda + lo];
     }
+    if (rowblk && q == 0) Rd[lo] = 1.0 / dg[(size_t)lo * lda + lo];  // 1 / U11(lo, lo), off the step chain
     __syncthreads();
     if (rowblk) {
         // X U11 = B: row lo of X, columns left to right.  The four threads of a
-        // row all divide column c; its result is stored a step later (after the
-        // barrier), when no thread reads that column any more.
+        // row all scale column c by 1/U11(c, c); its result is stored a step
+        // later (after the barrier), when no thread reads that column any more.
         double xprev = 0.0;
         for (int c = 0; c < kFB; ++c) {
-            const double xc = X[c * kFLd + lo] / Dg[c * kFLd + c];
+            const double xc = X[c * kFLd + lo] * Rd[c];
             if (q == 0 && c > 0) X[(c - 1) * kFLd + lo] = xprev;
             for (int c2 = c + 1 + q; c2 < kFB; c2 += 4) X[c2 * kFLd + lo] = fma(-xc, Dg[c2 * kFLd + c], X[c2 * kFLd + lo]);
             xprev = xc;
