@@ -331,6 +331,36 @@ for _f in _table_methods():
     setattr(Context, _f.__name__, _f)
 
 
+def _lu_methods():
+    """The simulation's GPU dense LU on a dense device matrix (added to Context below)."""
+
+    def lu_factor_dense_t(self, A, transpose=False):
+        """Factors the (n, n) float64 CUDA tensor A (row-major), or its transpose; returns
+        (singular, unpivoted).  Replaces the context's previous factorization."""
+        n = A.shape[0]
+        if A.dim() != 2 or A.shape[1] != n or A.stride(1) != 1:
+            raise ValueError("A must be a square row-major (n, n) tensor")
+        ld = A.stride(0)
+        rs, cs = (1, ld) if transpose else (ld, 1)
+        s, u = C.c_int(0), C.c_int(0)
+        self.check(self.L.d4_lu_factor_dense_dev(self.h, n, A.data_ptr(), rs, cs, C.byref(s), C.byref(u)))
+        return bool(s.value), bool(u.value)
+
+    def lu_solve_dense_t(self, b, out=None):
+        """x = A^-1 b for the matrix of the last lu_factor_dense_t (b: float64 CUDA vector)."""
+        import torch
+        b = b.contiguous()
+        x = out if out is not None else torch.empty_like(b)
+        self.check(self.L.d4_lu_solve_dense_dev(self.h, b.numel(), b.data_ptr(), x.data_ptr()))
+        return x
+
+    return [lu_factor_dense_t, lu_solve_dense_t]
+
+
+for _f in _lu_methods():
+    setattr(Context, _f.__name__, _f)
+
+
 def jac_params(h=0.0, tau_a=0.0, tau_b=0.0, n=None, cutoff_mult=None, heaviside=False, wfunc="c", tol_cap2=0.0,
                cutoff=None, exact=False):
     """Builds d4_jac_params; cutoff from the multiplier as SimulationData::updateCutOff (simulation_data.cpp:253-262)."""

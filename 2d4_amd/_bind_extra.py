@@ -42,6 +42,8 @@ SIGNATURES = {
     "d4_sim_get_stats": (C.c_int, [_vp, _P(SimStats)]),
     "d4_sim_write_dconf": (C.c_int, [_vp, C.c_char_p]),
     "d4_lu_solve_batch": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "d4_lu_factor_dense_dev": (C.c_int, [_vp, C.c_int, _vp, C.c_longlong, C.c_longlong, _P(C.c_int), _P(C.c_int)]),
+    "d4_lu_solve_dense_dev": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     # include/d4_batch.h
     "d4_batch_speed_pass": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "d4_batch_jacobian_pass": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _pi,

@@ -11,7 +11,8 @@ Jacobian); the forward map differentiated here is defined by this module:
     the same with d sigma/dy for dL/dy_j, and dL/dtau = sum_i b_i w_i.
   * ImplicitStep: one backward-Euler step x' = x + dt v(x', y, tau), solved by
     Newton's method with the dense Jacobian from the pair kernel
-    (M = -dt dv/dx, so the Newton matrix is I + M).  Its backward follows from
+    (M = -dt dv/dx, so the Newton matrix is I + M), each Newton system solved
+    by the library's GPU dense LU (d4_lu_factor_dense_dev).  Its backward follows from
     the implicit function theorem: mu = (I + M)^{-T} lambda at the converged x',
     dL/dx = mu, dL/dy = dt (dv/dy)^T mu, dL/dtau = dt b^T mu -- the last two
     from the VJP kernel.
@@ -66,13 +67,22 @@ class PairSpeed(torch.autograd.Function):
 
 
 def _newton_matrix(d4, x, y, tau, dt):
-    """(v(x), I + M) with M = -dt dv/dx (true derivative), in (row, column) orientation."""
+    """(v(x), A) with A = I + M, M = -dt dv/dx (true derivative), stored column-major as the
+    kernel writes it: the tensor's [col, row] entry is A's (row, col)."""
     n = x.numel()
     p = _pkg().jac_params(h=dt, tau_a=tau, tau_b=tau, n=n, exact=True)
-    vA, _, M, _ = d4.jacobian_dense_t(_xy(x, y), p)
-    A = M.t().contiguous()  # the kernel writes column-major: M[col, row]
+    vA, _, A, _ = d4.jacobian_dense_t(_xy(x, y), p)
     A.diagonal().add_(1.0)
     return vA, A
+
+
+def _solve(d4, A, rhs, transpose=False):
+    """A^-1 rhs (or A^-T rhs) with the library's GPU dense LU; A column-major as _newton_matrix
+    returns it.  A singular matrix is a Newton failure."""
+    singular, _ = d4.lu_factor_dense_t(A, transpose=not transpose)  # the tensor read row-major is A^T
+    if singular:
+        raise NewtonFailure("singular Newton matrix")
+    return d4.lu_solve_dense_t(rhs)
 
 
 class NewtonFailure(RuntimeError):
@@ -108,10 +118,7 @@ class ImplicitStep(torch.autograd.Function):
         v, A = _newton_matrix(d4, xn, y, t, dt)
         G = xn - x - dt * v
         for it in range(max_iter):
-            try:
-                delta = torch.linalg.solve(A, G)
-            except RuntimeError as e:  # a singular Newton matrix
-                raise NewtonFailure(str(e)) from e
+            delta = _solve(d4, A, G)
             if not bool(torch.isfinite(delta).all()):
                 raise NewtonFailure("non-finite Newton correction")
             if float(delta.abs().max()) <= tol:  # converged: the remaining error is O(|delta|^2)
@@ -150,7 +157,7 @@ class ImplicitStep(torch.autograd.Function):
     def backward(ctx, lam):
         xn, y = ctx.saved_tensors
         _, A = _newton_matrix(ctx.d4, xn, y, ctx.tau, ctx.dt)
-        mu = torch.linalg.solve(A.t(), lam.contiguous())
+        mu = _solve(ctx.d4, A, lam.contiguous(), transpose=True)
         _, gy, gt = ctx.d4.vjp_t(_xy(xn, y), mu.contiguous(), exact=True, want_dy=True)
         return mu, ctx.dt * gy, torch.tensor(ctx.dt * gt, dtype=torch.float64, device=xn.device), \
             None, None, None, None

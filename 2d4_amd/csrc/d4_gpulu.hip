@@ -172,6 +172,19 @@ __global__ void scatter_kernel(const MatDesc* __restrict__ desc, const int* __re
     for (int q = colp[m.col_off + c]; q < colp[m.col_off + c + 1]; ++q) col[rowi[q]] = vals[q];
 }
 
+// one dense matrix already on the device into the padded store: element (r, c)
+// of the source at src[r * rs + c * cs]; identity beyond n
+__global__ void dense_copy_kernel(const MatDesc* __restrict__ desc, double* __restrict__ A,
+                                  const double* __restrict__ src, long long rs, long long cs)
+{
+    const MatDesc m = desc[blockIdx.y];
+    const int c = blockIdx.x;
+    if (c >= m.npad) return;
+    double* col = A + m.a_off + (size_t)c * m.npad;
+    for (int r = threadIdx.x; r < m.npad; r += blockDim.x)
+        col[r] = (r < m.n && c < m.n) ? src[r * rs + c * cs] : (r == c ? 1.0 : 0.0);
+}
+
 // after the no-pivot factorization: flag[k] = 1 unless every pivot is finite
 // and nonzero and every multiplier |l_rc| <= kMaxMultiplier, i.e. unless
 // threshold partial pivoting with u = 1/kMaxMultiplier would keep the diagonal
@@ -515,14 +528,47 @@ public:
 
     int factor(int K, const Matrix* m, unsigned char* singular, unsigned char* unpivoted) override
     {
+        dense_ = DenseSrc();
+        return factor_impl(K, m, singular, unpivoted);
+    }
+
+    int factor_dense_dev(int n, const double* a, long long rs, long long cs, unsigned char* singular,
+                         unsigned char* unpivoted) override
+    {
+        K_ = 0;
+        if (n <= 0 || n > kMaxOrder || !a) return -1;
+        const Matrix m{n, nullptr, nullptr, nullptr};
+        dense_ = DenseSrc{a, rs, cs};
+        const int rc = factor_impl(1, &m, singular, unpivoted);
+        dense_ = DenseSrc();
+        return rc;
+    }
+
+    int solve_dense_dev(int n, const double* b, double* x) override
+    {
+        if (K_ != 1 || n != (int)b_off_[1] || !b || !x) return -1;
+        ThreadRes* r = thread_res(device_);
+        if (!r || hipSetDevice(device_) != hipSuccess) return -2;
+        hipStream_t st = r->stream;
+        if (hipMemcpyAsync(b_.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st) != hipSuccess) return -2;
+        if (launch_solve(n, st) || hipMemcpyAsync(x, b_.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return -2;
+        return 0;
+    }
+
+    int factor_impl(int K, const Matrix* m, unsigned char* singular, unsigned char* unpivoted)
+    {
         K_ = 0;
         if (K < 0 || (K > 0 && !m)) return -1;
         if (K == 0) return 0;
         if (K > kMaxMatrices) return -1;
+        const bool dense = dense_.a != nullptr;
         for (int k = 0; k < K; ++k)
-            if (m[k].n < 0 || (m[k].n > 0 && (!m[k].Ap || !m[k].Ai || !m[k].Ax)) || m[k].n > kMaxOrder) return -1;
+            if (m[k].n < 0 || (!dense && m[k].n > 0 && (!m[k].Ap || !m[k].Ai || !m[k].Ax)) || m[k].n > kMaxOrder)
+                return -1;
         std::vector<unsigned char> ok(K, 1);
-        par(K, [&](int k) { ok[k] = m[k].n == 0 || valid_csc(m[k]); });
+        if (!dense) par(K, [&](int k) { ok[k] = m[k].n == 0 || valid_csc(m[k]); });
         for (int k = 0; k < K; ++k)
             if (!ok[k]) return -1;
         ThreadRes* r = thread_res(device_);
@@ -547,7 +593,7 @@ public:
             a_total += (size_t)npad * npad;
             col_total += n + 1;
             piv_total += npad;
-            nz_off[k + 1] = nz_off[k] + (n > 0 ? m[k].Ap[n] : 0);
+            nz_off[k + 1] = nz_off[k] + (n > 0 && !dense ? m[k].Ap[n] : 0);
             b_off_[k + 1] = b_off_[k] + n;
             npad_max = std::max(npad_max, npad);
         }
@@ -572,7 +618,7 @@ public:
         int* flag = info_.p + K;      // [K, 2K): check (caller's order)
         int* info2 = info_.p + 2 * K; // [2K, 3K): pivoted pass (its group order)
 
-        par(K, [&](int k) {
+        if (!dense) par(K, [&](int k) {
             const int n = m[k].n;
             if (n == 0) return;
             const long long base = nz_off[k];
@@ -584,7 +630,8 @@ public:
         });
         for (int i = 0; i < K; ++i) ptr_h_.p[i] = A_.p + desc_h_[order[i]].a_off;
         if (hipMemcpyAsync(desc_.p, desc_h_.data(), sizeof(MatDesc) * K, hipMemcpyHostToDevice, st) != hipSuccess ||
-            hipMemcpyAsync(colp_.p, colp_h_.p, sizeof(int) * col_total, hipMemcpyHostToDevice, st) != hipSuccess ||
+            (!dense &&
+             hipMemcpyAsync(colp_.p, colp_h_.p, sizeof(int) * col_total, hipMemcpyHostToDevice, st) != hipSuccess) ||
             (nnz > 0 &&
              (hipMemcpyAsync(rowi_.p, rowi_h_.p, sizeof(int) * nnz, hipMemcpyHostToDevice, st) != hipSuccess ||
               hipMemcpyAsync(vals_.p, vals_h_.p, sizeof(double) * nnz, hipMemcpyHostToDevice, st) != hipSuccess)) ||
@@ -688,14 +735,7 @@ public:
         par(K, [&](int k) { std::copy(b[k], b[k] + ndesc_[k], bh_.p + b_off_[k]); });
         hipStream_t st = r->stream;
         if (hipMemcpyAsync(b_.p, bh_.p, sizeof(double) * total, hipMemcpyHostToDevice, st) != hipSuccess) return -2;
-        if (npad_max_ <= lds_order_)
-            hipLaunchKernelGGL(lu_solve_kernel<true>, dim3(nsolve_), dim3(kSolveThreads),
-                               sizeof(double) * (kSB * kSB + npad_max_), st, sdesc_.p, A_.p, ipiv_.p, b_.p,
-                               b_.p + total);
-        else
-            hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(nsolve_), dim3(kSolveThreads), sizeof(double) * kSB * kSB,
-                               st, sdesc_.p, A_.p, ipiv_.p, b_.p, b_.p + total);
-        if (hipGetLastError() != hipSuccess ||
+        if (launch_solve(total, st) ||
             hipMemcpyAsync(bh_.p, b_.p, sizeof(double) * total, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
             return -2;
@@ -729,6 +769,11 @@ private:
     DevBuf<double> A_, vals_, b_;
     DevBuf<int> ipiv_, iscr_, info_, colp_, rowi_;
     DevBuf<double*> ptrA_;
+    struct DenseSrc {
+        const double* a = nullptr;
+        long long rs = 0, cs = 0;
+    };
+    DenseSrc dense_;  // factor_dense_dev's source while it runs (assemble() copies from it)
     DevBuf<MatDesc> desc_;
     DevBuf<FacDesc> fdesc_;
     DevBuf<int> fpre_;
@@ -867,9 +912,29 @@ private:
         return 0;
     }
 
+    // the solve kernel over the right-hand sides packed in b_ (total entries; x in global scratch after them
+    // when the matrices are too large for LDS)
+    int launch_solve(long long total, hipStream_t st)
+    {
+        if (nsolve_ == 0) return 0;
+        if (npad_max_ <= lds_order_)
+            hipLaunchKernelGGL(lu_solve_kernel<true>, dim3(nsolve_), dim3(kSolveThreads),
+                               sizeof(double) * (kSB * kSB + npad_max_), st, sdesc_.p, A_.p, ipiv_.p, b_.p,
+                               b_.p + total);
+        else
+            hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(nsolve_), dim3(kSolveThreads), sizeof(double) * kSB * kSB,
+                               st, sdesc_.p, A_.p, ipiv_.p, b_.p, b_.p + total);
+        return hipGetLastError() != hipSuccess ? -1 : 0;
+    }
+
     int assemble(const MatDesc* desc, int K, int npad_max, hipStream_t st)
     {
         if (npad_max == 0) return 0;
+        if (dense_.a) {  // one matrix from the device
+            hipLaunchKernelGGL(dense_copy_kernel, dim3(npad_max, 1), dim3(256), 0, st, desc, A_.p, dense_.a, dense_.rs,
+                               dense_.cs);
+            return hipGetLastError() != hipSuccess ? -1 : 0;
+        }
         for (int k0 = 0; k0 < K; k0 += kGridY) {
             const int kc = std::min(kGridY, K - k0);
             hipLaunchKernelGGL(fill_identity_kernel, dim3(npad_max, kc), dim3(256), 0, st, desc + k0, A_.p);
@@ -925,6 +990,36 @@ BatchDenseLU* new_batch_dense_lu(int device) { return new BatchLU(device); }
 }  // namespace d4host
 
 void batch_lu_free(d4host::BatchDenseLU* lu) { delete lu; }
+
+extern "C" int d4_lu_factor_dense_dev(d4_ctx* ctx, int n, const double* a, long long row_stride,
+                                      long long col_stride, int* singular, int* unpivoted)
+{
+    if (!ctx) return D4_ERR_INVALID;
+    if (n <= 0 || !a) return ctx->fail(D4_ERR_INVALID, "d4_lu_factor_dense_dev: n <= 0 or null matrix");
+    if (!ctx->lu) ctx->lu = d4host::new_batch_dense_lu(ctx->device);
+    if (hipDeviceSynchronize() != hipSuccess)  // the matrix may come from any stream
+        return ctx->fail(D4_ERR_HIP, "d4_lu_factor_dense_dev: device synchronization failed");
+    unsigned char s = 0, u = 0;
+    const int rc = ctx->lu->factor_dense_dev(n, a, row_stride, col_stride, &s, &u);
+    if (rc == -1) return ctx->fail(D4_ERR_INVALID, "d4_lu_factor_dense_dev: invalid order");
+    if (rc < 0) return ctx->fail(D4_ERR_HIP, "d4_lu_factor_dense_dev: factorization failed on the GPU");
+    if (singular) *singular = s;
+    if (unpivoted) *unpivoted = u;
+    return D4_OK;
+}
+
+extern "C" int d4_lu_solve_dense_dev(d4_ctx* ctx, int n, const double* b, double* x)
+{
+    if (!ctx) return D4_ERR_INVALID;
+    if (!ctx->lu || !b || !x) return ctx->fail(D4_ERR_INVALID, "d4_lu_solve_dense_dev: nothing factored or null vector");
+    if (hipDeviceSynchronize() != hipSuccess)
+        return ctx->fail(D4_ERR_HIP, "d4_lu_solve_dense_dev: device synchronization failed");
+    const int rc = ctx->lu->solve_dense_dev(n, b, x);
+    if (rc == -1)
+        return ctx->fail(D4_ERR_INVALID, "d4_lu_solve_dense_dev: n differs from the last d4_lu_factor_dense_dev");
+    if (rc < 0) return ctx->fail(D4_ERR_HIP, "d4_lu_solve_dense_dev: solve failed on the GPU");
+    return D4_OK;
+}
 
 extern "C" int d4_lu_solve_batch(d4_ctx* ctx, int K, const int* n, const int* const* Ap, const int* const* Ai,
                                  const double* const* Ax, const double* const* b, double* const* x,

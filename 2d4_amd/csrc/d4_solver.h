@@ -104,6 +104,12 @@ public:
     virtual int factor(int K, const Matrix* m, unsigned char* singular, unsigned char* unpivoted) = 0;
     // x[k] = A_k^{-1} b[k] for the K matrices of the last factor(); 0 = ok, < 0 = failure
     virtual int solve(int K, const double* const* b, double* const* x) = 0;
+    // one dense n x n matrix on the device, element (r, c) at a[r * rs + c * cs]; the
+    // same factorization as factor() gives that matrix in CSC form
+    virtual int factor_dense_dev(int n, const double* a, long long rs, long long cs, unsigned char* singular,
+                                 unsigned char* unpivoted) = 0;
+    // x = A^{-1} b for the matrix of the last factor_dense_dev(); b and x on the device
+    virtual int solve_dense_dev(int n, const double* b, double* x) = 0;
 };
 BatchDenseLU* new_batch_dense_lu(int device);
 

@@ -84,6 +84,17 @@ int d4_lu_solve_batch(d4_ctx* ctx, int K, const int* n, const int* const* Ap, co
                       const double* const* Ax, const double* const* b, double* const* x, unsigned char* singular,
                       unsigned char* unpivoted);
 
+/* The same solver on one dense n x n matrix already on the device: element (r, c) at
+ * a[r * row_stride + c * col_stride] (a row-major array: n, 1; its transpose: 1, n).  No reference
+ * counterpart: the adjoint's Newton steps use it.  The factorization replaces the context's previous one
+ * (of this call or of d4_lu_solve_batch); *singular and *unpivoted as there (either may be null).  Both
+ * calls synchronize the device first (the inputs may come from any stream) and return with the results
+ * complete. */
+int d4_lu_factor_dense_dev(d4_ctx* ctx, int n, const double* a, long long row_stride, long long col_stride,
+                           int* singular, int* unpivoted);
+/* x = A^-1 b for the matrix of the last d4_lu_factor_dense_dev; b and x on the device (may be the same) */
+int d4_lu_solve_dense_dev(d4_ctx* ctx, int n, const double* b, double* x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -140,3 +140,25 @@ def test_invalid_matrices_are_rejected(ctx):
     # the context stays usable
     xs, _, _ = simmod.lu_solve_batch(ctx, [(Ap, Ai, Ax, np.arange(4.0))])
     np.testing.assert_array_equal(xs[0], np.arange(4.0))
+
+
+def test_dense_device_matrix_matches_csc_path(ctx):
+    # the same matrices as a dense CUDA tensor (row-major, and read transposed)
+    # give the bits of the CSC entry point, and solve against numpy's LAPACK
+    import torch
+    rng = np.random.default_rng(9)
+    for A in (_dominant(300, rng), _needs_pivoting(130, rng)):
+        n = A.shape[0]
+        b = rng.standard_normal(n)
+        (x_csc,), _, (unpiv,) = simmod.lu_solve_batch(ctx, [_sys(A, b)])
+        At = torch.from_numpy(A).to("cuda:0")
+        bt = torch.from_numpy(b).to("cuda:0")
+        sing, u = ctx.lu_factor_dense_t(At)
+        assert not sing and u == bool(unpiv)
+        x = ctx.lu_solve_dense_t(bt).cpu().numpy()
+        np.testing.assert_array_equal(x, x_csc)
+        assert _backward(A, x, b) <= 1e-14
+        # the transpose of the same storage: A^T x = b
+        ctx.lu_factor_dense_t(At, transpose=True)
+        xt = ctx.lu_solve_dense_t(bt).cpu().numpy()
+        assert _backward(A.T, xt, b) <= 1e-14
