@@ -215,9 +215,13 @@ D4_HD void batch_rcp7(const double* x, double* r)
 // cosh-as-sinh).  The remainders are below 1e-17 absolute, four orders under
 // the rounding of the near walls, so these three walls take the leading terms
 // and need no reciprocal; the other seven share one batch inversion.
+// With NEED_D, want_d = false skips the derivative (a per-lane choice: where
+// lanes of a wavefront differ, the shared sigma work is done once).
 template <bool NEED_D, bool EXACT>
-D4_HD void pair_field(double dx, double dy, double E, double R, double c, double& sig_out, double& dsig_out)
+D4_HD void pair_field(double dx, double dy, double E, double R, double c, double& sig_out, double& dsig_out,
+                      bool want_d = true)
 {
+    const bool dd = NEED_D && want_d;
     const Hyp h = hyperbolics(E, R);
     const bool pos = dx >= 0.0;
     const double sg = pos ? 1.0 : -1.0;
@@ -256,7 +260,7 @@ D4_HD void pair_field(double dx, double dy, double E, double R, double c, double
                         : k == 4 ? dx + 1.0 : k == 5 ? dx - 1.0 : dx;
         const double r2 = r[k] * r[k];
         sig = fma(uk, N[k] * r2, sig);
-        if (NEED_D) {
+        if (dd) {
             // sinh of the wall: the cosh value itself at +-3 (the reference's
             // cosh-as-sinh, Field.cpp:269-283) or its shift-signed copy (EXACT);
             // the identities for |k| <= 2
@@ -273,7 +277,10 @@ D4_HD void pair_field(double dx, double dy, double E, double R, double c, double
             dsig = fma(g, r2, dsig);
         }
     }
-    if (series) wall0_series<NEED_D>(dx, dy, sig, dsig);
+    if (series) {
+        if (dd) wall0_series<true>(dx, dy, sig, dsig);
+        else wall0_series<false>(dx, dy, sig, dsig);
+    }
 
     // the walls at -4s (weight 1), +4s (1 - |dx|) and -5s (|dx|)
     constexpr double kIch4 = 1.0 / kCh4, kIch5 = 1.0 / kCh5;
@@ -287,7 +294,7 @@ D4_HD void pair_field(double dx, double dy, double E, double R, double c, double
     sig = fma(1.0 - ad, f3, sig);
     sig = fma(ad, f4, sig);
     sig_out = kTwoPiSq * sig;
-    if (NEED_D) {
+    if (dd) {
         // s = +1 (reference); EXACT: the sign of the shift
         const double s2 = EXACT ? -sg : 1.0, s3 = EXACT ? sg : 1.0, s4 = EXACT ? -sg : 1.0;
         dsig = fma(rc2, fma(-kTwoPi * s2, u2, 1.0), dsig);

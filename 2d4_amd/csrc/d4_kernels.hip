@@ -27,6 +27,22 @@ __device__ __forceinline__ void atomic_min_nonneg(double* p, double v)
     atomicMin(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(__double_as_longlong(v)));
 }
 
+// The wave's buffered COO entries (cbuf[0, fill)) to the global list: one
+// atomic reserves their places, the lanes copy them; entries past the
+// capacity are counted but not stored (the host grows the list and repeats).
+__device__ __forceinline__ void coo_flush(const PassArgs& a, CooEntry* cbuf, int fill, int lane)
+{
+    __syncthreads();  // the workgroup is this one wave: the buffer's LDS writes are complete
+    int b0 = 0;
+    if (lane == 0) b0 = atomicAdd(a.coo_count, fill);
+    b0 = __shfl(b0, 0);
+    for (int k = lane; k < fill; k += 64) {
+        const int pos = b0 + k;
+        if (pos < a.coo_cap) a.coo[pos] = cbuf[k];
+    }
+    __syncthreads();  // every lane has read the buffer before it is refilled
+}
+
 // One 64 x 64 tile of a pass, evaluated by one wavefront (the LDS arrays are
 // per-workgroup allocations of this function's instantiation).
 template <int MODE, bool EXACT>
@@ -42,6 +58,11 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
     __shared__ double sx[kTile], sy[kTile], sw[kVjp ? kTile : 1];
     __shared__ double jq0[kTile], jq1[NQ == 2 ? kTile : 1];
     __shared__ double stage[kDense ? 4 * 16 * 17 : 1];
+    // COO mode: entries gathered in LDS and appended to the global list with
+    // one atomic per flush (a full buffer or the tile's end), not per step
+    constexpr int kCooBuf = 256;
+    __shared__ CooEntry cbuf[kCoo ? kCooBuf : 1];
+    int cfill = 0;  // entries in cbuf (the same in every lane)
 
     const int lane = threadIdx.x;
     const int base = a.off ? a.off[td.r] : 0;
@@ -117,16 +138,16 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                     q1 -= V;
                     atomicAdd(&jq1[jl], -V);
                 } else if (kCut) {
+                    // sigma for every pair, d sigma only inside the storage radius:
+                    // one evaluation whose derivative part runs for the lanes that need it
                     inr = d2 < rp.range_sq;
+                    pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig, inr);
                     if (inr) {
-                        pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
                         V = rp.h * bj * bi * dsig;
                         if (!rp.heaviside && d2 > rp.cut_sq) {  // simulation.cpp:469-470
                             const double e = sqrt(d2) - rp.cutoff;
                             V *= exp(-(e * e) * rp.one_per);
                         }
-                    } else {
-                        pair_field<false, false>(dx, dy, E, R, c, sig, dsig);
                     }
                     const double f = sig * bi * bj;
                     q0 += f;
@@ -150,22 +171,22 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                 }
             }
             if (kCoo) {
-                // wave-aggregated append: one atomic per wave, lane offsets from the ballot prefix
+                // lane offsets from the ballot prefix into the wave's LDS buffer
                 const unsigned long long mask = __ballot(inr);
                 if (mask) {
                     const int cnt = __popcll(mask);
-                    const int leader = __ffsll((long long)mask) - 1;
-                    int b0 = 0;
-                    if (lane == leader) b0 = atomicAdd(a.coo_count, cnt);
-                    b0 = __shfl(b0, leader);
-                    const int pos = b0 + __popcll(mask & ((1ull << lane) - 1ull));
-                    if (inr && pos < a.coo_cap) {
+                    if (cfill + cnt > kCooBuf) {
+                        coo_flush(a, cbuf, cfill, lane);
+                        cfill = 0;
+                    }
+                    if (inr) {
                         CooEntry e;
                         e.i = base + i;
                         e.j = base + j;
                         e.v = V;
-                        a.coo[pos] = e;
+                        cbuf[cfill + __popcll(mask & ((1ull << lane) - 1ull))] = e;
                     }
+                    cfill += cnt;
                 }
             }
             if (kDense) stage[((lane >> 4) * 16 + (lane & 15)) * 17 + ((lane & 15) ^ tt)] = V;
@@ -202,6 +223,7 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
             __syncthreads();
         }
     }
+    if (kCoo && cfill > 0) coo_flush(a, cbuf, cfill, lane);
     __syncthreads();
     double* P = a.part + (size_t)td.slot * NQ * 128;
     P[lane] = q0;
