@@ -162,3 +162,19 @@ def test_dense_device_matrix_matches_csc_path(ctx):
         ctx.lu_factor_dense_t(At, transpose=True)
         xt = ctx.lu_solve_dense_t(bt).cpu().numpy()
         assert _backward(A.T, xt, b) <= 1e-14
+
+
+def test_large_and_empty_systems_in_one_batch(ctx):
+    # order 4096 (64 block steps of the factorization) beside an empty system
+    # and a 1 x 1 one; the empty system's solution is empty
+    rng = np.random.default_rng(10)
+    big = _dominant(4096, rng, density=0.05)
+    b_big = rng.uniform(-1, 1, 4096)
+    empty = (np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0), np.zeros(0))
+    one = (np.array([0, 1], np.int32), np.array([0], np.int32), np.array([4.0]), np.array([2.0]))
+    xs, sing, unpiv = simmod.lu_solve_batch(ctx, [_sys(big, b_big), empty, one])
+    assert not sing.any() and unpiv.all()
+    assert xs[1].size == 0
+    assert xs[2][0] == 0.5
+    assert _rel_err(xs[0], np.linalg.solve(big, b_big)) <= 1e-12
+    assert _backward(big, xs[0], b_big) <= 1e-13  # the scale of n u = 4096 * 1.1e-16
