@@ -77,6 +77,13 @@ def build_test_helpers(verbose=False):
         _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", os.path.join(tdir, "libsim_cpu.so"),
               sim_shim, os.path.join(CSRC, "d4_sim.cpp"), os.path.join(CSRC, "d4_solver.cpp"), orc_obj, "-lm"],
              verbose)
+        # the same loop with the reference's optimisation flags (CMakeLists.txt:6; a portable -march in
+        # place of native): the CPU side of bench.py's implicit-step measurement
+        fast = ["-Ofast", "-ffast-math", "-march=x86-64-v3", "-fPIC"]
+        orc_fast = os.path.join(tdir, "d4_oracle_sim_fast.o")
+        _run(["gcc"] + fast + ["-std=c11", "-c", "-o", orc_fast, orc_src], verbose)
+        _run(["g++"] + fast + ["-std=c++17", "-shared", "-o", os.path.join(tdir, "libsim_cpu_fast.so"), sim_shim,
+              os.path.join(CSRC, "d4_sim.cpp"), os.path.join(CSRC, "d4_solver.cpp"), orc_fast, "-lm"], verbose)
     return out
 
 

@@ -154,6 +154,63 @@ def ensemble_bench(pkg, ctx, steps, R, seed=4242, plain=True):
             "bitwise_equal_to_plain": equal}
 
 
+def implicit_step_bench(pkg, ctx, gpu_steps=50, cpu_steps=10, with_cpu=True, seed=77):
+    """ms per implicit step (one loop iteration of Simulation::run, stages I-IV) of the plain
+    drop-in on the N = 1024 workload: uniform positions, heaviside cutoff 0.05 (-u 1.6 -h),
+    tau = 0.1 constant (-r 0 -s 0.1), default precision control.
+
+    GPU: pair passes on the GPU with the linear solves on the host (sparse LU, UMFPACK's place)
+    and with the GPU dense LU.  CPU: the same host loop with the oracle's single-threaded passes
+    and the host sparse LU (tests/csrc/libsim_cpu_fast.so, built with the reference's flags).
+    The roofline fraction counts the algorithmic pair work of the step: 138 FP64 ops per pair
+    for each of the two Jacobian and three speed passes, plus 252 per pair inside the cutoff
+    for each Jacobian pass."""
+    import ctypes as C
+    simmod = importlib.import_module("2d4_amd.sim")
+    n = 1024
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-0.5, 0.5, n)
+    y = rng.uniform(-0.5, 0.5, n)
+    b = np.where(np.arange(n) % 2 == 0, -1.0, 1.0)
+    params = dict(cutoff_mult=0.05 * np.sqrt(n), heaviside=True, stress=("fixed", 0.1, 0.0))
+    pairs = n * (n - 1) // 2
+    dx = x[:, None] - x[None, :]
+    dy = y[:, None] - y[None, :]
+    dx -= np.rint(dx)
+    dy -= np.rint(dy)
+    inside = int(np.count_nonzero(np.triu(dx * dx + dy * dy < 0.05 ** 2, 1)))
+    ops = pairs * 5 * 138 + inside * 2 * 252
+
+    def timed(make, steps):
+        sim = make()
+        sim.step()  # the first iteration includes run()'s initial speed pass
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.step()
+        dt = (time.perf_counter() - t0) / steps
+        st = sim.stats()
+        sim.close()
+        return dt, st
+
+    out = {"workload": "N=1024 uniform, heaviside cutoff 0.05, tau=0.1, adaptive (the C2 configuration)",
+           "pairs": pairs, "pairs_inside_cutoff": inside, "algorithmic_ops_per_step": ops}
+    for label, solver in (("gpu_passes_host_lu", 0), ("gpu_passes_gpu_lu", 1)):
+        dt, st = timed(lambda: simmod.Simulation.from_arrays(ctx, x, y, b, solver=solver, **params), gpu_steps)
+        out[label] = {"ms_per_step": dt * 1e3, "steps": gpu_steps, "accepted": st["successful_steps"],
+                      "failed": st["failed_steps"], "fraction_of_fp64_peak": ops / dt / 78.6e12}
+    shim = os.path.join(ROOT, "tests", "csrc", "libsim_cpu_fast.so")
+    if with_cpu and os.path.exists(shim):
+        extra = importlib.import_module("2d4_amd._bind_extra")
+        L = C.CDLL(shim)
+        extra.bind(L)
+        dt, st = timed(lambda: simmod.Simulation._create(L, C.c_void_p(1), x, y, b, **params), cpu_steps)
+        out["cpu"] = {"ms_per_step": dt * 1e3, "steps": cpu_steps, "cores": 1, "kind": "port",
+                      "accepted": st["successful_steps"], "failed": st["failed_steps"],
+                      "sample": "the host loop (d4_sim.cpp) with the oracle's single-threaded passes and the host "
+                                "sparse LU, -Ofast -ffast-math -march=x86-64-v3, as the reference runs (one core)"}
+    return out
+
+
 def adjoint_bench(n, T, dt, every):
     """Strain gradient through T backward-Euler steps (2d4_amd/adjoint.py) at
     N = n: times the forward pass, the checkpointed forward and the backward."""
@@ -210,6 +267,7 @@ def main():
     ap.add_argument("--ens-steps", type=int, default=2, help="timed lockstep iterations of the ensemble")
     ap.add_argument("--ens-r", type=int, default=256, help="realizations in the ensemble")
     ap.add_argument("--no-ens", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--no-step", action="store_true", help="skip the N=1024 implicit-step measurement")
     ap.add_argument("--ens-only", action="store_true",
                     help="only the batched ensemble (no plain comparison, no pair-pass headline), for profiling")
     ap.add_argument("--adjoint", type=str, default="",
@@ -321,6 +379,10 @@ def main():
     ensemble = None
     if not args.no_ens:
         ensemble = ensemble_bench(pkg, ctx, args.ens_steps, args.ens_r)
+    step = None
+    if not args.no_step:
+        log("[bench] implicit step, N=1024 (GPU, then the CPU loop)")
+        step = implicit_step_bench(pkg, ctx, with_cpu=not args.no_cpu)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_pair_dense_n16384.json")
@@ -362,6 +424,7 @@ def main():
                      "kernel_avg_ms": k_avg_ms, "ops_per_launch": OPS_JAC_PAIR * unordered},
         "cpu_baseline": cpu,
         "ensemble": ensemble,
+        "implicit_step": step,
         "extra": extra,
     }
     print(json.dumps(line), flush=True)
