@@ -108,7 +108,17 @@ int SparseLU::factor(int n, const int* Ap, const int* Ai, const double* Ax)
     const int64_t nnz = Ap[n];
     if (n <= 96 || nnz > (int64_t)n * n / 4) return factor_dense(n, Ap, Ai, Ax);
 
-    q_ = min_degree_order(n, Ap, Ai);
+    // the column order depends on the pattern alone: reuse it while the pattern
+    // is the previous factorization's (the rescaled stage-II matrix always, and
+    // mostly the next Jacobian too, whose cutoff set rarely changes within a step)
+    const bool same = n == pat_n_ && (int64_t)pat_i_.size() == nnz && std::equal(Ap, Ap + n + 1, pat_p_.begin()) &&
+                      std::equal(Ai, Ai + nnz, pat_i_.begin());
+    if (!same) {
+        q_ = min_degree_order(n, Ap, Ai);
+        pat_n_ = n;
+        pat_p_.assign(Ap, Ap + n + 1);
+        pat_i_.assign(Ai, Ai + nnz);
+    }
     pinv_.assign(n, -1);
     prow_.assign(n, -1);
     Lp_.assign(n + 1, 0);
