@@ -383,51 +383,56 @@ __global__ __launch_bounds__(256) void lu_diag_kernel(const FacDesc* __restrict_
     for (int c = cg; c < kFB; c += 4) blk[(size_t)c * lda + rr] = T[c * kFLd + rr];
 }
 
-__global__ __launch_bounds__(256) void lu_panel_kernel(const FacDesc* __restrict__ fd, const int* __restrict__ pre,
-                                                       int count, double* __restrict__ A, int j0)
+// One 64 x 64 block per wavefront, four blocks of one matrix per workgroup
+// (each matrix's items padded to a multiple of four), the diagonal block
+// staged in LDS once; after that barrier no wavefront waits on another.  In
+// an L21 block lane lo holds row lo of X in registers (X U11 = B, columns left
+// to right), in a U12 block column lo (L11 X = B, rows top to bottom); the
+// factor's entries are LDS broadcasts.  The order of operations of every
+// entry is that of the substitution written out: x_c2 -= x_c u_{c, c2}, c ascending.
+__global__ __launch_bounds__(256, 2) void lu_panel_kernel(const FacDesc* __restrict__ fd, const int* __restrict__ pre,
+                                                          int count, double* __restrict__ A, int j0)
 {
-    __shared__ double Dg[kFB * kFLd], X[kFB * kFLd], Rd[kFB];
-    const int g = xcd_item();
-    const int i = owner(pre, count, g);
+    __shared__ double Dg[kFB * kFLd];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int g0 = xcd_item() * 4;  // the workgroup's first item; its four items share a matrix
+    const int i = owner(pre, count, g0);
     const FacDesc m = fd[i];
     const size_t lda = (size_t)m.npad;
     const int T = m.npad / kFB - j0 / kFB - 1;  // trailing blocks
-    const int item = g - pre[i];
-    const bool rowblk = item < T;                      // L21 block (below) or U12 block (right)
-    const int b = (rowblk ? item : item - T) + 1;      // block index past the diagonal
     const double* dg = A + m.a_off + (size_t)j0 * lda + j0;
-    double* blk = rowblk ? A + m.a_off + (size_t)j0 * lda + (j0 + kFB * b)
-                         : A + m.a_off + (size_t)(j0 + kFB * b) * lda + j0;
     const int t = threadIdx.x, lo = t & 63, q = t >> 6;
-    for (int c = q; c < kFB; c += 4) {
-        Dg[c * kFLd + lo] = dg[(size_t)c * lda + lo];
-        X[c * kFLd + lo] = blk[(size_t)c * lda + lo];
-    }
-    if (rowblk && q == 0) Rd[lo] = 1.0 / dg[(size_t)lo * lda + lo];  // 1 / U11(lo, lo), off the step chain
+    for (int c = q; c < kFB; c += 4) Dg[c * kFLd + lo] = dg[(size_t)c * lda + lo];
     __syncthreads();
+    const int item = g0 - pre[i] + wv;
+    if (item >= 2 * T) return;                     // padding
+    const bool rowblk = item < T;                  // L21 block (below) or U12 block (right)
+    const int b = (rowblk ? item : item - T) + 1;  // block index past the diagonal
+    double x[kFB];
     if (rowblk) {
-        // X U11 = B: row lo of X, columns left to right.  The four threads of a
-        // row all scale column c by 1/U11(c, c); its result is stored a step
-        // later (after the barrier), when no thread reads that column any more.
-        double xprev = 0.0;
+        double* blk = A + m.a_off + (size_t)j0 * lda + (j0 + kFB * b);
+        const double rd = 1.0 / Dg[lo * kFLd + lo];  // lane c: 1 / U11(c, c)
+#pragma unroll
+        for (int c = 0; c < kFB; ++c) x[c] = blk[(size_t)c * lda + lo];
+#pragma unroll
         for (int c = 0; c < kFB; ++c) {
-            const double xc = X[c * kFLd + lo] * Rd[c];
-            if (q == 0 && c > 0) X[(c - 1) * kFLd + lo] = xprev;
-            for (int c2 = c + 1 + q; c2 < kFB; c2 += 4) X[c2 * kFLd + lo] = fma(-xc, Dg[c2 * kFLd + c], X[c2 * kFLd + lo]);
-            xprev = xc;
-            __syncthreads();
+            x[c] *= __shfl(rd, c);
+#pragma unroll
+            for (int c2 = c + 1; c2 < kFB; ++c2) x[c2] = fma(-x[c], Dg[c2 * kFLd + c], x[c2]);
         }
-        if (q == 0) X[(kFB - 1) * kFLd + lo] = xprev;
-        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < kFB; ++c) blk[(size_t)c * lda + lo] = x[c];
     } else {
-        // L11 X = B (unit lower): column lo of X, rows top to bottom
-        for (int r = 0; r < kFB - 1; ++r) {
-            const double xr = X[lo * kFLd + r];
-            for (int r2 = r + 1 + q; r2 < kFB; r2 += 4) X[lo * kFLd + r2] = fma(-Dg[r * kFLd + r2], xr, X[lo * kFLd + r2]);
-            __syncthreads();
-        }
+        double* col = A + m.a_off + (size_t)(j0 + kFB * b + lo) * lda + j0;
+#pragma unroll
+        for (int r = 0; r < kFB; ++r) x[r] = col[r];
+#pragma unroll
+        for (int r = 0; r < kFB - 1; ++r)
+#pragma unroll
+            for (int r2 = r + 1; r2 < kFB; ++r2) x[r2] = fma(-Dg[r * kFLd + r2], x[r], x[r2]);
+#pragma unroll
+        for (int r = 0; r < kFB; ++r) col[r] = x[r];
     }
-    for (int c = q; c < kFB; c += 4) blk[(size_t)c * lda + lo] = X[c * kFLd + lo];
 }
 
 // C -= L21 U12 for one kFB x kFB tile, computed transposed on the matrix cores:
@@ -827,7 +832,7 @@ private:
             sp.act = act;
             sp.pre2 = (int)pre.size();
             pre.push_back(0);
-            for (int i = 0; i < act; ++i) pre.push_back(pre.back() + 2 * (fd[i].npad / kFB - p - 1));
+            for (int i = 0; i < act; ++i) pre.push_back(pre.back() + (2 * (fd[i].npad / kFB - p - 1) + 3) / 4 * 4);
             sp.n2 = pre.back();
             sp.pre3 = (int)pre.size();
             pre.push_back(0);
@@ -847,8 +852,8 @@ private:
             const int j0 = kFB * p;
             hipLaunchKernelGGL(lu_diag_kernel, dim3(sp.act), dim3(256), 0, st, fdesc_.p, A_.p, j0);
             if (sp.n2 > 0)
-                hipLaunchKernelGGL(lu_panel_kernel, dim3(sp.n2), dim3(256), 0, st, fdesc_.p, fpre_.p + sp.pre2, sp.act,
-                                   A_.p, j0);
+                hipLaunchKernelGGL(lu_panel_kernel, dim3(sp.n2 / 4), dim3(256), 0, st, fdesc_.p,
+                                   fpre_.p + sp.pre2, sp.act, A_.p, j0);
             if (sp.n3 > 0)
                 hipLaunchKernelGGL(lu_update_kernel, dim3(sp.n3), dim3(256), 0, st, fdesc_.p, fpre_.p + sp.pre3,
                                    sp.act, A_.p, j0);

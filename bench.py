@@ -77,7 +77,7 @@ def cpu_baseline(n_single, n_omp, min_seconds):
     return out
 
 
-def ensemble_bench(pkg, ctx, steps, R, seed=4242, plain=True):
+def ensemble_bench(pkg, ctx, steps, R, seed=4242, plain=True, solver=1):
     """Batched ensemble vs the plain one-realization-at-a-time GPU path on the
     same realizations, same number of lockstep iterations, same solver.
 
@@ -86,7 +86,8 @@ def ensemble_bench(pkg, ctx, steps, R, seed=4242, plain=True):
     tau = 0.05 (-r 0 -s 0.05), default precision control, linear solves as a
     dense LU on the GPU.  A step is one loop iteration (stages I-IV) of one
     realization; the first iteration (with run()'s initial speed pass) is
-    untimed in both paths.  plain=False measures the batched path alone."""
+    untimed in both paths.  plain=False measures the batched path alone; solver=0
+    puts every realization's linear solves on the host sparse LU instead."""
     ens = importlib.import_module("2d4_amd.ensemble")
     simmod = importlib.import_module("2d4_amd.sim")
     rng = np.random.default_rng(seed)
@@ -96,7 +97,7 @@ def ensemble_bench(pkg, ctx, steps, R, seed=4242, plain=True):
     x = rng.uniform(-0.5, 0.5, T)
     y = rng.uniform(-0.5, 0.5, T)
     b = np.concatenate([np.where(np.arange(n) % 2 == 0, -1.0, 1.0) for n in sizes])
-    params = [dict(cutoff_mult=0.05 * float(np.sqrt(n)), heaviside=True, stress=("fixed", 0.05, 0.0), solver=1)
+    params = [dict(cutoff_mult=0.05 * float(np.sqrt(n)), heaviside=True, stress=("fixed", 0.05, 0.0), solver=solver)
               for n in sizes]
     log(f"[bench] ensemble R={R} sum N={T}: batched")
     e = ens.Ensemble(ctx, off, x, y, b, params)
@@ -266,6 +267,8 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the speed-only and scaled-matrix side measurements")
     ap.add_argument("--ens-steps", type=int, default=2, help="timed lockstep iterations of the ensemble")
     ap.add_argument("--ens-r", type=int, default=256, help="realizations in the ensemble")
+    ap.add_argument("--ens-solver", type=int, default=1, choices=(0, 1),
+                    help="with --ens-only: 1 = GPU dense LU (default), 0 = host sparse LU for every realization")
     ap.add_argument("--no-ens", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-step", action="store_true", help="skip the N=1024 implicit-step measurement")
     ap.add_argument("--ens-only", action="store_true",
@@ -298,7 +301,7 @@ def main():
     dev = torch.device("cuda:0")
     if args.ens_only:
         ectx = pkg.Context(0)
-        print(json.dumps({"ensemble": ensemble_bench(pkg, ectx, args.ens_steps, args.ens_r, plain=False)}), flush=True)
+        print(json.dumps({"ensemble": ensemble_bench(pkg, ectx, args.ens_steps, args.ens_r, plain=False, solver=args.ens_solver)}), flush=True)
         return
     n = args.n
     ctx = pkg.Context(0)
