@@ -9,7 +9,7 @@
 // solver the batched ensemble and large single runs use.
 //
 // Layout: matrix k is column-major of order npad_k = gpu_lu_padded(n_k) (n_k
-// rounded up to 256), the CSC entries scattered into its leading n_k x n_k
+// rounded up to 64, the factorization's block), the CSC entries scattered into its leading n_k x n_k
 // block and the identity beyond it.  The padding changes nothing in the
 // solution: its rows and columns are decoupled from the leading block and
 // never enter a pivot choice.  It gives every matrix an order that is a
@@ -56,7 +56,7 @@
 
 namespace d4host {
 
-int gpu_lu_padded(int n) { return n <= 0 ? 0 : (n + 255) / 256 * 256; }
+int gpu_lu_padded(int n) { return n <= 0 ? 0 : (n + 63) / 64 * 64; }
 
 namespace {
 

@@ -84,7 +84,7 @@ std::vector<int> min_degree_order(int n, const int* Ap, const int* Ai);
 LinearSolver* new_gpu_dense_lu(int device);
 
 // order of the padded device matrix for an n x n system: n rounded up to a
-// multiple of 256, so that matrices of similar order share a batched call
+// multiple of 64, the factorization's block
 int gpu_lu_padded(int n);
 
 // The dense LU of new_gpu_dense_lu for K matrices at once: the unpivoted

@@ -133,7 +133,7 @@ def test_ensemble_gpu_dense_solver_equals_plain_path(ctx):
 
 
 def test_ensemble_gpu_dense_solver_padded_orders_equal_plain_path(ctx):
-    # realizations over several padded orders (256 .. 1536) so that batched LU
+    # realizations over several padded orders (320 .. 1344) so that batched LU
     # calls hold different matrices than the plain path's one-matrix calls
     sizes = [700, 1300, 260, 520, 1030, 264, 1290]
     off, x, y, b = _ens_inputs(sizes, 21)
