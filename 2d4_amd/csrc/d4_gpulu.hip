@@ -132,11 +132,16 @@ struct PinBuf {
     }
 };
 
+// Symmetric order and block envelope of a matrix (aux, per matrix): at
+// ord_off, ord[new] = old then inv[old] = new (n entries each); at env_off,
+// per block b of nb = npad / kFB: last[b], then top[b], then first[b] (see
+// block_envelope).  -1: the natural order / the whole matrix.
 struct MatDesc {
     long long a_off;  // first entry of the matrix in the dense store
     int n, npad;
     int col_off;   // its n + 1 column pointers in the packed CSC
     int ipiv_off;  // its npad pivots (matrices refactored with pivoting)
+    int ord_off, env_off;
 };
 
 struct SolveDesc {
@@ -144,32 +149,46 @@ struct SolveDesc {
     int n, npad;
     int b_off;     // its right-hand side in the packed vector (n entries)
     int perm_off;  // its row permutation, or -1 when rows stayed in place
+    int ord_off, env_off;
 };
 
 constexpr double kMaxMultiplier = 1000.0;  // threshold partial pivoting with u = 0.001
-constexpr int kSB = 64;                  // block of the triangular solves
+constexpr int kFB = 64;                    // block order of the factorization and of the envelope
+constexpr int kSB = kFB;                   // block of the triangular solves
 constexpr int kSolveThreads = 1024;
 
-// identity beyond n, zero in the leading block: column blockIdx.x of matrix blockIdx.y
-__global__ void fill_identity_kernel(const MatDesc* __restrict__ desc, double* __restrict__ A)
+// identity beyond n, zero in the leading block: column blockIdx.x of matrix
+// blockIdx.y.  With an envelope, only the rows the factorization, the check
+// and the solves touch in this column: blocks top .. last of its block.
+__global__ void fill_identity_kernel(const MatDesc* __restrict__ desc, double* __restrict__ A,
+                                     const int* __restrict__ aux)
 {
     const MatDesc m = desc[blockIdx.y];
     const int c = blockIdx.x;
     if (c >= m.npad) return;
+    int r0 = 0, r1 = m.npad;
+    if (m.env_off >= 0) {
+        const int nb = m.npad / kFB, b = c / kFB;
+        r0 = aux[m.env_off + nb + b] * kFB;
+        r1 = min(m.npad, (aux[m.env_off + b] + 1) * kFB);
+    }
     double* col = A + m.a_off + (size_t)c * m.npad;
     const double diag = c >= m.n ? 1.0 : 0.0;
-    for (int r = threadIdx.x; r < m.npad; r += blockDim.x) col[r] = r == c ? diag : 0.0;
+    for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) col[r] = r == c ? diag : 0.0;
 }
 
-// the CSC entries of column c (one thread) into the dense matrix
+// the CSC entries of column c (one thread) into the dense matrix, rows and
+// columns renumbered by inv when the matrix is reordered
 __global__ void scatter_kernel(const MatDesc* __restrict__ desc, const int* __restrict__ colp,
-                               const int* __restrict__ rowi, const double* __restrict__ vals, double* __restrict__ A)
+                               const int* __restrict__ rowi, const double* __restrict__ vals, double* __restrict__ A,
+                               const int* __restrict__ aux)
 {
     const MatDesc m = desc[blockIdx.y];
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.n) return;
-    double* col = A + m.a_off + (size_t)c * m.npad;
-    for (int q = colp[m.col_off + c]; q < colp[m.col_off + c + 1]; ++q) col[rowi[q]] = vals[q];
+    const int* inv = m.ord_off >= 0 ? aux + m.ord_off + m.n : nullptr;
+    double* col = A + m.a_off + (size_t)(inv ? inv[c] : c) * m.npad;
+    for (int q = colp[m.col_off + c]; q < colp[m.col_off + c + 1]; ++q) col[inv ? inv[rowi[q]] : rowi[q]] = vals[q];
 }
 
 // one dense matrix already on the device into the padded store: element (r, c)
@@ -189,8 +208,9 @@ __global__ void dense_copy_kernel(const MatDesc* __restrict__ desc, double* __re
 // and nonzero and every multiplier |l_rc| <= kMaxMultiplier, i.e. unless
 // threshold partial pivoting with u = 1/kMaxMultiplier would keep the diagonal
 // at every stage
+// (with an envelope the multipliers below its last block are zero, unread)
 __global__ void multiplier_check_kernel(const MatDesc* __restrict__ desc, const double* __restrict__ A,
-                                        int* __restrict__ flag)
+                                        int* __restrict__ flag, const int* __restrict__ aux)
 {
     const MatDesc m = desc[blockIdx.y];
     const int c = blockIdx.x;
@@ -201,7 +221,8 @@ __global__ void multiplier_check_kernel(const MatDesc* __restrict__ desc, const 
         const double p = std::fabs(col[c]);
         bad = !(p > 0.0) || !(p <= 1.7976931348623157e308);
     }
-    for (int r = c + 1 + threadIdx.x; r < m.n; r += blockDim.x) bad |= !(std::fabs(col[r]) <= kMaxMultiplier);
+    const int r1 = m.env_off >= 0 ? min(m.n, (aux[m.env_off + c / kFB] + 1) * kFB) : m.n;
+    for (int r = c + 1 + threadIdx.x; r < r1; r += blockDim.x) bad |= !(std::fabs(col[r]) <= kMaxMultiplier);
     if (bad) atomicOr(flag + blockIdx.y, 1);
 }
 
@@ -251,12 +272,16 @@ __device__ inline double lane_value(double v, int lane)  // v of `lane` (uniform
 // thread updates rows outside the block with the block's columns
 // (x_r -= sum_c a_rc x_c, accumulated in ascending c).  The order of
 // operations is fixed by the matrix alone.  x lives in LDS (kLds) or in
-// global scratch; the arithmetic is the same either way.
+// global scratch; the arithmetic is the same either way.  A reordered matrix
+// gathers b through ord and scatters x back through it; with an envelope the
+// updates stop at its bounds, past which the factors' columns are zero (a
+// zero sum leaves x_r as it is, so the bits are those of the full sweep).
 template <bool kLds>
 __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc* __restrict__ desc,
                                                                   const double* __restrict__ A,
                                                                   const int* __restrict__ perm, double* __restrict__ b,
-                                                                  double* __restrict__ scratch)
+                                                                  double* __restrict__ scratch,
+                                                                  const int* __restrict__ aux)
 {
     extern __shared__ double lds[];
     const SolveDesc m = desc[blockIdx.x];
@@ -266,9 +291,13 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
     double* bm = b + m.b_off;
     double* D = lds;  // kSB x kSB diagonal block, column-major
     double* x = kLds ? lds + kSB * kSB : scratch + m.b_off;
+    const int* ord = m.ord_off >= 0 ? aux + m.ord_off : nullptr;
+    const int* env = m.env_off >= 0 ? aux + m.env_off : nullptr;
+    const int nb = m.npad / kSB;
     for (int i = tid; i < n; i += kSolveThreads) {
-        const int p = m.perm_off >= 0 ? perm[m.perm_off + i] : i;  // padded rows never move into [0, n)
-        x[i] = bm[(p >= 0 && p < n) ? p : i];
+        int p = m.perm_off >= 0 ? perm[m.perm_off + i] : i;  // padded rows never move into [0, n)
+        p = (p >= 0 && p < n) ? p : i;
+        x[i] = bm[ord ? ord[p] : p];
     }
     __syncthreads();
 
@@ -303,7 +332,8 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
                 if (l < B) x[jb + l] = v;
             }
             __syncthreads();
-            const int r0 = pass == 0 ? jb + B : 0, r1 = pass == 0 ? n : jb;
+            const int r0 = pass == 0 ? jb + B : (env ? env[2 * nb + bi] * kSB : 0);
+            const int r1 = pass == 0 ? (env ? min(n, (env[bi] + 1) * kSB) : n) : jb;
             for (int r = r0 + tid; r < r1; r += kSolveThreads) {
                 const double* a = Am + (size_t)jb * lda + r;
                 double acc = 0.0;
@@ -314,7 +344,7 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
             __syncthreads();
         }
     }
-    for (int i = tid; i < n; i += kSolveThreads) bm[i] = x[i];
+    for (int i = tid; i < n; i += kSolveThreads) bm[ord ? ord[i] : i] = x[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -322,12 +352,16 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
 // matrix of order npad, a multiple of kFB).  Step p (j0 = kFB p) covers the
 // matrices with npad > j0, a prefix of the batch sorted by npad, largest first:
 //   lu_diag_kernel    factors the diagonal block A11 = L11 U11 (one workgroup per matrix, LDS);
-//   lu_panel_kernel   L21 = A21 U11^-1 and U12 = L11^-1 A12, one workgroup per kFB x kFB block;
+//   lu_panel_kernel   L21 = A21 U11^-1 and U12 = L11^-1 A12, one wavefront per kFB x kFB block;
 //   lu_update_kernel  A22 -= L21 U12, one workgroup per kFB x kFB tile, on the matrix cores.
+// Per step and matrix, tb[i] = T: the blocks p + 1 .. p + T below and right
+// of the diagonal block are the ones that can be nonzero (all trailing blocks,
+// or those within the matrix's envelope); the panels and the update cover
+// those.  A block of L21 or U12 past the envelope is zero, and an update by
+// zero leaves an entry as it is, so skipping them changes no bits.
 // The order of operations for an entry depends on its matrix alone (the
 // workgroups of other matrices only add parallelism), so a matrix gets the
 // same factors in any batch.
-constexpr int kFB = 64;       // block order
 constexpr int kFLd = kFB + 1;  // LDS column stride (odd: conflict-free column and row walks)
 
 struct FacDesc {
@@ -391,7 +425,8 @@ __global__ __launch_bounds__(256) void lu_diag_kernel(const FacDesc* __restrict_
 // factor's entries are LDS broadcasts.  The order of operations of every
 // entry is that of the substitution written out: x_c2 -= x_c u_{c, c2}, c ascending.
 __global__ __launch_bounds__(256, 2) void lu_panel_kernel(const FacDesc* __restrict__ fd, const int* __restrict__ pre,
-                                                          int count, double* __restrict__ A, int j0)
+                                                          const int* __restrict__ tb, int count,
+                                                          double* __restrict__ A, int j0)
 {
     __shared__ double Dg[kFB * kFLd];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -399,7 +434,7 @@ __global__ __launch_bounds__(256, 2) void lu_panel_kernel(const FacDesc* __restr
     const int i = owner(pre, count, g0);
     const FacDesc m = fd[i];
     const size_t lda = (size_t)m.npad;
-    const int T = m.npad / kFB - j0 / kFB - 1;  // trailing blocks
+    const int T = tb[i];  // blocks below and right of the diagonal block that can be nonzero
     const double* dg = A + m.a_off + (size_t)j0 * lda + j0;
     const int t = threadIdx.x, lo = t & 63, q = t >> 6;
     for (int c = q; c < kFB; c += 4) Dg[c * kFLd + lo] = dg[(size_t)c * lda + lo];
@@ -441,14 +476,15 @@ __global__ __launch_bounds__(256, 2) void lu_panel_kernel(const FacDesc* __restr
 // L21 operand are read down columns (coalesced) and only U12 goes through LDS.
 // Wave w owns the 32 x 32 quadrant rows 32 (w >> 1), columns 32 (w & 1) of C.
 __global__ __launch_bounds__(256) void lu_update_kernel(const FacDesc* __restrict__ fd, const int* __restrict__ pre,
-                                                        int count, double* __restrict__ A, int j0)
+                                                        const int* __restrict__ tb, int count,
+                                                        double* __restrict__ A, int j0)
 {
     __shared__ double U[kFB * kFLd];  // U12 tile, U[k * kFLd + c]
     const int g = xcd_item();
     const int i = owner(pre, count, g);
     const FacDesc m = fd[i];
     const size_t lda = (size_t)m.npad;
-    const int T = m.npad / kFB - j0 / kFB - 1;
+    const int T = tb[i];
     const int item = g - pre[i];
     const int bi = item / T + 1, bj = item % T + 1;
     const size_t r0 = (size_t)j0 + kFB * bi, c0 = (size_t)j0 + kFB * bj;
@@ -595,6 +631,7 @@ public:
             d.npad = npad;
             d.col_off = (int)col_total;
             d.ipiv_off = -1;
+            d.ord_off = d.env_off = -1;
             a_total += (size_t)npad * npad;
             col_total += n + 1;
             piv_total += npad;
@@ -604,6 +641,44 @@ public:
         }
         if (nz_off[K] > INT_MAX || col_total > INT_MAX || piv_total > INT_MAX / 2) return -1;
         const long long nnz = nz_off[K];
+
+        // each matrix's symmetric order and block envelope, from its pattern
+        // (a dense device matrix keeps its order and is factored whole)
+        plan_.resize(K);
+        long long aux_total = 0;
+        if (!dense) {
+            par(K, [&](int k) {
+                Plan& pl = plan_[k];
+                pl.re = m[k].n > 0 &&
+                        band_order(m[k].n, desc_h_[k].npad, kFB, m[k].Ap, m[k].Ai, pl.ord, pl.inv, pl.env);
+            });
+            for (int k = 0; k < K; ++k) {
+                MatDesc& d = desc_h_[k];
+                if (d.n == 0) continue;
+                if (plan_[k].re) {
+                    d.ord_off = (int)aux_total;
+                    aux_total += 2LL * d.n;
+                }
+                d.env_off = (int)aux_total;
+                aux_total += 3LL * (d.npad / kFB);
+            }
+            if (aux_total > INT_MAX) return -1;
+            if (!aux_.ensure((size_t)aux_total) || !aux_h_.ensure((size_t)aux_total)) return -2;
+            par(K, [&](int k) {
+                const MatDesc& d = desc_h_[k];
+                const Plan& pl = plan_[k];
+                if (d.ord_off >= 0) {
+                    std::copy(pl.ord.begin(), pl.ord.end(), aux_h_.p + d.ord_off);
+                    std::copy(pl.inv.begin(), pl.inv.end(), aux_h_.p + d.ord_off + d.n);
+                }
+                if (d.env_off >= 0) {
+                    const int nb = d.npad / kFB;
+                    std::copy(pl.env.last.begin(), pl.env.last.end(), aux_h_.p + d.env_off);
+                    std::copy(pl.env.top.begin(), pl.env.top.end(), aux_h_.p + d.env_off + nb);
+                    std::copy(pl.env.first.begin(), pl.env.first.end(), aux_h_.p + d.env_off + 2 * nb);
+                }
+            });
+        }
 
         // every matrix first without pivoting (largest first: that is also the
         // solve's launch order)
@@ -635,6 +710,8 @@ public:
         });
         for (int i = 0; i < K; ++i) ptr_h_.p[i] = A_.p + desc_h_[order[i]].a_off;
         if (hipMemcpyAsync(desc_.p, desc_h_.data(), sizeof(MatDesc) * K, hipMemcpyHostToDevice, st) != hipSuccess ||
+            (aux_total > 0 &&
+             hipMemcpyAsync(aux_.p, aux_h_.p, sizeof(int) * aux_total, hipMemcpyHostToDevice, st) != hipSuccess) ||
             (!dense &&
              hipMemcpyAsync(colp_.p, colp_h_.p, sizeof(int) * col_total, hipMemcpyHostToDevice, st) != hipSuccess) ||
             (nnz > 0 &&
@@ -649,7 +726,7 @@ public:
             const int kc = std::min(kGridY, K - k0);
             if (npad_max > 0)
                 hipLaunchKernelGGL(multiplier_check_kernel, dim3(npad_max, kc), dim3(256), 0, st, desc_.p + k0, A_.p,
-                                   flag + k0);
+                                   flag + k0, aux_.p);
         }
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(info_h_.p, info, sizeof(int) * 2 * K, hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -674,6 +751,7 @@ public:
                 desc_h_[k].ipiv_off = (int)poff;
                 poff += desc_h_[k].npad;
                 sub[i] = desc_h_[k];
+                sub[i].env_off = -1;  // assembled and factored whole (in the same order)
                 redo_pos[k] = i;
                 ptr_h_.p[K + i] = A_.p + desc_h_[k].a_off;
                 npad2 = std::max(npad2, desc_h_[k].npad);
@@ -714,8 +792,9 @@ public:
         for (int i = 0; i < K; ++i) {
             const int k = order[i];
             if (desc_h_[k].n == 0) continue;
+            const bool pivoted = redo_pos[k] >= 0;
             sd.push_back({desc_h_[k].a_off, desc_h_[k].n, desc_h_[k].npad, (int)b_off_[k],
-                          redo_pos[k] >= 0 ? desc_h_[k].ipiv_off : -1});
+                          pivoted ? desc_h_[k].ipiv_off : -1, desc_h_[k].ord_off, pivoted ? -1 : desc_h_[k].env_off});
         }
         nsolve_ = (int)sd.size();
         npad_max_ = npad_max;
@@ -780,6 +859,14 @@ private:
     };
     DenseSrc dense_;  // factor_dense_dev's source while it runs (assemble() copies from it)
     DevBuf<MatDesc> desc_;
+    struct Plan {
+        bool re = false;  // reordered: ord (ord[new] = old) and inv hold the order
+        std::vector<int> ord, inv;
+        BlockEnvelope env;
+    };
+    std::vector<Plan> plan_;  // per matrix of the last factor() (CSC input)
+    DevBuf<int> aux_;         // orders and envelopes, at MatDesc::ord_off / env_off
+    PinBuf<int> aux_h_;
     DevBuf<FacDesc> fdesc_;
     DevBuf<int> fpre_;
     DevBuf<SolveDesc> sdesc_;
@@ -815,31 +902,40 @@ private:
     int factor_unpivoted(const std::vector<int>& order, hipStream_t st)
     {
         std::vector<FacDesc> fd;
+        std::vector<const int*> last;  // each matrix's envelope (last block per step), or null
         for (int k : order)
-            if (desc_h_[k].npad > 0) fd.push_back({desc_h_[k].a_off, desc_h_[k].npad, 0});
+            if (desc_h_[k].npad > 0) {
+                fd.push_back({desc_h_[k].a_off, desc_h_[k].npad, 0});
+                last.push_back(desc_h_[k].env_off >= 0 ? plan_[k].env.last.data() : nullptr);
+            }
         const int Ka = (int)fd.size();
         if (Ka == 0) return 0;
         const int P = fd[0].npad / kFB;
         struct Step {
-            int act, pre2, n2, pre3, n3;
+            int act, tbo, pre2, n2, pre3, n3;
         };
         std::vector<Step> steps(P);
-        std::vector<int> pre;
+        std::vector<int> pre, T(Ka);
         for (int p = 0; p < P; ++p) {
             int act = 0;
             while (act < Ka && fd[act].npad > kFB * p) ++act;
             Step& sp = steps[p];
             sp.act = act;
+            sp.tbo = (int)pre.size();
+            for (int i = 0; i < act; ++i) {
+                T[i] = last[i] ? std::max(0, last[i][p] - p) : fd[i].npad / kFB - p - 1;
+                pre.push_back(T[i]);
+            }
             sp.pre2 = (int)pre.size();
             pre.push_back(0);
-            for (int i = 0; i < act; ++i) pre.push_back(pre.back() + (2 * (fd[i].npad / kFB - p - 1) + 3) / 4 * 4);
+            for (int i = 0; i < act; ++i) pre.push_back(pre.back() + (2 * T[i] + 3) / 4 * 4);
             sp.n2 = pre.back();
             sp.pre3 = (int)pre.size();
             pre.push_back(0);
             for (int i = 0; i < act; ++i) {
-                const long long T = fd[i].npad / kFB - p - 1;
-                if (pre.back() + T * T > INT_MAX) return -1;
-                pre.push_back(pre.back() + (int)(T * T));
+                const long long t2 = (long long)T[i] * T[i];
+                if (pre.back() + t2 > INT_MAX) return -1;
+                pre.push_back(pre.back() + (int)t2);
             }
             sp.n3 = pre.back();
         }
@@ -853,10 +949,10 @@ private:
             hipLaunchKernelGGL(lu_diag_kernel, dim3(sp.act), dim3(256), 0, st, fdesc_.p, A_.p, j0);
             if (sp.n2 > 0)
                 hipLaunchKernelGGL(lu_panel_kernel, dim3(sp.n2 / 4), dim3(256), 0, st, fdesc_.p,
-                                   fpre_.p + sp.pre2, sp.act, A_.p, j0);
+                                   fpre_.p + sp.pre2, fpre_.p + sp.tbo, sp.act, A_.p, j0);
             if (sp.n3 > 0)
                 hipLaunchKernelGGL(lu_update_kernel, dim3(sp.n3), dim3(256), 0, st, fdesc_.p, fpre_.p + sp.pre3,
-                                   sp.act, A_.p, j0);
+                                   fpre_.p + sp.tbo, sp.act, A_.p, j0);
         }
         return hipGetLastError() != hipSuccess ? -1 : 0;
     }
@@ -925,10 +1021,10 @@ private:
         if (npad_max_ <= lds_order_)
             hipLaunchKernelGGL(lu_solve_kernel<true>, dim3(nsolve_), dim3(kSolveThreads),
                                sizeof(double) * (kSB * kSB + npad_max_), st, sdesc_.p, A_.p, ipiv_.p, b_.p,
-                               b_.p + total);
+                               b_.p + total, aux_.p);
         else
             hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(nsolve_), dim3(kSolveThreads), sizeof(double) * kSB * kSB,
-                               st, sdesc_.p, A_.p, ipiv_.p, b_.p, b_.p + total);
+                               st, sdesc_.p, A_.p, ipiv_.p, b_.p, b_.p + total, aux_.p);
         return hipGetLastError() != hipSuccess ? -1 : 0;
     }
 
@@ -942,9 +1038,9 @@ private:
         }
         for (int k0 = 0; k0 < K; k0 += kGridY) {
             const int kc = std::min(kGridY, K - k0);
-            hipLaunchKernelGGL(fill_identity_kernel, dim3(npad_max, kc), dim3(256), 0, st, desc + k0, A_.p);
+            hipLaunchKernelGGL(fill_identity_kernel, dim3(npad_max, kc), dim3(256), 0, st, desc + k0, A_.p, aux_.p);
             hipLaunchKernelGGL(scatter_kernel, dim3((npad_max + 255) / 256, kc), dim3(256), 0, st, desc + k0, colp_.p,
-                               rowi_.p, vals_.p, A_.p);
+                               rowi_.p, vals_.p, A_.p, aux_.p);
         }
         return hipGetLastError() != hipSuccess ? -1 : 0;
     }

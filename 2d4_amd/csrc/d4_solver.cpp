@@ -60,6 +60,129 @@ std::vector<int> min_degree_order(int n, const int* Ap, const int* Ai)
     return order;
 }
 
+namespace {
+
+// breadth-first levels of s's component in the graph whose neighbours of v
+// are the rows of column v: the number of levels, and the nodes of the last
+// one; stamp[v] == tag marks the nodes reached
+int bfs_depth(int s, const int* Ap, const int* Ai, std::vector<int>& stamp, int tag, std::vector<int>& queue,
+              std::vector<int>& last_level)
+{
+    queue.clear();
+    queue.push_back(s);
+    stamp[s] = tag;
+    size_t begin = 0;
+    int depth = 0;
+    while (begin < queue.size()) {
+        const size_t end = queue.size();
+        ++depth;
+        for (size_t h = begin; h < end; ++h) {
+            const int v = queue[h];
+            for (int k = Ap[v]; k < Ap[v + 1]; ++k)
+                if (stamp[Ai[k]] != tag) {
+                    stamp[Ai[k]] = tag;
+                    queue.push_back(Ai[k]);
+                }
+        }
+        if (queue.size() == end) last_level.assign(queue.begin() + begin, queue.begin() + end);
+        begin = end;
+    }
+    return depth;
+}
+
+}  // namespace
+
+std::vector<int> rcm_order(int n, const int* Ap, const int* Ai)
+{
+    auto deg = [&](int v) { return Ap[v + 1] - Ap[v]; };
+    std::vector<int> order, queue, last, last2, stamp(n, -1);
+    order.reserve(n);
+    std::vector<char> seen(n, 0);
+    int tag = 0;
+    for (int s0 = 0; s0 < n; ++s0) {
+        if (seen[s0]) continue;
+        // pseudo-peripheral start: move to a lowest-degree node of the last
+        // level while that lengthens the level structure
+        int v = s0;
+        int depth = bfs_depth(v, Ap, Ai, stamp, tag++, queue, last);
+        for (int it = 0; it < 16; ++it) {
+            int u = last[0];
+            for (int w : last)
+                if (deg(w) < deg(u) || (deg(w) == deg(u) && w < u)) u = w;
+            const int d2 = bfs_depth(u, Ap, Ai, stamp, tag++, queue, last2);
+            if (d2 <= depth) break;
+            v = u;
+            depth = d2;
+            last.swap(last2);
+        }
+        // Cuthill-McKee search from v, neighbours in row order
+        size_t head = order.size();
+        order.push_back(v);
+        seen[v] = 1;
+        while (head < order.size()) {
+            const int x = order[head++];
+            for (int k = Ap[x]; k < Ap[x + 1]; ++k)
+                if (!seen[Ai[k]]) {
+                    seen[Ai[k]] = 1;
+                    order.push_back(Ai[k]);
+                }
+        }
+    }
+    std::reverse(order.begin(), order.end());
+    return order;
+}
+
+BlockEnvelope block_envelope(int n, int npad, int bs, const int* Ap, const int* Ai, const int* inv)
+{
+    BlockEnvelope e;
+    const int nb = npad / bs;
+    std::vector<int> f(npad);
+    for (int i = 0; i < npad; ++i) f[i] = i;
+    for (int c = 0; c < n; ++c) {
+        const int nc = inv ? inv[c] : c;
+        for (int k = Ap[c]; k < Ap[c + 1]; ++k) {
+            const int nr = inv ? inv[Ai[k]] : Ai[k];
+            const int lo = std::min(nr, nc), hi = std::max(nr, nc);
+            f[hi] = std::min(f[hi], lo);
+        }
+    }
+    e.first.assign(nb, 0);
+    e.last.assign(nb, 0);
+    e.top.assign(nb, 0);
+    for (int b = 0; b < nb; ++b) {
+        int lo = f[b * bs];
+        for (int i = b * bs + 1; i < (b + 1) * bs; ++i) lo = std::min(lo, f[i]);
+        e.first[b] = lo / bs;
+        e.last[b] = b;
+    }
+    for (int b = 0; b < nb; ++b) e.last[e.first[b]] = std::max(e.last[e.first[b]], b);
+    for (int b = 1; b < nb; ++b) e.last[b] = std::max(e.last[b], e.last[b - 1]);
+    for (int b = 0, p = 0; b < nb; ++b) {
+        while (e.last[p] < b) ++p;
+        e.top[b] = p;
+    }
+    for (int p = 0; p < nb; ++p) e.tiles += (long long)(e.last[p] - p) * (e.last[p] - p);
+    return e;
+}
+
+bool band_order(int n, int npad, int bs, const int* Ap, const int* Ai, std::vector<int>& ord, std::vector<int>& inv,
+                BlockEnvelope& env)
+{
+    env = block_envelope(n, npad, bs, Ap, Ai, nullptr);
+    ord.clear();
+    inv.clear();
+    if (n < 2 * bs || env.tiles == 0) return false;
+    std::vector<int> o = rcm_order(n, Ap, Ai);
+    std::vector<int> iv(n);
+    for (int i = 0; i < n; ++i) iv[o[i]] = i;
+    BlockEnvelope e = block_envelope(n, npad, bs, Ap, Ai, iv.data());
+    if (2 * e.tiles > env.tiles) return false;
+    ord.swap(o);
+    inv.swap(iv);
+    env = std::move(e);
+    return true;
+}
+
 int SparseLU::factor_dense(int n, const int* Ap, const int* Ai, const double* Ax)
 {
     dense_ = true;

@@ -73,6 +73,43 @@ private:
 // minimum degree order of the symmetric pattern of A + A^T (exposed for tests)
 std::vector<int> min_degree_order(int n, const int* Ap, const int* Ai);
 
+// Reverse Cuthill-McKee order (ord[new] = old) over the graph whose
+// neighbours of node c are the rows of column c: per connected component
+// (taken by lowest index) a breadth-first search from a pseudo-peripheral
+// node (George and Liu's search), neighbours in row order; the whole sequence
+// reversed.  For a symmetric pattern (the step matrices') that is the graph
+// of A; for any other pattern the order is still a permutation, and
+// block_envelope measures the pattern of A + A^T under it.  Deterministic:
+// the order depends on the pattern alone.
+std::vector<int> rcm_order(int n, const int* Ap, const int* Ai);
+
+// Block envelope of the pattern of A + A^T with rows and columns renumbered
+// by inv (inv[old] = new; null: as they are), in blocks of bs, for a matrix
+// padded with the identity to npad (a multiple of bs).  With f(i) the lowest
+// column of row i's entries (f(i) <= i), Gaussian elimination without
+// interchanges keeps L(i, j) = 0 for j < f(i) and, the pattern being
+// symmetric, U(j, i) = 0 for j < f(i).  Per block b of nb = npad / bs:
+//   first[b]: min over the rows i of block b of f(i), in blocks;
+//   last[b]:  the highest block b' with first[b'] <= b: at factorization step
+//             b the blocks of L below and of U right of the diagonal block
+//             are zero beyond block last[b] (nondecreasing, last[b] >= b);
+//   top[b]:   the lowest step p with last[p] >= b (top[b] <= first[b]).
+// tiles = sum_p (last[p] - p)^2, the trailing-update tiles of a blocked LU
+// that skips the zero blocks.
+struct BlockEnvelope {
+    std::vector<int> first, last, top;
+    long long tiles = 0;
+};
+BlockEnvelope block_envelope(int n, int npad, int bs, const int* Ap, const int* Ai, const int* inv);
+
+// The symmetric order of a matrix for the GPU dense LU: reverse Cuthill-McKee
+// when its envelope needs at most half the trailing-update tiles of the given
+// order, else the given order.  Returns true and fills ord (ord[new] = old)
+// and inv (inv[old] = new) when it reorders; env is the envelope of the order
+// chosen.
+bool band_order(int n, int npad, int bs, const int* Ap, const int* Ai, std::vector<int>& ord, std::vector<int>& inv,
+                BlockEnvelope& env);
+
 // Dense LU on the GPU: the CSC matrix is scattered into a device matrix of
 // order gpu_lu_padded(n) (identity beyond n) and factored there with
 // threshold partial pivoting (u = 0.001, diagonal preferred, as UMFPACK's

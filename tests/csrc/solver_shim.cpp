@@ -12,3 +12,26 @@ extern "C" int ts_solve(int n, const int* Ap, const int* Ai, const double* Ax, c
     if (fill) *fill = lu.fill();
     return rc;
 }
+
+// the GPU dense LU's symmetric order and block envelope (host code)
+extern "C" int ts_band_order(int n, int npad, int bs, const int* Ap, const int* Ai, int* ord, int* first, int* last,
+                             int* top, long long* tiles)
+{
+    std::vector<int> o, inv;
+    d4host::BlockEnvelope e;
+    const bool re = d4host::band_order(n, npad, bs, Ap, Ai, o, inv, e);
+    for (int i = 0; i < n; ++i) ord[i] = re ? o[i] : i;
+    for (int b = 0; b < npad / bs; ++b) {
+        first[b] = e.first[b];
+        last[b] = e.last[b];
+        top[b] = e.top[b];
+    }
+    *tiles = e.tiles;
+    return re ? 1 : 0;
+}
+
+extern "C" void ts_rcm_order(int n, const int* Ap, const int* Ai, int* ord)
+{
+    const std::vector<int> o = d4host::rcm_order(n, Ap, Ai);
+    for (int i = 0; i < n; ++i) ord[i] = o[i];
+}

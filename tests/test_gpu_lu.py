@@ -2,8 +2,9 @@
 include/d4_sim.h; UMFPACK's place, simulation.cpp:635-659) against numpy's
 LAPACK solve, its threshold pivoting rule (u = 0.001, UMFPACK's symmetric
 strategy: the diagonal is kept while every multiplier is at most 1000 in
-magnitude), singular matrices, invalid input, and the independence of a
-system's bits from the rest of its batch."""
+magnitude), singular matrices, invalid input, the independence of a
+system's bits from the rest of its batch, and the reordering of cutoff
+patterns into a narrow envelope whose zero blocks the factorization skips."""
 from importlib import import_module
 
 import numpy as np
@@ -178,3 +179,84 @@ def test_large_and_empty_systems_in_one_batch(ctx):
     assert xs[2][0] == 0.5
     assert _rel_err(xs[0], np.linalg.solve(big, b_big)) <= 1e-12
     assert _backward(big, xs[0], b_big) <= 1e-13  # the scale of n u = 4096 * 1.1e-16
+
+
+def _cutoff_matrix(n, rng, rc=0.05, scale=0.5):
+    # a step matrix's shape: the identity plus entries on the heaviside-cutoff
+    # pattern of n random points in the periodic unit square, rows scaled so
+    # that the off-diagonal sum stays below `scale`
+    x, y = rng.uniform(-0.5, 0.5, n), rng.uniform(-0.5, 0.5, n)
+    dx = x[:, None] - x[None, :]
+    dy = y[:, None] - y[None, :]
+    dx -= np.rint(dx)
+    dy -= np.rint(dy)
+    P = (dx * dx + dy * dy < rc * rc) & ~np.eye(n, dtype=bool)
+    V = np.where(P, rng.uniform(-1, 1, (n, n)), 0.0)
+    return np.eye(n) + V * (scale / max(1.0, np.abs(V).sum(1).max()))
+
+
+def test_cutoff_systems_reordered_match_lapack(ctx):
+    # cutoff patterns are factored in reverse Cuthill-McKee order with the
+    # blocks outside the envelope skipped; the orders cover 1 to 64 blocks
+    rng = np.random.default_rng(12)
+    sizes = [90, 300, 1000, 2100, 4000]
+    mats = [_cutoff_matrix(n, rng) for n in sizes]
+    bs = [rng.uniform(-1, 1, n) for n in sizes]
+    xs, sing, unpiv = simmod.lu_solve_batch(ctx, [_sys(A, b) for A, b in zip(mats, bs)])
+    assert not sing.any() and unpiv.all()
+    for A, b, x in zip(mats, bs, xs):
+        assert _rel_err(x, np.linalg.solve(A, b)) <= 1e-12, A.shape
+        assert _backward(A, x, b) <= 1e-15 * A.shape[0], A.shape
+    # the bits of each system do not depend on its batch
+    alone = [simmod.lu_solve_batch(ctx, [_sys(A, b)])[0][0] for A, b in zip(mats, bs)]
+    for a, t in zip(alone, xs):
+        np.testing.assert_array_equal(a, t)
+
+
+def test_envelope_skipping_keeps_the_bits_of_the_full_factorization(ctx):
+    # a matrix already narrow in its own order keeps that order; the blocks the
+    # envelope skips are zero, so the CSC path (skipping) and the dense device
+    # path (every block) give the same bits
+    import torch
+    rng = np.random.default_rng(13)
+    n = 700
+    i, j = np.indices((n, n))
+    A = np.where(np.abs(i - j) <= 40, rng.uniform(-1, 1, (n, n)), 0.0) * 0.012 + np.eye(n)
+    b = rng.standard_normal(n)
+    (x_csc,), _, (unpiv,) = simmod.lu_solve_batch(ctx, [_sys(A, b)])
+    assert unpiv
+    sing, u = ctx.lu_factor_dense_t(torch.from_numpy(A).to("cuda:0"))
+    assert not sing and u
+    x = ctx.lu_solve_dense_t(torch.from_numpy(b).to("cuda:0")).cpu().numpy()
+    np.testing.assert_array_equal(x, x_csc)
+    assert _rel_err(x, np.linalg.solve(A, b)) <= 1e-13
+
+
+def test_reordered_matrix_agrees_with_its_natural_order(ctx):
+    # the same cutoff matrix through the CSC path (reordered) and the dense
+    # device path (natural order): different rounding, the same solution
+    import torch
+    rng = np.random.default_rng(14)
+    A = _cutoff_matrix(1500, rng)
+    b = rng.standard_normal(1500)
+    (x_re,), _, _ = simmod.lu_solve_batch(ctx, [_sys(A, b)])
+    ctx.lu_factor_dense_t(torch.from_numpy(A).to("cuda:0"))
+    x_nat = ctx.lu_solve_dense_t(torch.from_numpy(b).to("cuda:0")).cpu().numpy()
+    assert _rel_err(x_re, x_nat) <= 1e-13
+
+
+def test_reordered_matrix_needing_interchanges(ctx):
+    # a cutoff pattern with a diagonal far below its off-diagonal entries: the
+    # multiplier check fails and the reordered matrix is refactored whole with
+    # partial pivoting; the solution comes back in the caller's order
+    rng = np.random.default_rng(15)
+    A = _cutoff_matrix(800, rng, scale=1.0)
+    A[np.diag_indices(800)] = rng.uniform(0.5, 1.0, 800) * 1e-6
+    b = rng.standard_normal(800)
+    good = _cutoff_matrix(600, rng)
+    b2 = rng.standard_normal(600)
+    xs, sing, unpiv = simmod.lu_solve_batch(ctx, [_sys(A, b), _sys(good, b2)])
+    assert not sing.any()
+    assert unpiv.tolist() == [False, True]
+    assert _backward(A, xs[0], b) <= 1e-13
+    assert _rel_err(xs[1], np.linalg.solve(good, b2)) <= 1e-12

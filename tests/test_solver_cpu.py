@@ -90,3 +90,100 @@ def test_sparse_lu_reports_singular(ts):
     A[17, :] = 0.0
     rc_, x, _ = _solve(ts, A, np.ones(200))
     assert rc_ == 1
+
+
+# --- the GPU dense LU's symmetric order and block envelope (host code) -------
+
+def _band(ts, A, bs=64):
+    ts.ts_band_order.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 6 + [C.POINTER(C.c_longlong)]
+    n = A.shape[0]
+    npad = (n + bs - 1) // bs * bs
+    nb = npad // bs
+    Ap, Ai, _ = _csc(A)
+    ordv = np.empty(n, np.int32)
+    first, last, top = (np.empty(nb, np.int32) for _ in range(3))
+    tiles = C.c_longlong(0)
+    p = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
+    re = ts.ts_band_order(n, npad, bs, p(Ap), p(Ai), p(ordv), p(first), p(last), p(top), C.byref(tiles))
+    return bool(re), ordv, first, last, top, tiles.value
+
+
+def _cutoff_pattern(n, rng, rc=0.05):
+    # the heaviside-cutoff Jacobian pattern of n random points in the periodic unit square
+    x, y = rng.uniform(-0.5, 0.5, n), rng.uniform(-0.5, 0.5, n)
+    dx, dy = wrap(x[:, None] - x[None, :]), wrap(y[:, None] - y[None, :])
+    return dx * dx + dy * dy < rc * rc
+
+
+def _lu_nopivot(A):
+    A = A.copy()
+    for k in range(A.shape[0] - 1):
+        A[k + 1:, k] /= A[k, k]
+        A[k + 1:, k + 1:] -= np.outer(A[k + 1:, k], A[k, k + 1:])
+    return A
+
+
+def _outside_envelope(F, last, bs=64):
+    # nonzeros of the factors in blocks the envelope declares zero
+    nb = len(last)
+    bad = 0
+    for p in range(nb):
+        for b in range(last[p] + 1, nb):
+            bad += np.count_nonzero(F[b * bs:(b + 1) * bs, p * bs:(p + 1) * bs])
+            bad += np.count_nonzero(F[p * bs:(p + 1) * bs, b * bs:(b + 1) * bs])
+    return bad
+
+
+@pytest.mark.parametrize("n", [300, 1000])
+def test_cutoff_pattern_is_reordered_into_a_narrow_envelope(ts, n):
+    rng = np.random.default_rng(n)
+    P = _cutoff_pattern(n, rng)
+    re, ordv, first, last, top, tiles = _band(ts, P.astype(float))
+    nb = len(last)
+    assert re
+    assert np.array_equal(np.sort(ordv), np.arange(n))
+    dense_tiles = sum((nb - p - 1) ** 2 for p in range(nb))
+    assert tiles * 8 <= dense_tiles, (tiles, dense_tiles)
+    assert tiles == sum((last[p] - p) ** 2 for p in range(nb))
+    # envelope invariants: last nondecreasing and >= its block, top <= first <= block
+    assert np.all(np.diff(last) >= 0) and np.all(last >= np.arange(nb))
+    assert np.all(top <= first) and np.all(first <= np.arange(nb))
+    # elimination without interchanges of the reordered matrix (padded with
+    # the identity) leaves every block past the envelope exactly zero
+    vals = np.where(P, rng.uniform(-1, 1, P.shape), 0.0) / P.sum(1).max()
+    A = vals + 2.0 * np.eye(n)
+    npad = nb * 64
+    Ar = np.eye(npad)
+    Ar[:n, :n] = A[np.ix_(ordv, ordv)]
+    assert _outside_envelope(_lu_nopivot(Ar), last) == 0
+
+
+def test_narrow_natural_order_is_kept(ts):
+    # a band of half-width 40 in its own order: reverse Cuthill-McKee cannot halve the tiles
+    n = 700
+    i, j = np.indices((n, n))
+    A = np.where(np.abs(i - j) <= 40, 0.01, 0.0) + np.eye(n)
+    re, ordv, first, last, top, tiles = _band(ts, A)
+    assert not re and np.array_equal(ordv, np.arange(n))
+    assert list(last[:-1]) == [b + 1 for b in range(len(last) - 1)]
+    npad = len(last) * 64
+    Ap = np.eye(npad)
+    Ap[:n, :n] = A
+    assert _outside_envelope(_lu_nopivot(Ap), last) == 0
+
+
+def test_rcm_order_is_a_permutation_per_component(ts):
+    # two components and an isolated node: every node appears once
+    ts.ts_rcm_order.argtypes = [C.c_int] + [C.c_void_p] * 3
+    A = np.eye(9)
+    for a, b in [(0, 3), (3, 5), (1, 2), (2, 7), (7, 8)]:
+        A[a, b] = A[b, a] = 1.0
+    Ap, Ai, _ = _csc(A)
+    o = np.empty(9, np.int32)
+    ts.ts_rcm_order(9, C.c_void_p(Ap.ctypes.data), C.c_void_p(Ai.ctypes.data), C.c_void_p(o.ctypes.data))
+    assert sorted(o.tolist()) == list(range(9))
+    # a path graph is numbered end to end: neighbours stay adjacent
+    pos = np.empty(9, int)
+    pos[o] = np.arange(9)
+    for a, b in [(0, 3), (3, 5), (1, 2), (2, 7), (7, 8)]:
+        assert abs(pos[a] - pos[b]) == 1
