@@ -76,7 +76,10 @@ int d4_sim_write_dconf(const d4_sim* s, const char* path);
  * n[k] + 1 column pointers from 0, Ai[k] rows in [0, n[k]), Ax[k] values,
  * duplicates not allowed), b[k] its right-hand side, x[k] its solution (may be
  * b[k]).  Dense LU on the GPU with threshold partial pivoting (u = 0.001, the
- * diagonal preferred, as UMFPACK's symmetric strategy); singular[k] = 1 when the factorization met a zero
+ * diagonal preferred, as UMFPACK's symmetric strategy), in a symmetric
+ * bandwidth-reducing order (reverse Cuthill-McKee, taken from the pattern
+ * when it halves the work) whose zero blocks outside the envelope are
+ * skipped; singular[k] = 1 when the factorization met a zero
  * pivot (x[k] then holds what the solve gives, as with UMFPACK's warning),
  * unpivoted[k] = 1 when no rows were interchanged; either may be null.  A
  * system's solution does not depend on the other systems of the call. */
