@@ -395,26 +395,29 @@ __device__ inline int owner(const int* __restrict__ pre, int count, int g)
     return lo;
 }
 
-__global__ __launch_bounds__(256) void lu_diag_kernel(const FacDesc* __restrict__ fd, double* __restrict__ A, int j0)
+// LU without interchanges of each matrix's diagonal block, one wavefront per
+// matrix: lane r holds row r in registers.  At step j every lane reads the
+// pivot row (lane j) with readlane, and each row below forms its multiplier
+// l = a_rj / a_jj and updates a_rc -= l a_jc (c > j); rows at or above the
+// pivot take l = 0, which leaves them as they are.  No barrier and no LDS.
+__global__ __launch_bounds__(64) void lu_diag_kernel(const FacDesc* __restrict__ fd, double* __restrict__ A, int j0)
 {
-    __shared__ double T[kFB * kFLd];
     const FacDesc m = fd[blockIdx.x];
     const size_t lda = (size_t)m.npad;
     double* blk = A + m.a_off + (size_t)j0 * lda + j0;
-    const int t = threadIdx.x, rr = t & 63, cg = t >> 6;
-    for (int c = cg; c < kFB; c += 4) T[c * kFLd + rr] = blk[(size_t)c * lda + rr];
-    __syncthreads();
+    const int r = threadIdx.x;
+    double a[kFB];
+#pragma unroll
+    for (int c = 0; c < kFB; ++c) a[c] = blk[(size_t)c * lda + r];
+#pragma unroll
     for (int j = 0; j < kFB - 1; ++j) {
-        // every thread of row rr > j forms the same multiplier; the first writes it
-        const double l = rr > j ? T[j * kFLd + rr] / T[j * kFLd + j] : 0.0;
-        __syncthreads();
-        if (rr > j) {
-            if (cg == 0) T[j * kFLd + rr] = l;
-            for (int c = j + 1 + cg; c < kFB; c += 4) T[c * kFLd + rr] = fma(-l, T[c * kFLd + j], T[c * kFLd + rr]);
-        }
-        __syncthreads();
+        const double l = r > j ? a[j] / lane_value(a[j], j) : 0.0;
+        if (r > j) a[j] = l;
+#pragma unroll
+        for (int c = j + 1; c < kFB; ++c) a[c] = fma(-l, lane_value(a[c], j), a[c]);
     }
-    for (int c = cg; c < kFB; c += 4) blk[(size_t)c * lda + rr] = T[c * kFLd + rr];
+#pragma unroll
+    for (int c = 0; c < kFB; ++c) blk[(size_t)c * lda + r] = a[c];
 }
 
 // One 64 x 64 block per wavefront, four blocks of one matrix per workgroup
@@ -946,7 +949,7 @@ private:
         for (int p = 0; p < P; ++p) {
             const Step& sp = steps[p];
             const int j0 = kFB * p;
-            hipLaunchKernelGGL(lu_diag_kernel, dim3(sp.act), dim3(256), 0, st, fdesc_.p, A_.p, j0);
+            hipLaunchKernelGGL(lu_diag_kernel, dim3(sp.act), dim3(kFB), 0, st, fdesc_.p, A_.p, j0);
             if (sp.n2 > 0)
                 hipLaunchKernelGGL(lu_panel_kernel, dim3(sp.n2 / 4), dim3(256), 0, st, fdesc_.p,
                                    fpre_.p + sp.pre2, fpre_.p + sp.tbo, sp.act, A_.p, j0);
