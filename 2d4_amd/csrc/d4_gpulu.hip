@@ -6,7 +6,10 @@
 // At the sizes of a realization (N <= a few thousand) one dense LU on the GPU
 // takes milliseconds where a sparse LU of the cutoff pattern on a host core
 // takes up to seconds (the 2D cutoff graph fills in heavily), so this is the
-// solver the batched ensemble and large single runs use.
+// solver the batched ensemble and large single runs use.  A matrix given in
+// CSC form is factored in a symmetric order taken from its pattern (reverse
+// Cuthill-McKee when that halves the work, see band_order in d4_solver.h),
+// and only the blocks of its envelope are filled, factored and read.
 //
 // Layout: matrix k is column-major of order npad_k = gpu_lu_padded(n_k) (n_k
 // rounded up to 64, the factorization's block), the CSC entries scattered into its leading n_k x n_k
