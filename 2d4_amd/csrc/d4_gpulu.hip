@@ -427,13 +427,17 @@ __global__ __launch_bounds__(64) void lu_diag_kernel(const FacDesc* __restrict__
 // (each matrix's items padded to a multiple of four), the diagonal block
 // staged in LDS once; after that barrier no wavefront waits on another.  In
 // an L21 block lane lo holds row lo of X in registers (X U11 = B, columns left
-// to right), in a U12 block column lo (L11 X = B, rows top to bottom); the
-// factor's entries are LDS broadcasts.  The order of operations of every
-// entry is that of the substitution written out: x_c2 -= x_c u_{c, c2}, c ascending.
-__global__ __launch_bounds__(256, 2) void lu_panel_kernel(const FacDesc* __restrict__ fd, const int* __restrict__ pre,
+// to right), in a U12 block column lo (L11 X = B, rows top to bottom).  The
+// factor's entries reach every lane by readlane from the lane that holds
+// them (lane c2 keeps column c2 of U11, or row c2 of L11), each read from LDS
+// kAhead steps before its use, so no step waits on LDS.  The order of
+// operations of every entry is that of the substitution written out:
+// x_c2 -= x_c u_{c, c2}, c ascending.
+__global__ __launch_bounds__(256, 1) void lu_panel_kernel(const FacDesc* __restrict__ fd, const int* __restrict__ pre,
                                                           const int* __restrict__ tb, int count,
                                                           double* __restrict__ A, int j0)
 {
+    constexpr int kAhead = 8;
     __shared__ double Dg[kFB * kFLd];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int g0 = xcd_item() * 4;  // the workgroup's first item; its four items share a matrix
@@ -449,17 +453,20 @@ __global__ __launch_bounds__(256, 2) void lu_panel_kernel(const FacDesc* __restr
     if (item >= 2 * T) return;                     // padding
     const bool rowblk = item < T;                  // L21 block (below) or U12 block (right)
     const int b = (rowblk ? item : item - T) + 1;  // block index past the diagonal
-    double x[kFB];
+    double x[kFB], f[kFB];
     if (rowblk) {
         double* blk = A + m.a_off + (size_t)j0 * lda + (j0 + kFB * b);
         const double rd = 1.0 / Dg[lo * kFLd + lo];  // lane c: 1 / U11(c, c)
 #pragma unroll
         for (int c = 0; c < kFB; ++c) x[c] = blk[(size_t)c * lda + lo];
 #pragma unroll
-        for (int c = 0; c < kFB; ++c) {
-            x[c] *= __shfl(rd, c);
+        for (int k = 0; k < kAhead; ++k) f[k] = Dg[lo * kFLd + k];  // f[r] = U11(r, lo)
 #pragma unroll
-            for (int c2 = c + 1; c2 < kFB; ++c2) x[c2] = fma(-x[c], Dg[c2 * kFLd + c], x[c2]);
+        for (int c = 0; c < kFB; ++c) {
+            if (c + kAhead < kFB) f[c + kAhead] = Dg[lo * kFLd + c + kAhead];
+            x[c] *= lane_value(rd, c);
+#pragma unroll
+            for (int c2 = c + 1; c2 < kFB; ++c2) x[c2] = fma(-x[c], lane_value(f[c], c2), x[c2]);
         }
 #pragma unroll
         for (int c = 0; c < kFB; ++c) blk[(size_t)c * lda + lo] = x[c];
@@ -468,9 +475,13 @@ __global__ __launch_bounds__(256, 2) void lu_panel_kernel(const FacDesc* __restr
 #pragma unroll
         for (int r = 0; r < kFB; ++r) x[r] = col[r];
 #pragma unroll
-        for (int r = 0; r < kFB - 1; ++r)
+        for (int k = 0; k < kAhead; ++k) f[k] = Dg[k * kFLd + lo];  // f[k] = L11(lo, k)
 #pragma unroll
-            for (int r2 = r + 1; r2 < kFB; ++r2) x[r2] = fma(-Dg[r * kFLd + r2], x[r], x[r2]);
+        for (int r = 0; r < kFB - 1; ++r) {
+            if (r + kAhead < kFB) f[r + kAhead] = Dg[(r + kAhead) * kFLd + lo];
+#pragma unroll
+            for (int r2 = r + 1; r2 < kFB; ++r2) x[r2] = fma(-lane_value(f[r], r2), x[r], x[r2]);
+        }
 #pragma unroll
         for (int r = 0; r < kFB; ++r) col[r] = x[r];
     }
@@ -655,8 +666,15 @@ public:
         if (!dense) {
             par(K, [&](int k) {
                 Plan& pl = plan_[k];
-                pl.re = m[k].n > 0 &&
-                        band_order(m[k].n, desc_h_[k].npad, kFB, m[k].Ap, m[k].Ai, pl.ord, pl.inv, pl.env);
+                const int n = m[k].n;
+                // the order depends on the pattern alone: kept while slot k's pattern repeats
+                if (n > 0 && pl.n == n && std::equal(m[k].Ap, m[k].Ap + n + 1, pl.pat_p.begin()) &&
+                    std::equal(m[k].Ai, m[k].Ai + m[k].Ap[n], pl.pat_i.begin()))
+                    return;
+                pl.n = n;
+                pl.pat_p.assign(m[k].Ap, m[k].Ap + (n > 0 ? n + 1 : 0));
+                pl.pat_i.assign(m[k].Ai, m[k].Ai + (n > 0 ? m[k].Ap[n] : 0));
+                pl.re = n > 0 && band_order(n, desc_h_[k].npad, kFB, m[k].Ap, m[k].Ai, pl.ord, pl.inv, pl.env);
             });
             for (int k = 0; k < K; ++k) {
                 MatDesc& d = desc_h_[k];
@@ -869,8 +887,10 @@ private:
         bool re = false;  // reordered: ord (ord[new] = old) and inv hold the order
         std::vector<int> ord, inv;
         BlockEnvelope env;
+        int n = -1;  // the pattern the order was taken from
+        std::vector<int> pat_p, pat_i;
     };
-    std::vector<Plan> plan_;  // per matrix of the last factor() (CSC input)
+    std::vector<Plan> plan_;  // per matrix slot of the last factor() (CSC input)
     DevBuf<int> aux_;         // orders and envelopes, at MatDesc::ord_off / env_off
     PinBuf<int> aux_h_;
     DevBuf<FacDesc> fdesc_;

@@ -260,3 +260,28 @@ def test_reordered_matrix_needing_interchanges(ctx):
     assert unpiv.tolist() == [False, True]
     assert _backward(A, xs[0], b) <= 1e-13
     assert _rel_err(xs[1], np.linalg.solve(good, b2)) <= 1e-12
+
+
+def test_order_kept_for_a_repeated_pattern(ctx):
+    # a slot whose pattern repeats keeps its order; the values are new each time,
+    # and the results equal those of a context that computes the order afresh
+    import importlib
+    pkg = importlib.import_module("2d4_amd")
+    rng = np.random.default_rng(16)
+    A = _cutoff_matrix(1200, rng)
+    P = A != 0
+    A2 = np.where(P, A * rng.uniform(0.5, 1.5, A.shape), 0.0)
+    b = rng.uniform(-1, 1, 1200)
+    simmod.lu_solve_batch(ctx, [_sys(A, b)])
+    (x_again,), _, _ = simmod.lu_solve_batch(ctx, [_sys(A2, b)])
+    fresh = pkg.Context(0)
+    try:
+        (x_fresh,), _, _ = simmod.lu_solve_batch(fresh, [_sys(A2, b)])
+    finally:
+        fresh.close()
+    np.testing.assert_array_equal(x_again, x_fresh)
+    assert _rel_err(x_again, np.linalg.solve(A2, b)) <= 1e-12
+    # a different pattern in the same slot is ordered afresh
+    B = _cutoff_matrix(1200, rng)
+    (xb,), _, _ = simmod.lu_solve_batch(ctx, [_sys(B, b)])
+    assert _rel_err(xb, np.linalg.solve(B, b)) <= 1e-12
