@@ -279,6 +279,10 @@ __device__ inline double lane_value(double v, int lane)  // v of `lane` (uniform
 // gathers b through ord and scatters x back through it; with an envelope the
 // updates stop at its bounds, past which the factors' columns are zero (a
 // zero sum leaves x_r as it is, so the bits are those of the full sweep).
+// Latency: the next diagonal block is staged (by waves 1..15, into the other
+// of two LDS buffers) while wave 0 solves the current one; wave 0 reads its
+// entries of the block kAhead steps before their use; a row update issues
+// its column loads 32 at a time.
 template <bool kLds>
 __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc* __restrict__ desc,
                                                                   const double* __restrict__ A,
@@ -286,66 +290,94 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
                                                                   double* __restrict__ scratch,
                                                                   const int* __restrict__ aux)
 {
+    constexpr int kAhead = 8, kBatch = 32;
     extern __shared__ double lds[];
     const SolveDesc m = desc[blockIdx.x];
     const int n = m.n, tid = threadIdx.x;
     const size_t lda = (size_t)m.npad;
     const double* Am = A + m.a_off;
     double* bm = b + m.b_off;
-    double* D = lds;  // kSB x kSB diagonal block, column-major
-    double* x = kLds ? lds + kSB * kSB : scratch + m.b_off;
+    double* Dbuf = lds;  // two kSB x kSB diagonal blocks, column-major
+    double* x = kLds ? lds + 2 * kSB * kSB : scratch + m.b_off;
     const int* ord = m.ord_off >= 0 ? aux + m.ord_off : nullptr;
     const int* env = m.env_off >= 0 ? aux + m.env_off : nullptr;
     const int nb = m.npad / kSB;
+    const int nblk = (n + kSB - 1) / kSB;
+    // diagonal block number s of the two sweeps (0..2 nblk - 1) into buffer s & 1,
+    // by the threads t0, t0 + stride, ...
+    auto stage = [&](int s, int t0, int stride) {
+        const int bi = s < nblk ? s : 2 * nblk - 1 - s;
+        const int jb = bi * kSB, B = min(kSB, n - jb);
+        double* D = Dbuf + (s & 1) * kSB * kSB;
+        for (int e = t0; e < kSB * kSB; e += stride) {
+            const int r = e % kSB, c = e / kSB;
+            D[e] = (r < B && c < B) ? Am[(size_t)(jb + c) * lda + jb + r] : 0.0;
+        }
+    };
     for (int i = tid; i < n; i += kSolveThreads) {
         int p = m.perm_off >= 0 ? perm[m.perm_off + i] : i;  // padded rows never move into [0, n)
         p = (p >= 0 && p < n) ? p : i;
         x[i] = bm[ord ? ord[p] : p];
     }
+    if (nblk > 0) stage(0, tid, kSolveThreads);
     __syncthreads();
 
-    const int nblk = (n + kSB - 1) / kSB;
-    for (int pass = 0; pass < 2; ++pass) {  // 0: L y = P b top down, 1: U x = y bottom up
-        for (int s = 0; s < nblk; ++s) {
-            const int bi = pass == 0 ? s : nblk - 1 - s;
-            const int jb = bi * kSB, B = min(kSB, n - jb);
-            for (int e = tid; e < kSB * kSB; e += kSolveThreads) {
-                const int r = e % kSB, c = e / kSB;
-                D[e] = (r < B && c < B) ? Am[(size_t)(jb + c) * lda + jb + r] : 0.0;
-            }
-            __syncthreads();
-            if (tid < 64) {
-                const int l = tid;
-                double v = l < B ? x[jb + l] : 0.0;
-                if (pass == 0) {
-#pragma unroll 8
-                    for (int c = 0; c < kSB - 1; ++c) {
-                        const double yc = lane_value(v, c);
-                        if (l > c) v = fma(-D[c * kSB + l], yc, v);
-                    }
-                } else {
-#pragma unroll 8
-                    for (int c = kSB - 1; c >= 0; --c) {
-                        if (c >= B) continue;
-                        if (l == c) v = v / D[c * kSB + c];
+    for (int s = 0; s < 2 * nblk; ++s) {
+        const int pass = s < nblk ? 0 : 1;  // 0: L y = P b top down, 1: U x = y bottom up
+        const int bi = pass == 0 ? s : 2 * nblk - 1 - s;
+        const int jb = bi * kSB, B = min(kSB, n - jb);
+        const double* D = Dbuf + (s & 1) * kSB * kSB;
+        if (tid < 64) {
+            const int l = tid;
+            double v = l < B ? x[jb + l] : 0.0;
+            double d[kSB];  // d[c] = D(l, c)
+            if (pass == 0) {
+#pragma unroll
+                for (int k = 0; k < kAhead; ++k) d[k] = D[k * kSB + l];
+#pragma unroll
+                for (int c = 0; c < kSB - 1; ++c) {
+                    if (c + kAhead < kSB - 1) d[c + kAhead] = D[(c + kAhead) * kSB + l];
+                    const double yc = lane_value(v, c);
+                    if (l > c) v = fma(-d[c], yc, v);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < kAhead; ++k) d[kSB - 1 - k] = D[(kSB - 1 - k) * kSB + l];
+#pragma unroll
+                for (int c = kSB - 1; c >= 0; --c) {
+                    if (c - kAhead >= 0) d[c - kAhead] = D[(c - kAhead) * kSB + l];
+                    if (c < B) {
+                        if (l == c) v = v / d[c];
                         const double xc = lane_value(v, c);
-                        if (l < c) v = fma(-D[c * kSB + l], xc, v);
+                        if (l < c) v = fma(-d[c], xc, v);
                     }
                 }
-                if (l < B) x[jb + l] = v;
             }
-            __syncthreads();
-            const int r0 = pass == 0 ? jb + B : (env ? env[2 * nb + bi] * kSB : 0);
-            const int r1 = pass == 0 ? (env ? min(n, (env[bi] + 1) * kSB) : n) : jb;
-            for (int r = r0 + tid; r < r1; r += kSolveThreads) {
-                const double* a = Am + (size_t)jb * lda + r;
-                double acc = 0.0;
-#pragma unroll 8
-                for (int c = 0; c < B; ++c) acc = fma(a[c * lda], x[jb + c], acc);
-                x[r] -= acc;
-            }
-            __syncthreads();
+            if (l < B) x[jb + l] = v;
+        } else if (s + 1 < 2 * nblk) {
+            stage(s + 1, tid - 64, kSolveThreads - 64);
         }
+        __syncthreads();
+        const int r0 = pass == 0 ? jb + B : (env ? env[2 * nb + bi] * kSB : 0);
+        const int r1 = pass == 0 ? (env ? min(n, (env[bi] + 1) * kSB) : n) : jb;
+        for (int r = r0 + tid; r < r1; r += kSolveThreads) {
+            const double* a = Am + (size_t)jb * lda + r;
+            double acc = 0.0;
+            if (B == kSB) {
+#pragma unroll
+                for (int h = 0; h < kSB; h += kBatch) {
+                    double av[kBatch];
+#pragma unroll
+                    for (int c = 0; c < kBatch; ++c) av[c] = a[(size_t)(h + c) * lda];
+#pragma unroll
+                    for (int c = 0; c < kBatch; ++c) acc = fma(av[c], x[jb + h + c], acc);
+                }
+            } else {
+                for (int c = 0; c < B; ++c) acc = fma(a[c * lda], x[jb + c], acc);
+            }
+            x[r] -= acc;
+        }
+        __syncthreads();
     }
     for (int i = tid; i < n; i += kSolveThreads) bm[ord ? ord[i] : i] = x[i];
 }
@@ -567,7 +599,7 @@ public:
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lds = 65536;
         lds_bytes_ = lds;
-        lds_order_ = std::max(0, (lds / 8 - kSB * kSB) / 256 * 256);
+        lds_order_ = std::max(0, (lds / 8 - 2 * kSB * kSB) / 256 * 256);
     }
 
     ~BatchLU() override
@@ -1046,11 +1078,12 @@ private:
         if (nsolve_ == 0) return 0;
         if (npad_max_ <= lds_order_)
             hipLaunchKernelGGL(lu_solve_kernel<true>, dim3(nsolve_), dim3(kSolveThreads),
-                               sizeof(double) * (kSB * kSB + npad_max_), st, sdesc_.p, A_.p, ipiv_.p, b_.p,
+                               sizeof(double) * (2 * kSB * kSB + npad_max_), st, sdesc_.p, A_.p, ipiv_.p, b_.p,
                                b_.p + total, aux_.p);
         else
-            hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(nsolve_), dim3(kSolveThreads), sizeof(double) * kSB * kSB,
-                               st, sdesc_.p, A_.p, ipiv_.p, b_.p, b_.p + total, aux_.p);
+            hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(nsolve_), dim3(kSolveThreads),
+                               sizeof(double) * 2 * kSB * kSB, st, sdesc_.p, A_.p, ipiv_.p, b_.p, b_.p + total,
+                               aux_.p);
         return hipGetLastError() != hipSuccess ? -1 : 0;
     }
 
