@@ -22,7 +22,7 @@ class SimStats(C.Structure):
     _fields_ = [("sim_time", _d), ("step_size", _d), ("strain", _d), ("last_max_error_ratio_sq", _d),
                 ("successful_steps", C.c_int64), ("failed_steps", C.c_int64), ("pair_passes", C.c_int64),
                 ("factorizations", C.c_int64), ("singular_solves", C.c_int64),
-                ("unpivoted_factorizations", C.c_int64)]
+                ("unpivoted_factorizations", C.c_int64), ("pass_ms", _d), ("solve_ms", _d)]
 
 
 class EnsConfig(C.Structure):

@@ -170,7 +170,10 @@ struct PassJob {
 class PassEngine {
 public:
     virtual ~PassEngine() = default;
-    virtual int run(Simulation& sim, PassJob& job);
+    virtual int run(Simulation& sim, PassJob& job);  // run_job, timed into the simulation's counters
+
+protected:
+    int run_job(Simulation& sim, PassJob& job);
 };
 
 class Simulation {
@@ -184,6 +187,7 @@ public:
     int status() const { return status_; }
     const std::string& error() const { return error_; }
     int64_t pairPasses = 0, factorizations = 0, singularSolves = 0, unpivotedFactorizations = 0;
+    double passMs = 0.0, solveMs = 0.0;  // host wall time in pair passes and in linear solves
 
     // the step split at its pair passes and linear solves: begin_step() (false
     // when interrupt() holds) sets the first job, then each after_pass()

@@ -3,6 +3,7 @@
 #include "d4_host.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -240,6 +241,16 @@ void Simulation::set_rescale_job(double baseTime, double newTime, double oldTime
 }
 
 int PassEngine::run(Simulation& sim, PassJob& job)
+{
+    // host wall time of the job, pair passes and linear solves apart
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = run_job(sim, job);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    (job.kind == PassJob::Factor || job.kind == PassJob::Solve ? sim.solveMs : sim.passMs) += ms;
+    return rc;
+}
+
+int PassEngine::run_job(Simulation& sim, PassJob& job)
 {
     SimulationData& d = sim.data();
     PrecisionHandler& pH = sim.precision();
@@ -669,6 +680,8 @@ int d4_sim_get_stats(const d4_sim* s, d4_sim_stats* st)
     st->factorizations = s->sim->factorizations;
     st->singular_solves = s->sim->singularSolves;
     st->unpivoted_factorizations = s->sim->unpivotedFactorizations;
+    st->pass_ms = s->sim->passMs;
+    st->solve_ms = s->sim->solveMs;
     return D4_OK;
 }
 

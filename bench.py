@@ -185,12 +185,15 @@ def implicit_step_bench(pkg, ctx, gpu_steps=50, cpu_steps=10, with_cpu=True, see
     def timed(make, steps):
         sim = make()
         sim.step()  # the first iteration includes run()'s initial speed pass
+        st0 = sim.stats()
         t0 = time.perf_counter()
         for _ in range(steps):
             sim.step()
         dt = (time.perf_counter() - t0) / steps
         st = sim.stats()
         sim.close()
+        st["pass_ms_per_step"] = (st["pass_ms"] - st0["pass_ms"]) / steps
+        st["solve_ms_per_step"] = (st["solve_ms"] - st0["solve_ms"]) / steps
         return dt, st
 
     out = {"workload": "N=1024 uniform, heaviside cutoff 0.05, tau=0.1, adaptive (the C2 configuration)",
@@ -198,7 +201,9 @@ def implicit_step_bench(pkg, ctx, gpu_steps=50, cpu_steps=10, with_cpu=True, see
     for label, solver in (("gpu_passes_host_lu", 0), ("gpu_passes_gpu_lu", 1)):
         dt, st = timed(lambda: simmod.Simulation.from_arrays(ctx, x, y, b, solver=solver, **params), gpu_steps)
         out[label] = {"ms_per_step": dt * 1e3, "steps": gpu_steps, "accepted": st["successful_steps"],
-                      "failed": st["failed_steps"], "fraction_of_fp64_peak": ops / dt / 78.6e12}
+                      "failed": st["failed_steps"], "fraction_of_fp64_peak": ops / dt / 78.6e12,
+                      "pair_passes_ms_per_step": st["pass_ms_per_step"],
+                      "linear_solves_ms_per_step": st["solve_ms_per_step"]}
     shim = os.path.join(ROOT, "tests", "csrc", "libsim_cpu_fast.so")
     if with_cpu and os.path.exists(shim):
         extra = importlib.import_module("2d4_amd._bind_extra")
@@ -207,6 +212,8 @@ def implicit_step_bench(pkg, ctx, gpu_steps=50, cpu_steps=10, with_cpu=True, see
         dt, st = timed(lambda: simmod.Simulation._create(L, C.c_void_p(1), x, y, b, **params), cpu_steps)
         out["cpu"] = {"ms_per_step": dt * 1e3, "steps": cpu_steps, "cores": 1, "kind": "port",
                       "accepted": st["successful_steps"], "failed": st["failed_steps"],
+                      "pair_passes_ms_per_step": st["pass_ms_per_step"],
+                      "linear_solves_ms_per_step": st["solve_ms_per_step"],
                       "sample": "the host loop (d4_sim.cpp) with the oracle's single-threaded passes and the host "
                                 "sparse LU, -Ofast -ffast-math -march=x86-64-v3, as the reference runs (one core)"}
     return out

@@ -45,6 +45,8 @@ typedef struct {
     int64_t pair_passes, factorizations, singular_solves;
     int64_t unpivoted_factorizations; /* of factorizations: those the GPU dense LU finished
                                          without row interchanges (every multiplier <= 1000) */
+    double pass_ms, solve_ms;         /* wall time the caller spent in pair passes (with their
+                                         transfers) and in linear solves, in ms */
 } d4_sim_stats;
 
 /* Fills p with the reference's defaults (project_parser.cpp:44-50, 64). */
