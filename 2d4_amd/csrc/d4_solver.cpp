@@ -241,7 +241,14 @@ int SparseLU::factor(int n, const int* Ap, const int* Ai, const double* Ax)
         pat_n_ = n;
         pat_p_.assign(Ap, Ap + n + 1);
         pat_i_.assign(Ai, Ai + nnz);
+        have_reach_ = false;
+    } else if (have_reach_ && refactor(n, Ap, Ai, Ax)) {
+        ++refactorizations_;
+        return singular_ ? 1 : 0;
     }
+    have_reach_ = false;
+    reach_p_.assign(n + 1, 0);
+    reach_i_.clear();
     pinv_.assign(n, -1);
     prow_.assign(n, -1);
     Lp_.assign(n + 1, 0);
@@ -258,6 +265,7 @@ int SparseLU::factor(int n, const int* Ap, const int* Ai, const double* Ax)
     std::vector<double> x(n, 0.0);
     std::vector<int> stack(n), pstack(n), xi(n);
     std::vector<char> mark(n, 0);
+    bool structural = false;
     for (int k = 0; k < n; ++k) {
         const int col = q_[k];
         // reach of the column's pattern in the graph of L (pivot-indexed), topological order
@@ -292,6 +300,8 @@ int SparseLU::factor(int n, const int* Ap, const int* Ai, const double* Ax)
                 }
             }
         }
+        reach_i_.insert(reach_i_.end(), xi.begin() + top, xi.end());
+        reach_p_[k + 1] = (int)reach_i_.size();
         // x = L \ A(:, col) over the reach
         for (int p = Ap[col]; p < Ap[col + 1]; ++p) x[Ai[p]] = Ax[p];
         for (int t = top; t < n; ++t) {
@@ -319,6 +329,7 @@ int SparseLU::factor(int n, const int* Ap, const int* Ai, const double* Ax)
         }
         if (ipiv < 0) {  // structurally singular column
             singular_ = true;
+            structural = true;
             for (int i = 0; i < n; ++i)
                 if (pinv_[i] < 0) {
                     ipiv = i;
@@ -348,9 +359,69 @@ int SparseLU::factor(int n, const int* Ap, const int* Ai, const double* Ax)
         if (!mark[ipiv]) x[ipiv] = 0.0;
         Lp_[k + 1] = (int)Li_.size();
     }
-    // L row indices from original rows to pivot order
+    // L row indices from original rows to pivot order (kept in original rows for refactor())
+    Lrow_ = Li_;
     for (auto& i : Li_) i = pinv_[i];
+    have_reach_ = !structural;
     return singular_ ? 1 : 0;
+}
+
+bool SparseLU::refactor(int n, const int* Ap, const int* Ai, const double* Ax)
+{
+    // The pattern, the column order and (checked column by column) the pivot
+    // sequence are the last factorization's, so each column's reach and its
+    // topological order are too: replay them, with the same operations in the
+    // same order as factor() (bitwise its results).  false: a pivot choice
+    // differs, and factor() starts over.
+    singular_ = false;
+    work_.assign(n, 0.0);
+    double* x = work_.data();
+    size_t lpos = 0, upos = 0;
+    for (int k = 0; k < n; ++k) {
+        const int col = q_[k];
+        const int* R = reach_i_.data() + reach_p_[k];
+        const int nr = reach_p_[k + 1] - reach_p_[k];
+        for (int p = Ap[col]; p < Ap[col + 1]; ++p) x[Ai[p]] = Ax[p];
+        for (int t = 0; t < nr; ++t) {
+            const int jc = pinv_[R[t]];
+            if (jc >= k) continue;  // not pivotal before column k
+            const double xj = x[R[t]];
+            for (int s = Lp_[jc]; s < Lp_[jc + 1]; ++s) x[Lrow_[s]] -= Lx_[s] * xj;
+        }
+        int ipiv = -1;
+        double amax = -1.0;
+        bool has_col = false;
+        for (int t = 0; t < nr; ++t) {
+            const int i = R[t];
+            has_col |= i == col;
+            if (pinv_[i] >= k) {
+                const double v = std::fabs(x[i]);
+                if (v > amax) {
+                    amax = v;
+                    ipiv = i;
+                }
+            } else {
+                Ux_[upos++] = x[i];
+            }
+        }
+        if (ipiv >= 0 && pinv_[col] >= k && has_col && std::fabs(x[col]) >= 0.1 * amax) ipiv = col;
+        if (ipiv != prow_[k]) {
+            for (int t = 0; t < nr; ++t) x[R[t]] = 0.0;
+            for (int p = Ap[col]; p < Ap[col + 1]; ++p) x[Ai[p]] = 0.0;
+            return false;
+        }
+        const double piv = x[ipiv];
+        if (piv == 0.0) singular_ = true;
+        Ux_[upos++] = piv;
+        const double inv = piv != 0.0 ? 1.0 / piv : 0.0;
+        for (int t = 0; t < nr; ++t) {
+            const int i = R[t];
+            if (pinv_[i] > k) Lx_[lpos++] = x[i] * inv;
+            x[i] = 0.0;
+        }
+        x[ipiv] = 0.0;
+    }
+    return true;
 }
 
 void SparseLU::solve_const(const double* b, double* xout) const

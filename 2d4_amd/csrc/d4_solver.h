@@ -17,7 +17,12 @@
 //   * dense LU with partial pivoting instead when the matrix is nearly full;
 //   * the column order is kept while the pattern stays the previous
 //     factorization's (it depends on the pattern alone, so results are the same).
-//     The ordering dominates at these sizes: 2.5 of 3.5 ms at N = 1024.
+//     The ordering dominates at these sizes: 2.5 of 3.5 ms at N = 1024;
+//   * with the pattern unchanged, each column's reach (found by the depth-first
+//     search, half the work) is replayed from the last factorization for as long
+//     as every pivot choice repeats: the same operations in the same order, so
+//     the results are bitwise those of a full factorization; the first pivot
+//     that differs sends it back to the full one.
 #pragma once
 
 #include <cstdint>
@@ -52,6 +57,8 @@ public:
     void solve_const(const double* b, double* x) const;
     bool singular() const override { return singular_; }
     int64_t fill() const { return (int64_t)Li_.size() + (int64_t)Ui_.size(); }
+    // factorizations that replayed the previous one's reach (same pattern and pivots)
+    int64_t refactorizations() const { return refactorizations_; }
 
 private:
     int n_ = 0;
@@ -68,6 +75,13 @@ private:
     std::vector<double> D_;
     std::vector<int> dpiv_;
     int factor_dense(int n, const int* Ap, const int* Ai, const double* Ax);
+    // numeric refactorization over the last factorization's reach (see d4_solver.cpp)
+    bool refactor(int n, const int* Ap, const int* Ai, const double* Ax);
+    bool have_reach_ = false;
+    std::vector<int> reach_p_, reach_i_;  // per column, its reach in topological order
+    std::vector<int> Lrow_;               // L row indices in original rows
+    std::vector<double> work_;
+    int64_t refactorizations_ = 0;
 };
 
 // minimum degree order of the symmetric pattern of A + A^T (exposed for tests)

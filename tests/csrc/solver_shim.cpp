@@ -35,3 +35,17 @@ extern "C" void ts_rcm_order(int n, const int* Ap, const int* Ai, int* ord)
     const std::vector<int> o = d4host::rcm_order(n, Ap, Ai);
     for (int i = 0; i < n; ++i) ord[i] = o[i];
 }
+
+// two factorizations with one solver (the same pattern, values Ax1 then Ax2),
+// then x = A2^{-1} b; *replayed: how many of them replayed the first's reach
+extern "C" int ts_refactor(int n, const int* Ap, const int* Ai, const double* Ax1, const double* Ax2, const double* b,
+                           double* x, long long* replayed)
+{
+    d4host::SparseLU lu;
+    if (lu.factor(n, Ap, Ai, Ax1) < 0) return -1;
+    const int rc = lu.factor(n, Ap, Ai, Ax2);
+    if (rc < 0) return rc;
+    lu.solve_const(b, x);
+    *replayed = lu.refactorizations();
+    return rc;
+}

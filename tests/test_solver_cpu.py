@@ -187,3 +187,48 @@ def test_rcm_order_is_a_permutation_per_component(ts):
     pos[o] = np.arange(9)
     for a, b in [(0, 3), (3, 5), (1, 2), (2, 7), (7, 8)]:
         assert abs(pos[a] - pos[b]) == 1
+
+
+def _refactor(ts, A1, A2, b):
+    ts.ts_refactor.argtypes = [C.c_int] + [C.c_void_p] * 7
+    Ap, Ai, Ax1 = _csc(A1)
+    Ap2, Ai2, Ax2 = _csc(A2)
+    assert np.array_equal(Ap, Ap2) and np.array_equal(Ai, Ai2)
+    x = np.empty(A1.shape[0])
+    rep = C.c_longlong(-1)
+    p = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
+    rc = ts.ts_refactor(A1.shape[0], p(Ap), p(Ai), p(Ax1), p(Ax2), p(b), p(x), C.byref(rep))
+    return rc, x, rep.value
+
+
+def test_refactorization_replays_the_reach_bit_for_bit(ts):
+    # a cutoff pattern factored twice: new values with the same pivot choices
+    # replay the first factorization's reach and give a fresh factorization's bits
+    rng = np.random.default_rng(21)
+    n = 400
+    P = _cutoff_pattern(n, rng, rc=0.08)
+    A1 = np.where(P, rng.uniform(-1, 1, P.shape), 0.0) * 0.05 + np.eye(n)
+    A2 = np.where(P, A1 * rng.uniform(0.9, 1.1, P.shape), 0.0)
+    b = rng.uniform(-1, 1, n)
+    rc, x, rep = _refactor(ts, A1, A2, b)
+    assert rc == 0 and rep == 1
+    rc_f, x_fresh, _ = _solve(ts, A2, b)
+    np.testing.assert_array_equal(x, x_fresh)
+    assert np.max(np.abs(A2 @ x - b)) <= 1e-13
+
+
+def test_refactorization_falls_back_when_a_pivot_changes(ts):
+    # the second matrix's diagonal is too small to keep: the pivot sequence
+    # changes, the replay stops and the full factorization runs
+    rng = np.random.default_rng(22)
+    n = 400
+    P = _cutoff_pattern(n, rng, rc=0.08)
+    A1 = np.where(P, rng.uniform(-1, 1, P.shape), 0.0) * 0.05 + np.eye(n)
+    A2 = A1.copy()
+    A2[np.diag_indices(n)] = 1e-9
+    b = rng.uniform(-1, 1, n)
+    rc, x, rep = _refactor(ts, A1, A2, b)
+    assert rep == 0
+    rc_f, x_fresh, _ = _solve(ts, A2, b)
+    assert rc == rc_f
+    np.testing.assert_array_equal(x, x_fresh)
