@@ -62,6 +62,28 @@ std::vector<int> min_degree_order(int n, const int* Ap, const int* Ai)
 
 namespace {
 
+// the entries in which two CSC patterns of order n differ (rows ascending in each column)
+int64_t pattern_distance(int n, const int* Ap, const int* Ai, const std::vector<int>& Bp, const std::vector<int>& Bi)
+{
+    int64_t d = 0;
+    for (int c = 0; c < n; ++c) {
+        int p = Ap[c], q = Bp[c];
+        const int pe = Ap[c + 1], qe = Bp[c + 1];
+        while (p < pe && q < qe) {
+            if (Ai[p] == Bi[q]) {
+                ++p;
+                ++q;
+            } else {
+                ++d;
+                if (Ai[p] < Bi[q]) ++p;
+                else ++q;
+            }
+        }
+        d += (pe - p) + (qe - q);
+    }
+    return d;
+}
+
 // breadth-first levels of s's component in the graph whose neighbours of v
 // are the rows of column v: the number of levels, and the nodes of the last
 // one; stamp[v] == tag marks the nodes reached
@@ -231,13 +253,21 @@ int SparseLU::factor(int n, const int* Ap, const int* Ai, const double* Ax)
     const int64_t nnz = Ap[n];
     if (n <= 96 || nnz > (int64_t)n * n / 4) return factor_dense(n, Ap, Ai, Ax);
 
-    // the column order depends on the pattern alone: reuse it while the pattern
-    // is the previous factorization's (the rescaled stage-II matrix always, and
-    // mostly the next Jacobian too, whose cutoff set rarely changes within a step)
+    // The pattern of the previous factorization again (the rescaled stage-II
+    // matrix, a retried step): replay its reach.  Otherwise the column order
+    // (any column order is valid) is kept while the pattern differs in at most
+    // 1 % of its entries from the one it was computed for -- a few pairs cross
+    // the cutoff between Jacobians -- and computed afresh beyond that.
     const bool same = n == pat_n_ && (int64_t)pat_i_.size() == nnz && std::equal(Ap, Ap + n + 1, pat_p_.begin()) &&
                       std::equal(Ai, Ai + nnz, pat_i_.begin());
     if (!same) {
-        q_ = min_degree_order(n, Ap, Ai);
+        if (!(n == ord_n_ && 100 * pattern_distance(n, Ap, Ai, ord_p_, ord_i_) <= nnz)) {
+            q_ = min_degree_order(n, Ap, Ai);
+            ++orderings_;
+            ord_n_ = n;
+            ord_p_.assign(Ap, Ap + n + 1);
+            ord_i_.assign(Ai, Ai + nnz);
+        }
         pat_n_ = n;
         pat_p_.assign(Ap, Ap + n + 1);
         pat_i_.assign(Ai, Ai + nnz);

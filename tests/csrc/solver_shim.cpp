@@ -49,3 +49,19 @@ extern "C" int ts_refactor(int n, const int* Ap, const int* Ai, const double* Ax
     *replayed = lu.refactorizations();
     return rc;
 }
+
+// two factorizations with one solver, of two matrices that may differ in
+// pattern, then x = A2^{-1} b; the counts of orders computed and of replays
+extern "C" int ts_two_patterns(int n, const int* Ap1, const int* Ai1, const double* Ax1, const int* Ap2,
+                               const int* Ai2, const double* Ax2, const double* b, double* x, long long* orderings,
+                               long long* replayed)
+{
+    d4host::SparseLU lu;
+    if (lu.factor(n, Ap1, Ai1, Ax1) < 0) return -1;
+    const int rc = lu.factor(n, Ap2, Ai2, Ax2);
+    if (rc < 0) return rc;
+    lu.solve_const(b, x);
+    *orderings = lu.orderings();
+    *replayed = lu.refactorizations();
+    return rc;
+}

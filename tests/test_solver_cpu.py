@@ -232,3 +232,29 @@ def test_refactorization_falls_back_when_a_pivot_changes(ts):
     rc_f, x_fresh, _ = _solve(ts, A2, b)
     assert rc == rc_f
     np.testing.assert_array_equal(x, x_fresh)
+
+
+@pytest.mark.parametrize("moved,orders", [(2, 1), (60, 2)])
+def test_column_order_kept_while_the_pattern_drifts_by_at_most_one_percent(ts, moved, orders):
+    # between two Jacobians a few pairs cross the cutoff: the order of the first
+    # pattern stays while the patterns differ in <= 1 % of the entries
+    ts.ts_two_patterns.argtypes = [C.c_int] + [C.c_void_p] * 8 + [C.POINTER(C.c_longlong)] * 2
+    rng = np.random.default_rng(23)
+    n = 500
+    P = _cutoff_pattern(n, rng, rc=0.08)
+    A1 = np.where(P, rng.uniform(-1, 1, P.shape), 0.0) * 0.05 + np.eye(n)
+    A2 = A1.copy()
+    off = np.argwhere(np.triu(P, 1))
+    for i, j in off[rng.choice(len(off), moved // 2, replace=False)]:  # symmetric pairs leave the pattern
+        A2[i, j] = A2[j, i] = 0.0
+    b = rng.uniform(-1, 1, n)
+    c1, c2 = _csc(A1), _csc(A2)
+    removed = len(c1[1]) - len(c2[1])
+    assert removed == moved and (100 * removed <= len(c1[1])) == (orders == 1)
+    x = np.empty(n)
+    nord, nrep = C.c_longlong(0), C.c_longlong(0)
+    p = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
+    rc = ts.ts_two_patterns(n, p(c1[0]), p(c1[1]), p(c1[2]), p(c2[0]), p(c2[1]), p(c2[2]), p(b), p(x),
+                            C.byref(nord), C.byref(nrep))
+    assert rc == 0 and nord.value == orders and nrep.value == 0
+    assert np.max(np.abs(x - np.linalg.solve(A2, b))) <= 1e-12
