@@ -155,7 +155,7 @@ def ensemble_bench(pkg, ctx, steps, R, seed=4242, plain=True, solver=1):
             "bitwise_equal_to_plain": equal}
 
 
-def implicit_step_bench(pkg, ctx, gpu_steps=50, cpu_steps=10, with_cpu=True, seed=77):
+def implicit_step_bench(pkg, ctx, gpu_steps=50, cpu_steps=10, with_cpu=True, seed=77, solvers=(0, 1)):
     """ms per implicit step (one loop iteration of Simulation::run, stages I-IV) of the plain
     drop-in on the N = 1024 workload: uniform positions, heaviside cutoff 0.05 (-u 1.6 -h),
     tau = 0.1 constant (-r 0 -s 0.1), default precision control.
@@ -199,6 +199,8 @@ def implicit_step_bench(pkg, ctx, gpu_steps=50, cpu_steps=10, with_cpu=True, see
     out = {"workload": "N=1024 uniform, heaviside cutoff 0.05, tau=0.1, adaptive (the C2 configuration)",
            "pairs": pairs, "pairs_inside_cutoff": inside, "algorithmic_ops_per_step": ops}
     for label, solver in (("gpu_passes_host_lu", 0), ("gpu_passes_gpu_lu", 1)):
+        if solver not in solvers:
+            continue
         dt, st = timed(lambda: simmod.Simulation.from_arrays(ctx, x, y, b, solver=solver, **params), gpu_steps)
         out[label] = {"ms_per_step": dt * 1e3, "steps": gpu_steps, "accepted": st["successful_steps"],
                       "failed": st["failed_steps"], "fraction_of_fp64_peak": ops / dt / 78.6e12,
@@ -278,6 +280,10 @@ def main():
                     help="with --ens-only: 1 = GPU dense LU (default), 0 = host sparse LU for every realization")
     ap.add_argument("--no-ens", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-step", action="store_true", help="skip the N=1024 implicit-step measurement")
+    ap.add_argument("--step-only", action="store_true",
+                    help="only the N=1024 implicit-step measurement, without its CPU leg (for profiling)")
+    ap.add_argument("--step-solver", type=int, default=-1, choices=(-1, 0, 1),
+                    help="with --step-only: 0 = host sparse LU, 1 = GPU dense LU, -1 = both")
     ap.add_argument("--ens-only", action="store_true",
                     help="only the batched ensemble (no plain comparison, no pair-pass headline), for profiling")
     ap.add_argument("--adjoint", type=str, default="",
@@ -306,6 +312,12 @@ def main():
         return
     pkg = importlib.import_module("2d4_amd")
     dev = torch.device("cuda:0")
+    if args.step_only:
+        sctx = pkg.Context(0)
+        solvers = (0, 1) if args.step_solver < 0 else (args.step_solver,)
+        print(json.dumps({"implicit_step": implicit_step_bench(pkg, sctx, with_cpu=False, solvers=solvers)}),
+              flush=True)
+        return
     if args.ens_only:
         ectx = pkg.Context(0)
         print(json.dumps({"ensemble": ensemble_bench(pkg, ectx, args.ens_steps, args.ens_r, plain=False, solver=args.ens_solver)}), flush=True)
