@@ -23,6 +23,10 @@ d4_ctx::~d4_ctx()
         ev_pool.push_back(m.second);
     }
     for (auto e : ev_pool) (void)hipEventDestroy(e);
+    if (pin) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipHostFree(pin);
+    }
     if (own_stream) (void)hipStreamDestroy(own_stream);
 }
 
@@ -171,7 +175,72 @@ int d4_ctx::single_pass(int mode, int n, const double* d_xy, const RealParams& r
 int d4_ctx::sync(const char* where)
 {
     hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        pending.clear();
+        pin_used = 0;
+        return hip_fail(e, where);
+    }
+    for (const Pending& p : pending) std::memcpy(p.dst, static_cast<char*>(pin) + p.off, p.bytes);
+    pending.clear();
+    pin_used = 0;
+    return D4_OK;
+}
+
+void d4_ctx::stage_begin()
+{
+    if (pending.empty() && pin_used == 0) return;
+    (void)hipStreamSynchronize(stream);  // a pass that failed midway: drop what it left
+    pending.clear();
+    pin_used = 0;
+}
+
+int d4_ctx::stage(size_t bytes, size_t* off, const char* where)
+{
+    const size_t at = (pin_used + 255) & ~(size_t)255;
+    if (at + bytes > pin_cap) {
+        // grow once everything using the arena is done (the copies pending so far complete here)
+        const int rc = sync(where);
+        if (rc) return rc;
+        const size_t cap = std::max<size_t>(2 * pin_cap, std::max<size_t>(bytes, 1 << 20));
+        if (pin) (void)hipHostFree(pin);
+        pin = nullptr;
+        pin_cap = 0;
+        hipError_t e = hipHostMalloc(&pin, cap, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            pin = nullptr;
+            return hip_fail(e, where);
+        }
+        pin_cap = cap;
+        *off = 0;
+        pin_used = bytes;
+        return D4_OK;
+    }
+    *off = at;
+    pin_used = at + bytes;
+    return D4_OK;
+}
+
+int d4_ctx::h2d_s(void* dst, const void* src, size_t bytes, const char* where)
+{
+    if (!bytes) return D4_OK;
+    size_t off = 0;
+    const int rc = stage(bytes, &off, where);
+    if (rc) return rc;
+    std::memcpy(static_cast<char*>(pin) + off, src, bytes);
+    hipError_t e = hipMemcpyAsync(dst, static_cast<char*>(pin) + off, bytes, hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return hip_fail(e, where);
+    return D4_OK;
+}
+
+int d4_ctx::d2h_s(void* dst, const void* src, size_t bytes, const char* where)
+{
+    if (!bytes) return D4_OK;
+    size_t off = 0;
+    const int rc = stage(bytes, &off, where);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(static_cast<char*>(pin) + off, src, bytes, hipMemcpyDeviceToHost, stream);
+    if (e != hipSuccess) return hip_fail(e, where);
+    pending.push_back({dst, off, bytes});
     return D4_OK;
 }
 
@@ -360,19 +429,20 @@ int d4_speed_pass(d4_ctx* ctx, int n, const double* xy, double tau, double* v, d
     if (!ctx || n < 0 || (n > 0 && (!xy || !v))) return D4_ERR_INVALID;
     if (n == 0) return D4_OK;
     const size_t b = sizeof(double) * (size_t)n;
+    ctx->stage_begin();
     D4_TRY(ctx->ensure(ctx->xy, 2 * b, "positions"));
     D4_TRY(ctx->ensure(ctx->v0, b, "speeds"));
     double* dtol = nullptr;
     if (tol && tol_cap2 > 0) {
         D4_TRY(ctx->ensure(ctx->tol, b, "tolerance"));
         dtol = static_cast<double*>(ctx->tol.p);
-        D4_TRY(ctx->h2d(dtol, tol, b, "tolerance upload"));
+        D4_TRY(ctx->h2d_s(dtol, tol, b, "tolerance upload"));
     }
-    D4_TRY(ctx->h2d(ctx->xy.p, xy, 2 * b, "position upload"));
+    D4_TRY(ctx->h2d_s(ctx->xy.p, xy, 2 * b, "position upload"));
     D4_TRY(d4_speed_pass_dev(ctx, n, static_cast<double*>(ctx->xy.p), tau, static_cast<double*>(ctx->v0.p), dtol,
                              tol_cap2));
-    D4_TRY(ctx->d2h(v, ctx->v0.p, b, "speed download"));
-    if (dtol) D4_TRY(ctx->d2h(tol, dtol, b, "tolerance download"));
+    D4_TRY(ctx->d2h_s(v, ctx->v0.p, b, "speed download"));
+    if (dtol) D4_TRY(ctx->d2h_s(tol, dtol, b, "tolerance download"));
     return ctx->sync("speed pass");
 }
 
@@ -435,6 +505,7 @@ int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* 
     *nnz = 0;
     if (n == 0) return D4_OK;
     const size_t b = sizeof(double) * (size_t)n;
+    ctx->stage_begin();
     D4_TRY(ctx->ensure(ctx->xy, 2 * b, "positions"));
     D4_TRY(ctx->ensure(ctx->v0, b, "speeds A"));
     D4_TRY(ctx->ensure(ctx->v1, b, "speeds B"));
@@ -450,37 +521,37 @@ int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* 
     if (tol && p->tol_cap2 > 0) {
         D4_TRY(ctx->ensure(ctx->tol, b, "tolerance"));
         dtol = static_cast<double*>(ctx->tol.p);
-        D4_TRY(ctx->h2d(dtol, tol, b, "tolerance upload"));
+        D4_TRY(ctx->h2d_s(dtol, tol, b, "tolerance upload"));
     }
-    D4_TRY(ctx->h2d(ctx->xy.p, xy, 2 * b, "position upload"));
+    D4_TRY(ctx->h2d_s(ctx->xy.p, xy, 2 * b, "position upload"));
     RealParams rp = real_params(n, p, 0.0);
     if (!dtol) rp.tol_cap2 = 0.0;
     int ncoo = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
         hipError_t e = hipMemsetAsync(ctx->coo_count.p, 0, sizeof(int), ctx->stream);
         if (e != hipSuccess) return ctx->hip_fail(e, "counter reset");
-        if (attempt == 1 && dtol) D4_TRY(ctx->h2d(dtol, tol, b, "tolerance upload"));
+        if (attempt == 1 && dtol) D4_TRY(ctx->h2d_s(dtol, tol, b, "tolerance upload"));
         D4_TRY(ctx->single_pass(kModeJacCoo, n, static_cast<double*>(ctx->xy.p), rp, dtol,
                                 static_cast<double*>(ctx->v0.p), static_cast<double*>(ctx->v1.p), nullptr, nullptr,
                                 nullptr, 0, nullptr, p->exact ? 1 : 0, 0, 1));
-        D4_TRY(ctx->d2h(&ncoo, ctx->coo_count.p, sizeof(int), "pair count"));
+        D4_TRY(ctx->d2h_s(&ncoo, ctx->coo_count.p, sizeof(int), "pair count"));
         D4_TRY(ctx->sync("jacobian pass"));
         if ((size_t)ncoo <= ctx->coo.bytes / sizeof(CooEntry)) break;
         if (attempt == 1) return ctx->fail(D4_ERR_HIP, "pair list overflow after resize");
         D4_TRY(ctx->ensure(ctx->coo, sizeof(CooEntry) * ((size_t)ncoo + 1024), "pair list"));
     }
     // CSC assembly: counts (diagonal + both copies), scan, fill, sort/finish, scale
-    std::vector<int> ones(n, 1);
-    D4_TRY(ctx->h2d(ctx->cnt.p, ones.data(), sizeof(int) * (size_t)n, "column count init"));
     int* dcnt = static_cast<int*>(ctx->cnt.p);
+    hipError_t e = launch_fill_int(dcnt, 1, n, ctx->stream);  // the diagonal of every column
+    if (e != hipSuccess) return ctx->hip_fail(e, "column count init");
     int* dAp = static_cast<int*>(ctx->Ap.p);
     const CooEntry* dcoo = static_cast<const CooEntry*>(ctx->coo.p);
-    hipError_t e = launch_csc_count(dcoo, ncoo, dcnt, n, ctx->stream);
+    e = launch_csc_count(dcoo, ncoo, dcnt, n, ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "csc count");
     e = launch_scan(dcnt, dAp, n, ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "csc scan");
     int total_nnz = 0;
-    D4_TRY(ctx->d2h(&total_nnz, dAp + n, sizeof(int), "nnz"));
+    D4_TRY(ctx->d2h_s(&total_nnz, dAp + n, sizeof(int), "nnz"));
     D4_TRY(ctx->sync("csc scan"));
     *nnz = total_nnz;
     if (total_nnz > cap || !Ai || !Ax) {
@@ -499,14 +570,14 @@ int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* 
     if (e != hipSuccess) return ctx->hip_fail(e, "csc finish");
     e = launch_csc_scale(dAp, dAi, dAx, didx, dd, n, ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "csc scale");
-    D4_TRY(ctx->d2h(vA, ctx->v0.p, b, "speed download"));
-    D4_TRY(ctx->d2h(vB, ctx->v1.p, b, "speed download"));
-    D4_TRY(ctx->d2h(Ap, dAp, sizeof(int) * ((size_t)n + 1), "Ap download"));
-    D4_TRY(ctx->d2h(Ai, dAi, sizeof(int) * (size_t)total_nnz, "Ai download"));
-    D4_TRY(ctx->d2h(Ax, dAx, sizeof(double) * (size_t)total_nnz, "Ax download"));
-    D4_TRY(ctx->d2h(indexes, didx, sizeof(int) * (size_t)n, "indexes download"));
-    D4_TRY(ctx->d2h(dvec, dd, b, "weights download"));
-    if (dtol) D4_TRY(ctx->d2h(tol, dtol, b, "tolerance download"));
+    D4_TRY(ctx->d2h_s(vA, ctx->v0.p, b, "speed download"));
+    D4_TRY(ctx->d2h_s(vB, ctx->v1.p, b, "speed download"));
+    D4_TRY(ctx->d2h_s(Ap, dAp, sizeof(int) * ((size_t)n + 1), "Ap download"));
+    D4_TRY(ctx->d2h_s(Ai, dAi, sizeof(int) * (size_t)total_nnz, "Ai download"));
+    D4_TRY(ctx->d2h_s(Ax, dAx, sizeof(double) * (size_t)total_nnz, "Ax download"));
+    D4_TRY(ctx->d2h_s(indexes, didx, sizeof(int) * (size_t)n, "indexes download"));
+    D4_TRY(ctx->d2h_s(dvec, dd, b, "weights download"));
+    if (dtol) D4_TRY(ctx->d2h_s(tol, dtol, b, "tolerance download"));
     return ctx->sync("jacobian pass");
 }
 
@@ -517,17 +588,18 @@ int d4_jacobian_rescale(d4_ctx* ctx, int n, const int* Ap, const int* Ai, double
     if (n == 0) return D4_OK;
     const int nnz = Ap[n];
     const size_t b = sizeof(double) * (size_t)n;
+    ctx->stage_begin();
     D4_TRY(ctx->ensure(ctx->Ap, sizeof(int) * ((size_t)n + 1), "Ap"));
     D4_TRY(ctx->ensure(ctx->Ai, sizeof(int) * (size_t)std::max(nnz, 1), "Ai"));
     D4_TRY(ctx->ensure(ctx->Ax, sizeof(double) * (size_t)std::max(nnz, 1), "Ax"));
     D4_TRY(ctx->ensure(ctx->dvec, b, "weights"));
     D4_TRY(ctx->ensure(ctx->nw, b, "new weights"));
     D4_TRY(ctx->ensure(ctx->v0, b, "speeds"));
-    D4_TRY(ctx->h2d(ctx->Ap.p, Ap, sizeof(int) * ((size_t)n + 1), "Ap upload"));
-    D4_TRY(ctx->h2d(ctx->Ai.p, Ai, sizeof(int) * (size_t)nnz, "Ai upload"));
-    D4_TRY(ctx->h2d(ctx->Ax.p, Ax, sizeof(double) * (size_t)nnz, "Ax upload"));
-    D4_TRY(ctx->h2d(ctx->dvec.p, dvec, b, "weights upload"));
-    if (speed) D4_TRY(ctx->h2d(ctx->v0.p, speed, b, "speed upload"));
+    D4_TRY(ctx->h2d_s(ctx->Ap.p, Ap, sizeof(int) * ((size_t)n + 1), "Ap upload"));
+    D4_TRY(ctx->h2d_s(ctx->Ai.p, Ai, sizeof(int) * (size_t)nnz, "Ai upload"));
+    D4_TRY(ctx->h2d_s(ctx->Ax.p, Ax, sizeof(double) * (size_t)nnz, "Ax upload"));
+    D4_TRY(ctx->h2d_s(ctx->dvec.p, dvec, b, "weights upload"));
+    if (speed) D4_TRY(ctx->h2d_s(ctx->v0.p, speed, b, "speed upload"));
     RescaleArgs a{};
     a.Ap = static_cast<const int*>(ctx->Ap.p);
     a.Ai = static_cast<const int*>(ctx->Ai.p);
@@ -541,9 +613,9 @@ int d4_jacobian_rescale(d4_ctx* ctx, int n, const int* Ap, const int* Ai, double
     a.total = n;
     hipError_t e = launch_rescale(a, ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "rescale launch");
-    D4_TRY(ctx->d2h(Ax, ctx->Ax.p, sizeof(double) * (size_t)nnz, "Ax download"));
-    D4_TRY(ctx->d2h(dvec, ctx->dvec.p, b, "weights download"));
-    if (speed) D4_TRY(ctx->d2h(speed, ctx->v0.p, b, "speed download"));
+    D4_TRY(ctx->d2h_s(Ax, ctx->Ax.p, sizeof(double) * (size_t)nnz, "Ax download"));
+    D4_TRY(ctx->d2h_s(dvec, ctx->dvec.p, b, "weights download"));
+    if (speed) D4_TRY(ctx->d2h_s(speed, ctx->v0.p, b, "speed download"));
     return ctx->sync("rescale");
 }
 

@@ -89,4 +89,20 @@ struct d4_ctx {
     int sync(const char* where);
     int h2d(void* dst, const void* src, size_t bytes, const char* where);
     int d2h(void* dst, const void* src, size_t bytes, const char* where);
+
+    // Pinned staging for the single-simulation passes, whose host arrays are
+    // ordinary (pageable) memory: an upload is copied into the arena and sent
+    // asynchronously; a download lands in the arena and is copied to the
+    // caller's array when sync() returns.  A pass calls stage_begin() first.
+    struct Pending {
+        void* dst;
+        size_t off, bytes;
+    };
+    void* pin = nullptr;
+    size_t pin_cap = 0, pin_used = 0;
+    std::vector<Pending> pending;
+    void stage_begin();
+    int stage(size_t bytes, size_t* off, const char* where);
+    int h2d_s(void* dst, const void* src, size_t bytes, const char* where);
+    int d2h_s(void* dst, const void* src, size_t bytes, const char* where);
 };

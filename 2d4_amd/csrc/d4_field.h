@@ -444,7 +444,7 @@ D4_HD double weight_inv(double w, char kind)
 // column kx; n a power of two).  dx, dy are wrapped differences; the same
 // operations in the same order on host and device, without contraction, so a
 // host build of this function gives bit-identical values.
-D4_HD inline double table_bilinear(const double* tab, int n, double dx, double dy)
+D4_HD double table_bilinear(const double* tab, int n, double dx, double dy)
 {
 #pragma clang fp contract(off)
     const double u = (dx + 0.5) * n;
