@@ -514,71 +514,77 @@ int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* 
     D4_TRY(ctx->ensure(ctx->cnt, sizeof(int) * (size_t)n, "column counts"));
     D4_TRY(ctx->ensure(ctx->cursor, sizeof(int) * (size_t)n, "column cursors"));
     D4_TRY(ctx->ensure(ctx->Ap, sizeof(int) * ((size_t)n + 1), "Ap"));
-    // COO capacity: start from the caller's CSC capacity, grow on overflow
-    size_t want = std::max<size_t>((size_t)std::max(cap, n) / 2 + 1024, ctx->coo.bytes / sizeof(CooEntry));
-    D4_TRY(ctx->ensure(ctx->coo, sizeof(CooEntry) * want, "pair list"));
+    // Pair-list capacity: from the caller's CSC capacity, grown on overflow.  The
+    // device CSC holds n + 2 * capacity entries (each listed pair gives at most
+    // two, beside the n diagonals), so every write of the assembly stays inside
+    // it whatever the pass finds; the one synchronization at the end tells
+    // whether the list overflowed (pass run again) or the caller's CSC is too
+    // small (D4_ERR_CAPACITY with *nnz set, as before).
+    size_t coo_cap = std::max<size_t>((size_t)std::max(cap, n) / 2 + 1024, ctx->coo.bytes / sizeof(CooEntry));
     double* dtol = nullptr;
     if (tol && p->tol_cap2 > 0) {
         D4_TRY(ctx->ensure(ctx->tol, b, "tolerance"));
         dtol = static_cast<double*>(ctx->tol.p);
-        D4_TRY(ctx->h2d_s(dtol, tol, b, "tolerance upload"));
     }
     D4_TRY(ctx->h2d_s(ctx->xy.p, xy, 2 * b, "position upload"));
     RealParams rp = real_params(n, p, 0.0);
     if (!dtol) rp.tol_cap2 = 0.0;
-    int ncoo = 0;
+    int* dcnt = static_cast<int*>(ctx->cnt.p);
+    int* dAp = static_cast<int*>(ctx->Ap.p);
+    int* didx = static_cast<int*>(ctx->idx.p);
+    double* dd = static_cast<double*>(ctx->dvec.p);
+    const int* dcount = static_cast<const int*>(ctx->coo_count.p);
+    const size_t down = (Ai && Ax && cap > 0) ? (size_t)cap : 0;  // CSC entries returned to the caller
+    int ncoo = 0, total_nnz = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
+        D4_TRY(ctx->ensure(ctx->coo, sizeof(CooEntry) * coo_cap, "pair list"));
+        const size_t csc_cap = (size_t)n + 2 * coo_cap;
+        D4_TRY(ctx->ensure(ctx->Ai, sizeof(int) * csc_cap, "Ai"));
+        D4_TRY(ctx->ensure(ctx->Ax, sizeof(double) * csc_cap, "Ax"));
+        int* dAi = static_cast<int*>(ctx->Ai.p);
+        double* dAx = static_cast<double*>(ctx->Ax.p);
+        const CooEntry* dcoo = static_cast<const CooEntry*>(ctx->coo.p);
+        // the pass updates the tolerance in place; on a second attempt the caller's array already holds
+        // the first attempt's result, and the update (a minimum over the same pairs) leaves it unchanged
+        if (dtol) D4_TRY(ctx->h2d_s(dtol, tol, b, "tolerance upload"));
         hipError_t e = hipMemsetAsync(ctx->coo_count.p, 0, sizeof(int), ctx->stream);
         if (e != hipSuccess) return ctx->hip_fail(e, "counter reset");
-        if (attempt == 1 && dtol) D4_TRY(ctx->h2d_s(dtol, tol, b, "tolerance upload"));
         D4_TRY(ctx->single_pass(kModeJacCoo, n, static_cast<double*>(ctx->xy.p), rp, dtol,
                                 static_cast<double*>(ctx->v0.p), static_cast<double*>(ctx->v1.p), nullptr, nullptr,
                                 nullptr, 0, nullptr, p->exact ? 1 : 0, 0, 1));
-        D4_TRY(ctx->d2h_s(&ncoo, ctx->coo_count.p, sizeof(int), "pair count"));
+        // CSC assembly: counts (diagonal + both copies), scan, fill, sort/finish, scale
+        e = launch_fill_int(dcnt, 1, n, ctx->stream);  // the diagonal of every column
+        if (e == hipSuccess) e = launch_csc_count(dcoo, (int)coo_cap, dcnt, n, ctx->stream, dcount);
+        if (e == hipSuccess) e = launch_scan(dcnt, dAp, n, ctx->stream);
+        if (e == hipSuccess)
+            e = launch_csc_fill(dcoo, (int)coo_cap, dAp, static_cast<int*>(ctx->cursor.p), dAi, dAx, n, ctx->stream,
+                                dcount);
+        if (e == hipSuccess) e = launch_csc_finish(dAp, dAi, dAx, didx, dd, nullptr, nullptr, p->wfunc, n, ctx->stream);
+        if (e == hipSuccess) e = launch_csc_scale(dAp, dAi, dAx, didx, dd, n, ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "csc assembly");
+        D4_TRY(ctx->d2h_s(&ncoo, dcount, sizeof(int), "pair count"));
+        D4_TRY(ctx->d2h_s(&total_nnz, dAp + n, sizeof(int), "nnz"));
+        D4_TRY(ctx->d2h_s(vA, ctx->v0.p, b, "speed download"));
+        D4_TRY(ctx->d2h_s(vB, ctx->v1.p, b, "speed download"));
+        D4_TRY(ctx->d2h_s(Ap, dAp, sizeof(int) * ((size_t)n + 1), "Ap download"));
+        if (down) {
+            D4_TRY(ctx->d2h_s(Ai, dAi, sizeof(int) * down, "Ai download"));
+            D4_TRY(ctx->d2h_s(Ax, dAx, sizeof(double) * down, "Ax download"));
+        }
+        D4_TRY(ctx->d2h_s(indexes, didx, sizeof(int) * (size_t)n, "indexes download"));
+        D4_TRY(ctx->d2h_s(dvec, dd, b, "weights download"));
+        if (dtol) D4_TRY(ctx->d2h_s(tol, dtol, b, "tolerance download"));
         D4_TRY(ctx->sync("jacobian pass"));
-        if ((size_t)ncoo <= ctx->coo.bytes / sizeof(CooEntry)) break;
+        if ((size_t)ncoo <= coo_cap) break;
         if (attempt == 1) return ctx->fail(D4_ERR_HIP, "pair list overflow after resize");
-        D4_TRY(ctx->ensure(ctx->coo, sizeof(CooEntry) * ((size_t)ncoo + 1024), "pair list"));
+        coo_cap = (size_t)ncoo + 1024;
     }
-    // CSC assembly: counts (diagonal + both copies), scan, fill, sort/finish, scale
-    int* dcnt = static_cast<int*>(ctx->cnt.p);
-    hipError_t e = launch_fill_int(dcnt, 1, n, ctx->stream);  // the diagonal of every column
-    if (e != hipSuccess) return ctx->hip_fail(e, "column count init");
-    int* dAp = static_cast<int*>(ctx->Ap.p);
-    const CooEntry* dcoo = static_cast<const CooEntry*>(ctx->coo.p);
-    e = launch_csc_count(dcoo, ncoo, dcnt, n, ctx->stream);
-    if (e != hipSuccess) return ctx->hip_fail(e, "csc count");
-    e = launch_scan(dcnt, dAp, n, ctx->stream);
-    if (e != hipSuccess) return ctx->hip_fail(e, "csc scan");
-    int total_nnz = 0;
-    D4_TRY(ctx->d2h_s(&total_nnz, dAp + n, sizeof(int), "nnz"));
-    D4_TRY(ctx->sync("csc scan"));
     *nnz = total_nnz;
     if (total_nnz > cap || !Ai || !Ax) {
         ctx->err = "CSC capacity too small";
         return D4_ERR_CAPACITY;
     }
-    D4_TRY(ctx->ensure(ctx->Ai, sizeof(int) * (size_t)total_nnz, "Ai"));
-    D4_TRY(ctx->ensure(ctx->Ax, sizeof(double) * (size_t)total_nnz, "Ax"));
-    int* dAi = static_cast<int*>(ctx->Ai.p);
-    double* dAx = static_cast<double*>(ctx->Ax.p);
-    int* didx = static_cast<int*>(ctx->idx.p);
-    double* dd = static_cast<double*>(ctx->dvec.p);
-    e = launch_csc_fill(dcoo, ncoo, dAp, static_cast<int*>(ctx->cursor.p), dAi, dAx, n, ctx->stream);
-    if (e != hipSuccess) return ctx->hip_fail(e, "csc fill");
-    e = launch_csc_finish(dAp, dAi, dAx, didx, dd, nullptr, nullptr, p->wfunc, n, ctx->stream);
-    if (e != hipSuccess) return ctx->hip_fail(e, "csc finish");
-    e = launch_csc_scale(dAp, dAi, dAx, didx, dd, n, ctx->stream);
-    if (e != hipSuccess) return ctx->hip_fail(e, "csc scale");
-    D4_TRY(ctx->d2h_s(vA, ctx->v0.p, b, "speed download"));
-    D4_TRY(ctx->d2h_s(vB, ctx->v1.p, b, "speed download"));
-    D4_TRY(ctx->d2h_s(Ap, dAp, sizeof(int) * ((size_t)n + 1), "Ap download"));
-    D4_TRY(ctx->d2h_s(Ai, dAi, sizeof(int) * (size_t)total_nnz, "Ai download"));
-    D4_TRY(ctx->d2h_s(Ax, dAx, sizeof(double) * (size_t)total_nnz, "Ax download"));
-    D4_TRY(ctx->d2h_s(indexes, didx, sizeof(int) * (size_t)n, "indexes download"));
-    D4_TRY(ctx->d2h_s(dvec, dd, b, "weights download"));
-    if (dtol) D4_TRY(ctx->d2h_s(tol, dtol, b, "tolerance download"));
-    return ctx->sync("jacobian pass");
+    return D4_OK;
 }
 
 int d4_jacobian_rescale(d4_ctx* ctx, int n, const int* Ap, const int* Ai, double* Ax, double* dvec, char wfunc,
