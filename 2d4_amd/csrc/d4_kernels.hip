@@ -17,6 +17,7 @@
 #include "d4_kernels.h"
 
 #include <algorithm>
+#include <climits>
 
 namespace d4 {
 
@@ -538,47 +539,77 @@ hipError_t launch_fill_int(int* p, int v, int n, hipStream_t s)
 // Per column: sort rows ascending (the order getElement + the j > i loop
 // produce, simulation.cpp:362-374, 444-478), then the diagonal
 // s = -sum of the column in that order (simulation.cpp:370, 475, 512) and
-// d = weight(s) (simulation.cpp:513).
-__global__ void csc_finish_kernel(const int* Ap, int* Ai, double* Ax, int* indexes, double* dvec,
-                                  const int* elem_real, const int* wf_arr, int wf0, int total)
+// d = weight(s) (simulation.cpp:513).  One wavefront per column: for up to 64
+// entries lane k holds entry k, counts the rows below its own (the rows of a
+// column are distinct) and writes the entry to that rank; the sum then runs
+// over the sorted entries in ascending order, the same for every lane.
+// Longer columns are sorted by lane 0 in place (shell sort).
+__device__ inline int csc_lane_int(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+__global__ __launch_bounds__(256) void csc_finish_kernel(const int* Ap, int* Ai, double* Ax, int* indexes,
+                                                         double* dvec, const int* elem_real, const int* wf_arr,
+                                                         int wf0, int total)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= total) return;
+    const int c = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (c >= total) return;  // uniform over the wavefront
     const int lo = Ap[c], hi = Ap[c + 1], len = hi - lo;
-    // shell sort with a fixed gap sequence ending in an insertion pass
-    const int gaps[8] = {701, 301, 132, 57, 23, 10, 4, 1};
-    for (int gi = 0; gi < 8; ++gi) {
-        const int gap = gaps[gi];
-        if (gap >= len) continue;
-        for (int k = lo + gap; k < hi; ++k) {
-            const int key = Ai[k];
-            const double val = Ax[k];
-            int m = k;
-            while (m - gap >= lo && Ai[m - gap] > key) {
-                Ai[m] = Ai[m - gap];
-                Ax[m] = Ax[m - gap];
-                m -= gap;
-            }
-            Ai[m] = key;
-            Ax[m] = val;
-        }
-    }
     double sub = 0.0;
     int dpos = lo;
-    for (int k = lo; k < hi; ++k) {
-        if (Ai[k] == c) dpos = k;
-        else sub -= Ax[k];
+    if (len <= 64) {
+        const int key = lane < len ? Ai[lo + lane] : INT_MAX;
+        const double val = lane < len ? Ax[lo + lane] : 0.0;
+        int rank = 0;
+        for (int j = 0; j < len; ++j) rank += csc_lane_int(key, j) < key ? 1 : 0;
+        // the entries in sorted order, lane r holding rank r
+        const int to = 4 * min(rank, 63);
+        const int sk = __builtin_amdgcn_ds_permute(to, key);
+        const int slo = __builtin_amdgcn_ds_permute(to, __double2loint(val));
+        const int shi = __builtin_amdgcn_ds_permute(to, __double2hiint(val));
+        for (int r = 0; r < len; ++r) {
+            if (csc_lane_int(sk, r) == c) dpos = lo + r;
+            else sub -= __hiloint2double(csc_lane_int(shi, r), csc_lane_int(slo, r));
+        }
+        if (lane < len) {
+            Ai[lo + rank] = key;
+            Ax[lo + rank] = key == c ? sub : val;
+        }
+    } else if (lane == 0) {
+        // shell sort with a fixed gap sequence ending in an insertion pass
+        const int gaps[8] = {701, 301, 132, 57, 23, 10, 4, 1};
+        for (int gi = 0; gi < 8; ++gi) {
+            const int gap = gaps[gi];
+            if (gap >= len) continue;
+            for (int k = lo + gap; k < hi; ++k) {
+                const int key = Ai[k];
+                const double val = Ax[k];
+                int m = k;
+                while (m - gap >= lo && Ai[m - gap] > key) {
+                    Ai[m] = Ai[m - gap];
+                    Ax[m] = Ax[m - gap];
+                    m -= gap;
+                }
+                Ai[m] = key;
+                Ax[m] = val;
+            }
+        }
+        for (int k = lo; k < hi; ++k) {
+            if (Ai[k] == c) dpos = k;
+            else sub -= Ax[k];
+        }
+        Ax[dpos] = sub;
     }
-    Ax[dpos] = sub;
-    indexes[c] = dpos;
-    const int wf = wf_arr ? wf_arr[elem_real ? elem_real[c] : 0] : wf0;
-    dvec[c] = weight(sub, (char)wf);
+    if (lane == 0) {
+        indexes[c] = dpos;
+        const int wf = wf_arr ? wf_arr[elem_real ? elem_real[c] : 0] : wf0;
+        dvec[c] = weight(sub, (char)wf);
+    }
 }
 
 hipError_t launch_csc_finish(const int* Ap, int* Ai, double* Ax, int* indexes, double* dvec, const int* elem_real,
                              const int* wf_arr, int wf0, int total, hipStream_t s)
 {
-    hipLaunchKernelGGL(csc_finish_kernel, dim3((total + 255) / 256), dim3(256), 0, s, Ap, Ai, Ax, indexes, dvec,
+    hipLaunchKernelGGL(csc_finish_kernel, dim3((total + 3) / 4), dim3(256), 0, s, Ap, Ai, Ax, indexes, dvec,
                        elem_real, wf_arr, wf0, total);
     return hipGetLastError();
 }
