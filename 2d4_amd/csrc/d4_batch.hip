@@ -388,7 +388,7 @@ struct BatchPasses {
         const CooEntry* coo = static_cast<const CooEntry*>(coo_d.p);
         if (e == hipSuccess) e = launch_fill_int(cnt, 1, total, s);
         if (e == hipSuccess) e = launch_csc_count(coo, (int)coo_cap, cnt, total, s, ncoo);
-        if (e == hipSuccess) e = launch_scan(cnt, Ap, total, s);
+        if (e == hipSuccess) e = launch_scan(cnt, Ap, total, s, static_cast<int*>(cursor_d.p));  // cursors: free until the fill
         if (e == hipSuccess)
             e = launch_csc_fill(coo, (int)coo_cap, Ap, static_cast<int*>(cursor_d.p), Ai, Ax, total, s, ncoo);
         if (e == hipSuccess)

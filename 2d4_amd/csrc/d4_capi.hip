@@ -555,7 +555,7 @@ int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* 
         // CSC assembly: counts (diagonal + both copies), scan, fill, sort/finish, scale
         e = launch_fill_int(dcnt, 1, n, ctx->stream);  // the diagonal of every column
         if (e == hipSuccess) e = launch_csc_count(dcoo, (int)coo_cap, dcnt, n, ctx->stream, dcount);
-        if (e == hipSuccess) e = launch_scan(dcnt, dAp, n, ctx->stream);
+        if (e == hipSuccess) e = launch_scan(dcnt, dAp, n, ctx->stream, static_cast<int*>(ctx->cursor.p));
         if (e == hipSuccess)
             e = launch_csc_fill(dcoo, (int)coo_cap, dAp, static_cast<int*>(ctx->cursor.p), dAi, dAx, n, ctx->stream,
                                 dcount);

@@ -129,7 +129,9 @@ hipError_t launch_reduce(const ReduceArgs& a, hipStream_t s);
 // ncoo_dev (optional): device-side pair count, capped by ncoo (the grid is sized by ncoo)
 hipError_t launch_csc_count(const CooEntry* coo, int ncoo, int* cnt, int total, hipStream_t s,
                             const int* ncoo_dev = nullptr);
-hipError_t launch_scan(const int* cnt, int* Ap, int total, hipStream_t s);  // Ap[0..total]
+// Ap[0..total] = exclusive scan of cnt; scratch (2 * ceil(total / 2048) + 1 ints, or null: one
+// workgroup) lets a long scan spread over many workgroups
+hipError_t launch_scan(const int* cnt, int* Ap, int total, hipStream_t s, int* scratch);
 hipError_t launch_csc_fill(const CooEntry* coo, int ncoo, const int* Ap, int* cursor, int* Ai, double* Ax,
                            int total, hipStream_t s, const int* ncoo_dev = nullptr);
 // p[0..n) = v (a kernel, so it can sit in a captured graph)

@@ -481,9 +481,69 @@ __global__ __launch_bounds__(1024) void scan_kernel(const int* cnt, int* Ap, int
     }
 }
 
-hipError_t launch_scan(const int* cnt, int* Ap, int total, hipStream_t s)
+// Longer scans in three launches: per block of kScanBlock counts its total,
+// the exclusive scan of the totals (one workgroup), then each block's own
+// exclusive scan offset by its total's prefix.  Integer sums: any grouping
+// gives the same Ap.
+constexpr int kScanBlock = 2048;  // 256 threads x 8 counts
+
+__device__ inline int block_exclusive_scan(int v, int* part)  // 256 threads; returns the sum of the values before v
 {
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, Ap, total);
+    const int t = threadIdx.x;
+    part[t] = v;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const int u = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += u;
+        __syncthreads();
+    }
+    return part[t] - v;
+}
+
+__global__ __launch_bounds__(256) void scan_totals_kernel(const int* cnt, int total, int* bsum)
+{
+    __shared__ int part[256];
+    const int k0 = blockIdx.x * kScanBlock + 8 * (int)threadIdx.x;
+    int s = 0;
+    for (int k = k0; k < min(k0 + 8, total); ++k) s += cnt[k];
+    block_exclusive_scan(s, part);
+    if (threadIdx.x == 255) bsum[blockIdx.x] = part[255];
+}
+
+__global__ __launch_bounds__(256) void scan_apply_kernel(const int* cnt, int total, const int* boff, int* Ap)
+{
+    __shared__ int part[256];
+    const int k0 = blockIdx.x * kScanBlock + 8 * (int)threadIdx.x;
+    int v[8], s = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        v[q] = k0 + q < total ? cnt[k0 + q] : 0;
+        s += v[q];
+    }
+    int run = boff[blockIdx.x] + block_exclusive_scan(s, part);
+    if (blockIdx.x == 0 && threadIdx.x == 0) Ap[0] = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+        if (k0 + q < total) {
+            run += v[q];
+            Ap[k0 + q + 1] = run;
+        }
+}
+
+hipError_t launch_scan(const int* cnt, int* Ap, int total, hipStream_t s, int* scratch)
+{
+    const int nblk = (total + kScanBlock - 1) / kScanBlock;
+    if (total <= 16 * kScanBlock || !scratch) {
+        hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, cnt, Ap, total);
+        return hipGetLastError();
+    }
+    // scratch: nblk totals, then their nblk + 1 exclusive prefix sums
+    int* bsum = scratch;
+    int* boff = scratch + nblk;
+    hipLaunchKernelGGL(scan_totals_kernel, dim3(nblk), dim3(256), 0, s, cnt, total, bsum);
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, s, bsum, boff, nblk);
+    hipLaunchKernelGGL(scan_apply_kernel, dim3(nblk), dim3(256), 0, s, cnt, total, boff, Ap);
     return hipGetLastError();
 }
 

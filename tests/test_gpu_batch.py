@@ -190,3 +190,17 @@ def test_ensemble_step_limit_stops_realizations(ctx):
     assert e.step(5) == 0
     st = e.stats()
     assert [s["successful_steps"] for s in st] == [2, 4]
+
+
+def test_ensemble_over_32768_columns_equals_plain_path(ctx):
+    # more than 16 x 2048 columns in one batched pass: the CSC scan of the
+    # batched Jacobian spreads over many workgroups (the plain path's is one)
+    sizes = [4096, 4000, 3500, 4096, 3800, 4096, 3900, 4096, 2600]
+    assert sum(sizes) > 32768
+    off, x, y, b = _ens_inputs(sizes, 31)
+    params = [dict(cutoff_mult=0.05 * np.sqrt(n), heaviside=True, stress=("fixed", 0.05, 0.0), solver=1)
+              for n in sizes]
+    e = ens.Ensemble(ctx, off, x, y, b, params)
+    e.step(2)
+    _check_equal(e, off, _plain(ctx, off, x, y, b, params, 2))
+    e.close()
