@@ -175,6 +175,7 @@ struct BatchPasses {
     HostBuf xy_h, rp_h, wf_h, tolin_h, tol_h, vA_h, vB_h, Ap_h, Ai_h, Ax_h, idx_h, dvec_h, ncoo_h, fac_h, dtau_h;
     size_t coo_cap = 0;
     hipGraphExec_t g_speed[2] = {nullptr, nullptr}, g_jac[2] = {nullptr, nullptr};  // [with tolerance]
+    uint64_t csc_gen = 0;  // counts the passes that rewrote the device CSC (Ap_d, Ai_d, Ax_d)
 
     void drop_graphs()
     {
@@ -459,6 +460,7 @@ struct BatchPasses {
     // Jacobian pass -> vA_h, vB_h, Ap_h, idx_h, dvec_h, tol_h and (fetch_entries) Ai_h, Ax_h
     int jacobian(bool with_tol, bool fetch_entries, int* nnz_out)
     {
+        ++csc_gen;
         finish_params();
         if (coo_cap == 0) D4_TRY(ensure_coo(std::max<size_t>((size_t)total * 8, 1 << 16)));
         for (int attempt = 0; attempt < 2; ++attempt) {
@@ -485,6 +487,7 @@ struct BatchPasses {
     // dvec_h, speeds vA_h, factors fac_h, stress steps dtau_h, weight kinds wf_h
     int rescale(int nnz)
     {
+        ++csc_gen;
         const size_t T = total;
         if (!Ai_d.p || Ai_d.bytes < sizeof(int) * (size_t)nnz) {
             D4_TRY(ctx->ensure(Ai_d, sizeof(int) * (size_t)std::max(nnz, 1), "batch Ai"));
@@ -543,6 +546,10 @@ struct d4_ens {
     std::unique_ptr<BatchPasses> bp;
     std::unique_ptr<ThreadPool> pool;
     std::unique_ptr<d4host::BatchDenseLU> blu;  // the solver-1 realizations' LUs, batched
+    // the realizations (in pass order) and column offsets of the last batched pass that left their CSC on
+    // the device, and that pass's generation: still there while bp->csc_gen has not moved on
+    uint64_t csc_gen = 0;
+    std::vector<int> csc_act, csc_off;
     d4_ens_config cfg{0, 0, 0, 0};
     bool started = false;
     std::string err;
@@ -595,9 +602,24 @@ int run_linear(d4_ens* e, const std::vector<int>& act, PassJob::Kind kind)
         const int K = (int)gpu.size();
         if (kind == PassJob::Factor) {
             std::vector<d4host::BatchDenseLU::Matrix> m(K);
+            // the matrices are still on the device when this group's Jacobian or rescale pass was the last to
+            // write the device CSC (it precedes the Factor job): the solver reads them there
+            const bool on_dev = e->bp && e->csc_gen == e->bp->csc_gen && e->csc_act == act;
+            std::vector<int> pos;
+            if (on_dev) {
+                pos.assign(e->data.size(), -1);
+                for (int k = 0; k < (int)act.size(); ++k) pos[act[k]] = k;
+            }
             for (int k = 0; k < K; ++k) {
                 const sdddstCore::SimulationData& d = *e->data[gpu[k]];
                 m[k] = {(int)d.dc, d.Ap.data(), d.Ai.data(), d.Ax.data()};
+                if (on_dev) {
+                    const int q = pos[gpu[k]];
+                    m[k].dAp = static_cast<const int*>(e->bp->Ap_d.p) + e->csc_off[q];
+                    m[k].dAi = static_cast<const int*>(e->bp->Ai_d.p);
+                    m[k].dAx = static_cast<const double*>(e->bp->Ax_d.p);
+                    m[k].row_base = e->csc_off[q];
+                }
             }
             std::vector<unsigned char> sing(K, 0), unpiv(K, 0);
             if (e->blu->factor(K, m.data(), sing.data(), unpiv.data()) < 0)
@@ -668,6 +690,9 @@ int run_batched(d4_ens* e, const std::vector<int>& act)
         } else {
             int nnz = 0;
             rc = bp.jacobian(with_tol, true, &nnz);
+            e->csc_gen = bp.csc_gen;
+            e->csc_act = act;
+            e->csc_off = off;
         }
         if (rc) return rc;
         const int* Ap = bp.Ap_h.as<int>();
@@ -727,6 +752,9 @@ int run_batched(d4_ens* e, const std::vector<int>& act)
         });
         rc = bp.rescale(nnz);
         if (rc) return rc;
+        e->csc_gen = bp.csc_gen;
+        e->csc_act = act;
+        e->csc_off = off;
         e->pool->parallel_for(R, [&](int k) {
             Simulation& s = *e->sims[act[k]];
             PassJob& j = s.job();

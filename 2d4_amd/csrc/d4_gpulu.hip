@@ -142,9 +142,12 @@ struct PinBuf {
 struct MatDesc {
     long long a_off;  // first entry of the matrix in the dense store
     int n, npad;
-    int col_off;   // its n + 1 column pointers in the packed CSC
     int ipiv_off;  // its npad pivots (matrices refactored with pivoting)
     int ord_off, env_off;
+    int row_base;  // rows stored as r + row_base in ai
+    const int* ap;  // its CSC on the device: column c at ai/ax[ap[c] .. ap[c + 1])
+    const int* ai;
+    const double* ax;
 };
 
 struct SolveDesc {
@@ -182,16 +185,17 @@ __global__ void fill_identity_kernel(const MatDesc* __restrict__ desc, double* _
 
 // the CSC entries of column c (one thread) into the dense matrix, rows and
 // columns renumbered by inv when the matrix is reordered
-__global__ void scatter_kernel(const MatDesc* __restrict__ desc, const int* __restrict__ colp,
-                               const int* __restrict__ rowi, const double* __restrict__ vals, double* __restrict__ A,
-                               const int* __restrict__ aux)
+__global__ void scatter_kernel(const MatDesc* __restrict__ desc, double* __restrict__ A, const int* __restrict__ aux)
 {
     const MatDesc m = desc[blockIdx.y];
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.n) return;
     const int* inv = m.ord_off >= 0 ? aux + m.ord_off + m.n : nullptr;
     double* col = A + m.a_off + (size_t)(inv ? inv[c] : c) * m.npad;
-    for (int q = colp[m.col_off + c]; q < colp[m.col_off + c + 1]; ++q) col[inv ? inv[rowi[q]] : rowi[q]] = vals[q];
+    for (int q = m.ap[c]; q < m.ap[c + 1]; ++q) {
+        const int r = m.ai[q] - m.row_base;
+        col[inv ? inv[r] : r] = m.ax[q];
+    }
 }
 
 // one dense matrix already on the device into the padded store: element (r, c)
@@ -655,7 +659,8 @@ public:
         if (K > kMaxMatrices) return -1;
         const bool dense = dense_.a != nullptr;
         for (int k = 0; k < K; ++k)
-            if (m[k].n < 0 || (!dense && m[k].n > 0 && (!m[k].Ap || !m[k].Ai || !m[k].Ax)) || m[k].n > kMaxOrder)
+            if (m[k].n < 0 || (!dense && m[k].n > 0 && (!m[k].Ap || !m[k].Ai || !m[k].Ax)) || m[k].n > kMaxOrder ||
+                (m[k].dAp && (!m[k].dAi || !m[k].dAx)))
                 return -1;
         std::vector<unsigned char> ok(K, 1);
         if (!dense) par(K, [&](int k) { ok[k] = m[k].n == 0 || valid_csc(m[k]); });
@@ -667,7 +672,7 @@ public:
 
         // layout in the caller's order
         desc_h_.resize(K);
-        std::vector<long long> nz_off(K + 1, 0);
+        std::vector<long long> nz_off(K + 1, 0), col_off(K, 0);
         b_off_.assign(K + 1, 0);
         size_t a_total = 0;
         long long col_total = 0, piv_total = 0;
@@ -678,13 +683,16 @@ public:
             d.a_off = (long long)a_total;
             d.n = n;
             d.npad = npad;
-            d.col_off = (int)col_total;
             d.ipiv_off = -1;
             d.ord_off = d.env_off = -1;
+            d.row_base = 0;
+            d.ap = d.ai = nullptr;
+            d.ax = nullptr;
+            col_off[k] = col_total;
             a_total += (size_t)npad * npad;
-            col_total += n + 1;
+            if (!dense && !m[k].dAp) col_total += n + 1;  // staged from the host
             piv_total += npad;
-            nz_off[k + 1] = nz_off[k] + (n > 0 && !dense ? m[k].Ap[n] : 0);
+            nz_off[k + 1] = nz_off[k] + (n > 0 && !dense && !m[k].dAp ? m[k].Ap[n] : 0);
             b_off_[k + 1] = b_off_[k] + n;
             npad_max = std::max(npad_max, npad);
         }
@@ -756,9 +764,20 @@ public:
 
         if (!dense) par(K, [&](int k) {
             const int n = m[k].n;
+            MatDesc& d = desc_h_[k];
+            if (m[k].dAp) {  // already on the device
+                d.ap = m[k].dAp;
+                d.ai = m[k].dAi;
+                d.ax = m[k].dAx;
+                d.row_base = m[k].row_base;
+                return;
+            }
+            d.ap = colp_.p + col_off[k];
+            d.ai = rowi_.p;
+            d.ax = vals_.p;
             if (n == 0) return;
             const long long base = nz_off[k];
-            int* cp = colp_h_.p + desc_h_[k].col_off;
+            int* cp = colp_h_.p + col_off[k];
             for (int c = 0; c <= n; ++c) cp[c] = (int)(base + m[k].Ap[c]);
             const int cnt = m[k].Ap[n];
             std::copy(m[k].Ai, m[k].Ai + cnt, rowi_h_.p + base);
@@ -768,7 +787,7 @@ public:
         if (hipMemcpyAsync(desc_.p, desc_h_.data(), sizeof(MatDesc) * K, hipMemcpyHostToDevice, st) != hipSuccess ||
             (aux_total > 0 &&
              hipMemcpyAsync(aux_.p, aux_h_.p, sizeof(int) * aux_total, hipMemcpyHostToDevice, st) != hipSuccess) ||
-            (!dense &&
+            (col_total > 0 &&
              hipMemcpyAsync(colp_.p, colp_h_.p, sizeof(int) * col_total, hipMemcpyHostToDevice, st) != hipSuccess) ||
             (nnz > 0 &&
              (hipMemcpyAsync(rowi_.p, rowi_h_.p, sizeof(int) * nnz, hipMemcpyHostToDevice, st) != hipSuccess ||
@@ -1098,8 +1117,8 @@ private:
         for (int k0 = 0; k0 < K; k0 += kGridY) {
             const int kc = std::min(kGridY, K - k0);
             hipLaunchKernelGGL(fill_identity_kernel, dim3(npad_max, kc), dim3(256), 0, st, desc + k0, A_.p, aux_.p);
-            hipLaunchKernelGGL(scatter_kernel, dim3((npad_max + 255) / 256, kc), dim3(256), 0, st, desc + k0, colp_.p,
-                               rowi_.p, vals_.p, A_.p, aux_.p);
+            hipLaunchKernelGGL(scatter_kernel, dim3((npad_max + 255) / 256, kc), dim3(256), 0, st, desc + k0, A_.p,
+                               aux_.p);
         }
         return hipGetLastError() != hipSuccess ? -1 : 0;
     }

@@ -152,9 +152,15 @@ class BatchDenseLU {
 public:
     struct Matrix {
         int n;
-        const int* Ap;
+        const int* Ap;  // the pattern on the host (always: it chooses the order)
         const int* Ai;
         const double* Ax;
+        // optionally the same matrix already on the device, used instead of copying Ap/Ai/Ax there:
+        // column c's entries at dAi/dAx[dAp[c] .. dAp[c + 1]), row r stored as r + row_base
+        const int* dAp = nullptr;
+        const int* dAi = nullptr;
+        const double* dAx = nullptr;
+        int row_base = 0;
     };
     // runs fn(k) for k in [0, count), possibly on several threads
     using ParallelFor = void (*)(void* user, int count, void (*fn)(void*, int), void* arg);
