@@ -484,19 +484,22 @@ struct BatchPasses {
     }
 
     // stage-II rescale of the staged block-diagonal CSC (Ap_h, Ai_h, Ax_h global),
-    // dvec_h, speeds vA_h, factors fac_h, stress steps dtau_h, weight kinds wf_h
-    int rescale(int nnz)
+    // dvec_h, speeds vA_h, factors fac_h, stress steps dtau_h, weight kinds wf_h;
+    // upload_csc = false: the device CSC already holds exactly that matrix
+    int rescale(int nnz, bool upload_csc)
     {
         ++csc_gen;
         const size_t T = total;
-        if (!Ai_d.p || Ai_d.bytes < sizeof(int) * (size_t)nnz) {
-            D4_TRY(ctx->ensure(Ai_d, sizeof(int) * (size_t)std::max(nnz, 1), "batch Ai"));
-            D4_TRY(ctx->ensure(Ax_d, sizeof(double) * (size_t)std::max(nnz, 1), "batch Ax"));
-            drop_graphs();
+        if (upload_csc) {
+            if (!Ai_d.p || Ai_d.bytes < sizeof(int) * (size_t)nnz) {
+                D4_TRY(ctx->ensure(Ai_d, sizeof(int) * (size_t)std::max(nnz, 1), "batch Ai"));
+                D4_TRY(ctx->ensure(Ax_d, sizeof(double) * (size_t)std::max(nnz, 1), "batch Ax"));
+                drop_graphs();
+            }
+            D4_TRY(h2d(Ap_d, Ap_h, sizeof(int) * (T + 1), "batch Ap upload"));
+            D4_TRY(h2d(Ai_d, Ai_h, sizeof(int) * (size_t)nnz, "batch Ai upload"));
+            D4_TRY(h2d(Ax_d, Ax_h, sizeof(double) * (size_t)nnz, "batch Ax upload"));
         }
-        D4_TRY(h2d(Ap_d, Ap_h, sizeof(int) * (T + 1), "batch Ap upload"));
-        D4_TRY(h2d(Ai_d, Ai_h, sizeof(int) * (size_t)nnz, "batch Ai upload"));
-        D4_TRY(h2d(Ax_d, Ax_h, sizeof(double) * (size_t)nnz, "batch Ax upload"));
         D4_TRY(h2d(dvec_d, dvec_h, sizeof(double) * T, "batch weights upload"));
         D4_TRY(h2d(vA_d, vA_h, sizeof(double) * T, "batch speed upload"));
         D4_TRY(h2d(fac_d, fac_h, sizeof(double) * R, "batch factor upload"));
@@ -726,6 +729,9 @@ int run_batched(d4_ens* e, const std::vector<int>& act)
             j.nnz = nnz;
         });
     } else if (kind == PassJob::Rescale) {
+        // the group's Jacobian pass left this very CSC on the device (only speed passes ran since, and they
+        // do not touch it): rescale it there instead of staging it from the host again
+        const bool on_dev = e->csc_gen == bp.csc_gen && e->csc_act == act;
         std::vector<int> base(R + 1, 0);
         for (int k = 0; k < R; ++k) base[k + 1] = base[k] + e->data[act[k]]->Ap[e->data[act[k]]->dc];
         const int nnz = base[R];
@@ -738,19 +744,21 @@ int run_batched(d4_ens* e, const std::vector<int>& act)
             PassJob& j = s.job();
             sdddstCore::SimulationData& d = s.data();
             const int n = off[k + 1] - off[k];
-            for (int c = 0; c < n; ++c) Ap[off[k] + c] = base[k] + d.Ap[c];
-            if (k == R - 1) Ap[off[R]] = nnz;
-            int* Ai = bp.Ai_h.as<int>() + base[k];
-            const int m = base[k + 1] - base[k];
-            for (int q = 0; q < m; ++q) Ai[q] = d.Ai[q] + off[k];
-            std::memcpy(bp.Ax_h.as<double>() + base[k], d.Ax.data(), sizeof(double) * m);
+            if (!on_dev) {
+                for (int c = 0; c < n; ++c) Ap[off[k] + c] = base[k] + d.Ap[c];
+                if (k == R - 1) Ap[off[R]] = nnz;
+                int* Ai = bp.Ai_h.as<int>() + base[k];
+                const int m = base[k + 1] - base[k];
+                for (int q = 0; q < m; ++q) Ai[q] = d.Ai[q] + off[k];
+                std::memcpy(bp.Ax_h.as<double>() + base[k], d.Ax.data(), sizeof(double) * m);
+            }
             std::memcpy(bp.dvec_h.as<double>() + off[k], d.dVec.data(), sizeof(double) * n);
             std::memcpy(bp.vA_h.as<double>() + off[k], j.speed->data(), sizeof(double) * n);
             bp.fac_h.as<double>()[k] = j.factor;
             bp.dtau_h.as<double>()[k] = j.dtau;
             bp.wf_h.as<int>()[k] = d.weightFunc;
         });
-        rc = bp.rescale(nnz);
+        rc = bp.rescale(nnz, !on_dev);
         if (rc) return rc;
         e->csc_gen = bp.csc_gen;
         e->csc_act = act;
