@@ -102,8 +102,9 @@ typedef struct {
  * diagonal included (Ap n+1, Ai/Ax nnz), indexes[i] = position of A_ii in Ax,
  * dvec = weight(s).  Pairs are stored iff d^2 < (heaviside ? 1 : 49) cut_sq,
  * tapered by exp(-(d - rc)^2/rc^2) beyond rc when not heaviside.
- * If cap < nnz nothing is written to Ai/Ax, *nnz gets the required size and
- * D4_ERR_CAPACITY is returned. */
+ * Ai/Ax may be written up to cap entries (past nnz too).  If cap < nnz,
+ * *nnz gets the required size and D4_ERR_CAPACITY is returned; the caller
+ * grows Ai/Ax and calls again (the entries written are not the matrix). */
 int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, double* vA, double* vB,
                      int* Ap, int* Ai, double* Ax, int cap, int* nnz, int* indexes, double* dvec, double* tol);
 
