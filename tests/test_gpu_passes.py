@@ -153,6 +153,31 @@ def test_jacobian_pass_vs_oracle(pkg, ctx, orc, case):
     assert np.all(np.abs(G["vB"] - R["vB"]) <= allow + 1e-15)
 
 
+def test_jacobian_pass_list_overflow_on_a_fresh_context(pkg, ctx, orc):
+    # a new context's pair list starts from the caller's capacity (here none):
+    # the tapered 300-point case lists far more pairs, so the pass runs again
+    # with a longer list, then reports the CSC size; the wrapper retries with it
+    xy, k = _jac_case("taper300")
+    n = xy.shape[0]
+    cutoff, cut_sq, one_per = orc.cutoff(n, k["mult"], k["hv"])
+    p = pkg.jac_params(h=k["h"], tau_a=k["ta"], tau_b=k["tb"], cutoff=cutoff, heaviside=k["hv"], wfunc=k["wf"])
+    R = orc.jacobian_pass(xy, k["h"], k["ta"], k["tb"], cutoff, cut_sq, one_per, k["hv"], k["wf"])
+    assert (R["nnz"] - n) // 2 > n // 2 + 1024  # more pairs than the first list holds
+    fresh = pkg.Context(0)
+    try:
+        G = fresh.jacobian_pass(xy, p, cap=0)
+    finally:
+        fresh.close()
+    assert G["nnz"] == R["nnz"]
+    assert np.array_equal(G["Ap"], R["Ap"]) and np.array_equal(G["Ai"], R["Ai"])
+    assert np.array_equal(G["indexes"], R["indexes"])
+    # the sorted CSC does not depend on the order the list filled in: the same bits as a context whose
+    # list was long enough from the start
+    S = ctx.jacobian_pass(xy, p, cap=G["nnz"])
+    for key in ("vA", "vB", "Ap", "Ai", "Ax", "indexes", "dvec"):
+        np.testing.assert_array_equal(G[key], S[key])
+
+
 def test_jacobian_pass_capacity_and_tolerance(pkg, ctx, orc):
     xy, k = _jac_case("heaviside1024")
     n = xy.shape[0]

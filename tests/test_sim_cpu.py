@@ -164,6 +164,15 @@ def test_capacity_growth_keeps_results(L, orc):
     assert dconf.max_periodic_dx(s.positions(), ref) <= 1e-13
 
 
+def test_stats_carry_pass_and_solve_time(L):
+    xyb = dconf.read_dconf(FIX)
+    s = _new(L, xyb[:, 0], xyb[:, 1], xyb[:, 2], **dict(C1, step_limit=3))
+    s.run()
+    st = s.stats()
+    assert st["pair_passes"] > 0 and st["factorizations"] > 0
+    assert st["pass_ms"] > 0.0 and st["solve_ms"] > 0.0
+
+
 def test_step_interface_and_stop(L):
     xyb = dconf.read_dconf(FIX)
     s = _new(L, xyb[:, 0], xyb[:, 1], xyb[:, 2], **dict(C1, step_limit=1))
