@@ -457,8 +457,9 @@ struct BatchPasses {
         return ctx->sync("batch speed pass");
     }
 
-    // Jacobian pass -> vA_h, vB_h, Ap_h, idx_h, dvec_h, tol_h and (fetch_entries) Ai_h, Ax_h
-    int jacobian(bool with_tol, bool fetch_entries, int* nnz_out)
+    // Jacobian pass -> vA_h, vB_h, Ap_h, idx_h, dvec_h, tol_h and (fetch_entries) Ai_h and
+    // (fetch_values) Ax_h
+    int jacobian(bool with_tol, bool fetch_entries, int* nnz_out, bool fetch_values = true)
     {
         ++csc_gen;
         finish_params();
@@ -474,10 +475,11 @@ struct BatchPasses {
         const int nnz = Ap_h.as<int>()[total];
         if (nnz_out) *nnz_out = nnz;
         if (fetch_entries) {
-            if (!Ai_h.ensure(sizeof(int) * (size_t)nnz) || !Ax_h.ensure(sizeof(double) * (size_t)nnz))
+            if (!Ai_h.ensure(sizeof(int) * (size_t)nnz) ||
+                (fetch_values && !Ax_h.ensure(sizeof(double) * (size_t)nnz)))
                 return fail(D4_ERR_HIP, "pinned host allocation failed");
             D4_TRY(d2h(Ai_h, Ai_d, sizeof(int) * (size_t)nnz, "batch Ai download"));
-            D4_TRY(d2h(Ax_h, Ax_d, sizeof(double) * (size_t)nnz, "batch Ax download"));
+            if (fetch_values) D4_TRY(d2h(Ax_h, Ax_d, sizeof(double) * (size_t)nnz, "batch Ax download"));
             D4_TRY(ctx->sync("batch entries download"));
         }
         return D4_OK;
@@ -486,7 +488,7 @@ struct BatchPasses {
     // stage-II rescale of the staged block-diagonal CSC (Ap_h, Ai_h, Ax_h global),
     // dvec_h, speeds vA_h, factors fac_h, stress steps dtau_h, weight kinds wf_h;
     // upload_csc = false: the device CSC already holds exactly that matrix
-    int rescale(int nnz, bool upload_csc)
+    int rescale(int nnz, bool upload_csc, bool fetch_values)
     {
         ++csc_gen;
         const size_t T = total;
@@ -520,7 +522,7 @@ struct BatchPasses {
         a.total = total;
         hipError_t e = launch_rescale(a, ctx->stream);
         if (e != hipSuccess) return ctx->hip_fail(e, "batch rescale");
-        D4_TRY(d2h(Ax_h, Ax_d, sizeof(double) * (size_t)nnz, "batch Ax download"));
+        if (fetch_values) D4_TRY(d2h(Ax_h, Ax_d, sizeof(double) * (size_t)nnz, "batch Ax download"));
         D4_TRY(d2h(dvec_h, dvec_d, sizeof(double) * T, "batch weights download"));
         D4_TRY(d2h(vA_h, vA_d, sizeof(double) * T, "batch speed download"));
         return ctx->sync("batch rescale");
@@ -553,6 +555,9 @@ struct d4_ens {
     // the device, and that pass's generation: still there while bp->csc_gen has not moved on
     uint64_t csc_gen = 0;
     std::vector<int> csc_act, csc_off;
+    // per realization: its host CSC values (SimulationData::Ax) were not fetched, the device CSC being
+    // the copy its GPU solver and the rescale read
+    std::vector<char> ax_stale;
     d4_ens_config cfg{0, 0, 0, 0};
     bool started = false;
     std::string err;
@@ -608,6 +613,9 @@ int run_linear(d4_ens* e, const std::vector<int>& act, PassJob::Kind kind)
             // the matrices are still on the device when this group's Jacobian or rescale pass was the last to
             // write the device CSC (it precedes the Factor job): the solver reads them there
             const bool on_dev = e->bp && e->csc_gen == e->bp->csc_gen && e->csc_act == act;
+            if (!on_dev)
+                for (int r : gpu)
+                    if (e->ax_stale[r]) return e->fail(D4_ERR_HIP, "ensemble: CSC values no longer on the device");
             std::vector<int> pos;
             if (on_dev) {
                 pos.assign(e->data.size(), -1);
@@ -688,11 +696,14 @@ int run_batched(d4_ens* e, const std::vector<int>& act)
                 bp.wf_h.as<int>()[k] = j.jp.wfunc;
             }
         });
+        // the host solver's realizations need the values on the host; the GPU solver's read them on the device
+        bool host_values = false;
+        for (int k = 0; k < R; ++k) host_values |= e->data[act[k]]->solverKind != 1;
         if (kind == PassJob::Speed) {
             rc = bp.speed(with_tol);
         } else {
             int nnz = 0;
-            rc = bp.jacobian(with_tol, true, &nnz);
+            rc = bp.jacobian(with_tol, true, &nnz, host_values);
             e->csc_gen = bp.csc_gen;
             e->csc_act = act;
             e->csc_off = off;
@@ -723,7 +734,8 @@ int run_batched(d4_ens* e, const std::vector<int>& act)
             for (int c = 0; c <= n; ++c) d.Ap[c] = Ap[off[k] + c] - base;
             const int* Ai = bp.Ai_h.as<int>() + base;
             for (int m = 0; m < nnz; ++m) d.Ai[m] = Ai[m] - off[k];
-            std::memcpy(d.Ax.data(), bp.Ax_h.as<double>() + base, sizeof(double) * nnz);
+            e->ax_stale[act[k]] = d.solverKind == 1;
+            if (d.solverKind != 1) std::memcpy(d.Ax.data(), bp.Ax_h.as<double>() + base, sizeof(double) * nnz);
             const int* idx = bp.idx_h.as<int>() + off[k];
             for (int c = 0; c < n; ++c) d.indexes[c] = idx[c] - base;
             j.nnz = nnz;
@@ -732,6 +744,12 @@ int run_batched(d4_ens* e, const std::vector<int>& act)
         // the group's Jacobian pass left this very CSC on the device (only speed passes ran since, and they
         // do not touch it): rescale it there instead of staging it from the host again
         const bool on_dev = e->csc_gen == bp.csc_gen && e->csc_act == act;
+        bool host_values = false;
+        for (int k = 0; k < R; ++k) {
+            host_values |= e->data[act[k]]->solverKind != 1;
+            if (!on_dev && e->ax_stale[act[k]])
+                return e->fail(D4_ERR_HIP, "ensemble: CSC values no longer on the device");
+        }
         std::vector<int> base(R + 1, 0);
         for (int k = 0; k < R; ++k) base[k + 1] = base[k] + e->data[act[k]]->Ap[e->data[act[k]]->dc];
         const int nnz = base[R];
@@ -758,7 +776,7 @@ int run_batched(d4_ens* e, const std::vector<int>& act)
             bp.dtau_h.as<double>()[k] = j.dtau;
             bp.wf_h.as<int>()[k] = d.weightFunc;
         });
-        rc = bp.rescale(nnz, !on_dev);
+        rc = bp.rescale(nnz, !on_dev, host_values);
         if (rc) return rc;
         e->csc_gen = bp.csc_gen;
         e->csc_act = act;
@@ -768,7 +786,9 @@ int run_batched(d4_ens* e, const std::vector<int>& act)
             PassJob& j = s.job();
             sdddstCore::SimulationData& d = s.data();
             const int n = off[k + 1] - off[k];
-            std::memcpy(d.Ax.data(), bp.Ax_h.as<double>() + base[k], sizeof(double) * (base[k + 1] - base[k]));
+            e->ax_stale[act[k]] = d.solverKind == 1;
+            if (d.solverKind != 1)
+                std::memcpy(d.Ax.data(), bp.Ax_h.as<double>() + base[k], sizeof(double) * (base[k + 1] - base[k]));
             std::memcpy(d.dVec.data(), bp.dvec_h.as<double>() + off[k], sizeof(double) * n);
             std::memcpy(j.speed->data(), bp.vA_h.as<double>() + off[k], sizeof(double) * n);
         });
@@ -880,6 +900,7 @@ int d4_ens_create(d4_ctx* ctx, int R, const int* off, const double* x, const dou
     }
     e->bp.reset(new BatchPasses(ctx));
     e->pool.reset(new ThreadPool(default_threads()));
+    e->ax_stale.assign(e->data.size(), 0);
     *out = e.release();
     return D4_OK;
 }
