@@ -16,10 +16,8 @@
 // block and the identity beyond it.  The padding changes nothing in the
 // solution: its rows and columns are decoupled from the leading block and
 // never enter a pivot choice.  It gives every matrix an order that is a
-// multiple of the factorization's block and lets matrices of one order share
-// a batched rocSOLVER call (pointer arrays) in the pivoted fallback, whose
-// batched routines treat each matrix of a batch alone.  A matrix's factors
-// do not depend on the batch it is in (bitwise; the ensemble tests check it).
+// multiple of the factorization's block.  A matrix's factors do not depend
+// on the batch it is in (bitwise; the ensemble tests check it).
 // GpuDenseLU is BatchLU with K = 1.
 //
 // Pivoting: threshold partial pivoting with a preference for the diagonal and
@@ -33,7 +31,7 @@
 // cores); if every pivot is finite and nonzero and every multiplier
 // |l_rc| <= 1/u = 1000, the diagonal was an acceptable pivot at every stage
 // and the factorization stands.  Otherwise the matrix is scattered again
-// and factored with partial pivoting (rocSOLVER getrf_batched).  The choice
+// and factored with partial pivoting (rocSOLVER getrf, one call per matrix).  The choice
 // depends on the matrix values only.
 //
 // Solves: a triangular-solve kernel of this file (lu_solve_kernel), one
@@ -751,12 +749,12 @@ public:
         for (int i = 0; i < K; ++i) pos[order[i]] = i;
 
         if (!A_.ensure(a_total) || !ipiv_.ensure((size_t)piv_total) || !iscr_.ensure(2 * (size_t)piv_total) ||
-            !info_.ensure(3 * (size_t)K) || !ptrA_.ensure(2 * (size_t)K) || !desc_.ensure(2 * (size_t)K) ||
+            !info_.ensure(3 * (size_t)K) || !desc_.ensure(2 * (size_t)K) ||
             !sdesc_.ensure(K) || !colp_.ensure((size_t)col_total) || !rowi_.ensure((size_t)nnz) ||
             !vals_.ensure((size_t)nnz) || !b_.ensure(2 * (size_t)b_off_[K]))
             return -2;
         if (!colp_h_.ensure((size_t)col_total) || !rowi_h_.ensure((size_t)nnz) || !vals_h_.ensure((size_t)nnz) ||
-            !ptr_h_.ensure(2 * (size_t)K) || !info_h_.ensure(3 * (size_t)K))
+            !info_h_.ensure(3 * (size_t)K))
             return -2;
         int* info = info_.p;          // [0, K): no-pivot pass (group order)
         int* flag = info_.p + K;      // [K, 2K): check (caller's order)
@@ -783,7 +781,6 @@ public:
             std::copy(m[k].Ai, m[k].Ai + cnt, rowi_h_.p + base);
             std::copy(m[k].Ax, m[k].Ax + cnt, vals_h_.p + base);
         });
-        for (int i = 0; i < K; ++i) ptr_h_.p[i] = A_.p + desc_h_[order[i]].a_off;
         if (hipMemcpyAsync(desc_.p, desc_h_.data(), sizeof(MatDesc) * K, hipMemcpyHostToDevice, st) != hipSuccess ||
             (aux_total > 0 &&
              hipMemcpyAsync(aux_.p, aux_h_.p, sizeof(int) * aux_total, hipMemcpyHostToDevice, st) != hipSuccess) ||
@@ -791,8 +788,7 @@ public:
              hipMemcpyAsync(colp_.p, colp_h_.p, sizeof(int) * col_total, hipMemcpyHostToDevice, st) != hipSuccess) ||
             (nnz > 0 &&
              (hipMemcpyAsync(rowi_.p, rowi_h_.p, sizeof(int) * nnz, hipMemcpyHostToDevice, st) != hipSuccess ||
-              hipMemcpyAsync(vals_.p, vals_h_.p, sizeof(double) * nnz, hipMemcpyHostToDevice, st) != hipSuccess)) ||
-            hipMemcpyAsync(ptrA_.p, ptr_h_.p, sizeof(double*) * K, hipMemcpyHostToDevice, st) != hipSuccess)
+              hipMemcpyAsync(vals_.p, vals_h_.p, sizeof(double) * nnz, hipMemcpyHostToDevice, st) != hipSuccess)))
             return -2;
         if (assemble(desc_.p, K, npad_max, st)) return -2;
         if (hipMemsetAsync(info, 0, sizeof(int) * 2 * K, st) != hipSuccess) return -2;
@@ -817,8 +813,13 @@ public:
         if (F > 0) {
             std::vector<int> order2 = by_order(F, [&](int i) { return redo[i]; });
             for (int& i : order2) i = redo[i];
-            std::vector<Group> groups2 = make_groups(order2);
+            // one rocSOLVER getrf per matrix (its blocked panels: ~34 ms at order 4096, where getrf_batched
+            // with one matrix steps column by column and took ~180 ms); a matrix gets the same bits from
+            // it whatever else is refactored in the call
+            std::vector<Group> groups2;
+            for (int i = 0; i < F; ++i) groups2.push_back({i, 1, desc_h_[order2[i]].npad});
             std::vector<MatDesc> sub(F);
+            std::vector<double*> redo_a(F);  // each matrix's store
             int npad2 = 0;
             long long poff = 0;
             for (int i = 0; i < F; ++i) {
@@ -828,19 +829,16 @@ public:
                 sub[i] = desc_h_[k];
                 sub[i].env_off = -1;  // assembled and factored whole (in the same order)
                 redo_pos[k] = i;
-                ptr_h_.p[K + i] = A_.p + desc_h_[k].a_off;
+                redo_a[i] = A_.p + desc_h_[k].a_off;
                 npad2 = std::max(npad2, desc_h_[k].npad);
             }
-            if (hipMemcpyAsync(desc_.p + K, sub.data(), sizeof(MatDesc) * F, hipMemcpyHostToDevice, st) != hipSuccess ||
-                hipMemcpyAsync(ptrA_.p + K, ptr_h_.p + K, sizeof(double*) * F, hipMemcpyHostToDevice, st) !=
-                    hipSuccess)
+            if (hipMemcpyAsync(desc_.p + K, sub.data(), sizeof(MatDesc) * F, hipMemcpyHostToDevice, st) != hipSuccess)
                 return -2;
             if (assemble(desc_.p + K, F, npad2, st)) return -2;
             if (hipMemsetAsync(info2, 0, sizeof(int) * F, st) != hipSuccess) return -2;
             if (run_groups(groups2, r, [&](rocblas_handle h, const Group& g) {
-                    return rocsolver_dgetrf_batched(h, g.npad, g.npad, ptrA_.p + K + g.first, g.npad,
-                                                    ipiv_.p + desc_h_[order2[g.first]].ipiv_off, g.npad,
-                                                    info2 + g.first, g.count);
+                    return rocsolver_dgetrf(h, g.npad, g.npad, redo_a[g.first], g.npad,
+                                            ipiv_.p + desc_h_[order2[g.first]].ipiv_off, info2 + g.first);
                 }))
                 return -2;
             if (npad2 <= lds_bytes_ / 8)
@@ -927,7 +925,6 @@ private:
     std::vector<int> ndesc_;
     DevBuf<double> A_, vals_, b_;
     DevBuf<int> ipiv_, iscr_, info_, colp_, rowi_;
-    DevBuf<double*> ptrA_;
     struct DenseSrc {
         const double* a = nullptr;
         long long rs = 0, cs = 0;
@@ -949,7 +946,6 @@ private:
     DevBuf<SolveDesc> sdesc_;
     PinBuf<int> colp_h_, rowi_h_, info_h_;
     PinBuf<double> vals_h_, bh_;
-    PinBuf<double*> ptr_h_;
 
     // indices 0..count-1 (through key to the caller's index) by padded order, largest first, stable
     template <class Key>
@@ -960,18 +956,6 @@ private:
         std::stable_sort(o.begin(), o.end(),
                          [&](int a, int b) { return desc_h_[key(a)].npad > desc_h_[key(b)].npad; });
         return o;
-    }
-
-    std::vector<Group> make_groups(const std::vector<int>& order) const
-    {
-        std::vector<Group> g;
-        for (int i = 0; i < (int)order.size(); ++i) {
-            const int npad = desc_h_[order[i]].npad;
-            if (npad == 0) continue;
-            if (g.empty() || g.back().npad != npad) g.push_back({i, 0, npad});
-            g.back().count++;
-        }
-        return g;
     }
 
     // the blocked no-pivot LU of every matrix, in `order` (largest first):

@@ -145,7 +145,7 @@ int gpu_lu_padded(int n);
 // The dense LU of new_gpu_dense_lu for K matrices at once: the unpivoted
 // factorization covers every matrix in the same launches (three per 64
 // columns), the matrices that need row interchanges go through rocSOLVER
-// getrf_batched per padded order, and all solves through one launch of the
+// getrf (one call per matrix, spread over side streams), and all solves through one launch of the
 // triangular-solve kernel.  A matrix's factors and
 // solutions do not depend on which other matrices share the call.
 class BatchDenseLU {
