@@ -520,7 +520,9 @@ int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* 
     // it whatever the pass finds; the one synchronization at the end tells
     // whether the list overflowed (pass run again) or the caller's CSC is too
     // small (D4_ERR_CAPACITY with *nnz set, as before).
-    size_t coo_cap = std::max<size_t>((size_t)std::max(cap, n) / 2 + 1024, ctx->coo.bytes / sizeof(CooEntry));
+    const size_t max_pairs = (size_t)n * (size_t)(n - 1) / 2;  // bounds for a caller capacity far above need
+    size_t coo_cap = std::max<size_t>(std::min<size_t>((size_t)std::max(cap, n) / 2 + 1024, max_pairs + 1024),
+                                      ctx->coo.bytes / sizeof(CooEntry));
     double* dtol = nullptr;
     if (tol && p->tol_cap2 > 0) {
         D4_TRY(ctx->ensure(ctx->tol, b, "tolerance"));
@@ -534,7 +536,9 @@ int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* 
     int* didx = static_cast<int*>(ctx->idx.p);
     double* dd = static_cast<double*>(ctx->dvec.p);
     const int* dcount = static_cast<const int*>(ctx->coo_count.p);
-    const size_t down = (Ai && Ax && cap > 0) ? (size_t)cap : 0;  // CSC entries returned to the caller
+    // CSC entries returned to the caller: its capacity, at most n^2 (and inside the device CSC, which holds
+    // n + 2 * coo_cap >= min(cap, n^2) entries)
+    const size_t down = (Ai && Ax && cap > 0) ? std::min<size_t>((size_t)cap, (size_t)n * (size_t)n) : 0;
     int ncoo = 0, total_nnz = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
         D4_TRY(ctx->ensure(ctx->coo, sizeof(CooEntry) * coo_cap, "pair list"));
