@@ -84,6 +84,12 @@ def build_test_helpers(verbose=False):
         _run(["gcc"] + fast + ["-std=c11", "-c", "-o", orc_fast, orc_src], verbose)
         _run(["g++"] + fast + ["-std=c++17", "-shared", "-o", os.path.join(tdir, "libsim_cpu_fast.so"), sim_shim,
               os.path.join(CSRC, "d4_sim.cpp"), os.path.join(CSRC, "d4_solver.cpp"), orc_fast, "-lm"], verbose)
+    # a compiled caller that sees only the C ABI headers, linked against the built library
+    drv = os.path.join(tdir, "capi_driver.cpp")
+    if os.path.exists(drv) and os.path.exists(LIB):
+        _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o",
+              os.path.join(tdir, "capi_driver"), drv, "-L", LIBDIR, "-l2d4",
+              "-Wl,-rpath,$ORIGIN/../../2d4_amd/lib", "-Wl,-rpath,/opt/rocm/lib"], verbose)
     return out
 
 
