@@ -151,7 +151,7 @@ def ensemble_bench(pkg, ctx, steps, R, seed=4242, plain=True, solver=1):
             "unpivoted_factorizations": int(sum(s["unpivoted_factorizations"] for s in st_b)),
             "launch_config": cfg,
             "solver": "dense LU on the GPU, both paths: threshold pivoting u = 0.001 (blocked LU without interchanges, "
-                      "rocSOLVER getrf_batched where a multiplier exceeds 1000), triangular-solve kernel",
+                      "one rocSOLVER getrf per matrix where a multiplier exceeds 1000), triangular-solve kernel",
             "bitwise_equal_to_plain": equal}
 
 
