@@ -13,7 +13,7 @@
 // The central wall switches to the reference's 6th-order series when
 // dx^2 + dy^2 <= 1e-6 (Field.cpp:176-188, 213-250).
 //
-// Arithmetic layout (the part that is this build's own): one reciprocal per
+// Arithmetic layout (the part that is this library's own): one reciprocal per
 // image wall, shared by the wall's sigma and d sigma terms,
 //   r = 1/D,  t = N r,  f = u t r,  f' = r^2 (N + 2 pi u S (c - 2 t)),
 // and one reciprocal of E shared by all hyperbolics; every division of the

@@ -292,8 +292,8 @@ def main():
                     help="only run a plain device copy of this many MiB three times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is described "
-              "in DESIGN.md and not run here", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): this library runs each pass on one "
+              "GPU and does not shard it across several", file=sys.stderr)
         sys.exit(2)
 
     import torch

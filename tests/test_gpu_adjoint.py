@@ -1,7 +1,7 @@
 """Adjoint of the pair-speed map (VJP kernel) and gradients through implicit
 steps (2d4_amd/adjoint.py), checked against the transpose of the Jacobian the
 reference assembles, against the oracle's closed form of d sigma/dy, and
-against central finite differences of this build's own forward map."""
+against central finite differences of the library's own forward map."""
 from importlib import import_module
 
 import numpy as np
