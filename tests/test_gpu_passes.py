@@ -11,7 +11,8 @@ import os
 import numpy as np
 import pytest
 
-from helpers import TOL, all_pairs, bsign, pair_allowance, random_config, scale_dsigma, scale_sigma, wrap
+from helpers import (TOL, all_pairs, bsign, pair_allowance, random_config, scale_dsigma, scale_sigma,
+                     sort_reference, wrap)
 
 pytestmark = pytest.mark.gpu
 GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -115,10 +116,25 @@ def _jac_case(name):
     if name == "taper300":
         xy = random_config(300, 10)
         return xy, dict(h=2e-4, ta=0.02, tb=0.05, mult=0.05 * np.sqrt(300), hv=False, wf="m")
+    if name == "pair2":  # the smallest configuration: one pair, one dislocation of each sign
+        xy = sort_reference(np.array([[0.1, 0.2], [-0.3, -0.1]]))
+        return xy, dict(h=1e-3, ta=0.05, tb=0.07, mult=1e20, hv=False, wf="p")
+    if name == "nopairs16":  # a cutoff no pair is inside: the matrix is its diagonal alone
+        xy = random_config(16, 12)
+        return xy, dict(h=1e-3, ta=0.0, tb=0.0, mult=1e-4 * 4, hv=True, wf="c")
+    if name == "close8":
+        # a dipole 5e-4 apart (dx^2 + dy^2 < 1e-6, the near-coincident branch of Field.cpp:176-188)
+        # and a pair straddling the periodic boundary in x
+        rng = np.random.default_rng(13)
+        xy = rng.uniform(-0.4, 0.4, (8, 2))
+        xy[2] = xy[1] + (3e-4, 4e-4)
+        xy[3] = (0.4999, 0.05)
+        xy[4] = (-0.4998, 0.0502)
+        return sort_reference(xy), dict(h=1e-4, ta=0.01, tb=0.01, mult=0.3 * np.sqrt(8), hv=True, wf="c")
     raise KeyError(name)
 
 
-@pytest.mark.parametrize("case", ["dense64", "heaviside1024", "taper300"])
+@pytest.mark.parametrize("case", ["dense64", "heaviside1024", "taper300", "pair2", "nopairs16", "close8"])
 def test_jacobian_pass_vs_oracle(pkg, ctx, orc, case):
     xy, k = _jac_case(case)
     n = xy.shape[0]
@@ -130,6 +146,10 @@ def test_jacobian_pass_vs_oracle(pkg, ctx, orc, case):
     R = orc.jacobian_pass(xy, k["h"], k["ta"], k["tb"], cutoff, cut_sq, one_per, k["hv"], k["wf"])
     # structure: bit-exact
     assert G["nnz"] == R["nnz"]
+    if case == "nopairs16":
+        assert G["nnz"] == n
+    if case == "close8":
+        assert G["nnz"] > n + 2  # the dipole and the boundary pair are both inside the cutoff
     assert np.array_equal(G["Ap"], R["Ap"]) and np.array_equal(G["Ai"], R["Ai"])
     assert np.array_equal(G["indexes"], R["indexes"])
     # values: per-entry bound from the pair bounds on d sigma
