@@ -226,15 +226,22 @@ class Context:
                                               wfunc.encode() if isinstance(wfunc, str) else wfunc,
                                               float(factor), float(dtau), _ptr(speed)))
 
-    def jacobian_dense(self, xy, params, tol=None, scale=False):
-        """Dense Jacobian pass: (vA, vB, M, dvec); M column-major as (col, row) = M[i][j] -> A[j, i]."""
+    def jacobian_dense(self, xy, params, tol=None, scale=False, out=None):
+        """Dense Jacobian pass: (vA, vB, M, dvec) with M in (row, col) orientation.  With out, an (n, n)
+        C-contiguous float64 array, the library writes the matrix there column-major (out[c, r] = M[r, c])
+        and out itself is returned, without the transposed copy."""
         xy = _f64(xy).reshape(-1, 2)
         n = xy.shape[0]
         vA, vB, dvec = np.empty(n), np.empty(n), np.empty(n)
-        M = np.empty((n, n))
+        if out is not None:
+            if out.shape != (n, n) or out.dtype != np.float64 or not out.flags.c_contiguous:
+                raise ValueError("out must be an (n, n) C-contiguous float64 array")
+            M = out
+        else:
+            M = np.empty((n, n))
         self.check(self.L.d4_jacobian_dense(self.h, n, _ptr(xy), C.byref(params), _ptr(vA), _ptr(vB), _ptr(M),
                                             _ptr(dvec), _ptr(tol), 1 if scale else 0))
-        return vA, vB, M.T.copy(), dvec  # returned in (row, col) orientation
+        return vA, vB, (out if out is not None else M.T.copy()), dvec
 
     # ---------------------------------------------------------- torch API (device tensors)
     def speed_pass_t(self, xy, tau=0.0, tol=None, tol_cap2=0.0, out=None):

@@ -396,6 +396,20 @@ def main():
             extra[name] = dict(ms_per_pass=(t1 - t0) * 1e3 / args.steps, kernel_ms=k_ms2 / max(k_l2, 1),
                                pair_evals_per_s=pairs / ((t1 - t0) / args.steps), table="1024 x 1024 FP64")
         del tab_s, tab_d
+        # the same pass through the C ABI with host buffers (d4_jacobian_dense): positions up, then speeds,
+        # weights and the n x n matrix down into the caller's pageable memory; reported beside `value`, not as it
+        xy_h = xy.cpu().numpy()
+        Mh = np.empty((n, n))
+        ctx.jacobian_dense(xy_h, p, out=Mh)  # touches the host pages once
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.jacobian_dense(xy_h, p, out=Mh)
+        t1 = time.perf_counter()
+        extra["host_buffer_pass"] = dict(ms_per_pass=(t1 - t0) * 1e3 / reps, passes=reps,
+                                         pair_evals_per_s=pairs / ((t1 - t0) / reps),
+                                         matrix_bytes_to_host=8 * n * n)
+        del Mh, xy_h
     del M
     torch.cuda.empty_cache()
     ensemble = None

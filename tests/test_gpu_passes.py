@@ -248,6 +248,10 @@ def test_jacobian_dense_vs_oracle(pkg, ctx, orc, n):
     allow = speed_allowance(orc, xy)
     assert np.all(np.abs(vA - vA0) <= allow + 1e-15) and np.all(np.abs(vB - vB0) <= allow + 1e-15)
     assert np.all(np.abs(d - d0) <= np.diag(aV))
+    # into the caller's buffer: column-major, the same bits
+    out = np.full((n, n), np.nan)
+    vA2, _, Mo, _ = ctx.jacobian_dense(xy, p, out=out)
+    assert Mo is out and np.array_equal(out.T, M) and np.array_equal(vA2, vA)
     # scaled form equals the CSC pass's A (dense cutoff) to rounding
     _, _, A, _ = ctx.jacobian_dense(xy, p, scale=True)
     G = ctx.jacobian_pass(xy, p)
