@@ -109,6 +109,7 @@ private:
     }
     void loop()
     {
+        d4host::t_host_worker = true;
         uint64_t seen = 0;
         for (;;) {
             {
@@ -439,11 +440,12 @@ struct BatchPasses {
 
     int ensure_coo(size_t want)
     {
-        if (coo_cap >= want && Ai_d.bytes >= sizeof(int) * ((size_t)total + 2 * coo_cap)) return D4_OK;
+        // every pair listed gives at most two CSC entries beside the total diagonals
+        const size_t cap = std::max(want, coo_cap), nz = (size_t)total + 2 * cap;
+        if (coo_cap == cap && Ai_d.bytes >= sizeof(int) * nz && Ax_d.bytes >= sizeof(double) * nz) return D4_OK;
         drop_graphs();
-        coo_cap = std::max(want, coo_cap);
+        coo_cap = cap;
         D4_TRY(ctx->ensure(coo_d, sizeof(CooEntry) * coo_cap, "batch pair list"));
-        const size_t nz = (size_t)total + 2 * coo_cap;
         D4_TRY(ctx->ensure(Ai_d, sizeof(int) * nz, "batch Ai"));
         D4_TRY(ctx->ensure(Ax_d, sizeof(double) * nz, "batch Ax"));
         return D4_OK;
@@ -463,7 +465,9 @@ struct BatchPasses {
     {
         ++csc_gen;
         finish_params();
-        if (coo_cap == 0) D4_TRY(ensure_coo(std::max<size_t>((size_t)total * 8, 1 << 16)));
+        // the device CSC must hold total + 2 * coo_cap entries for this layout's total, which may have grown
+        // since the list was last sized (a later call over more realizations)
+        D4_TRY(ensure_coo(std::max(coo_cap, std::max<size_t>((size_t)total * 8, 1 << 16))));
         for (int attempt = 0; attempt < 2; ++attempt) {
             D4_TRY(run(g_jac[with_tol], [&] { return enqueue_jacobian(with_tol); }));
             D4_TRY(ctx->sync("batch jacobian pass"));
@@ -493,7 +497,7 @@ struct BatchPasses {
         ++csc_gen;
         const size_t T = total;
         if (upload_csc) {
-            if (!Ai_d.p || Ai_d.bytes < sizeof(int) * (size_t)nnz) {
+            if (!Ai_d.p || Ai_d.bytes < sizeof(int) * (size_t)nnz || Ax_d.bytes < sizeof(double) * (size_t)nnz) {
                 D4_TRY(ctx->ensure(Ai_d, sizeof(int) * (size_t)std::max(nnz, 1), "batch Ai"));
                 D4_TRY(ctx->ensure(Ax_d, sizeof(double) * (size_t)std::max(nnz, 1), "batch Ax"));
                 drop_graphs();
@@ -975,7 +979,11 @@ int d4_ens_set_config(d4_ens* e, const d4_ens_config* c)
     if (!e || !c || c->host_threads < 0) return D4_ERR_INVALID;
     D4_TRY(e->check(e->bp->set_config(*c)));
     const int want = c->host_threads > 0 ? std::min(c->host_threads, 64) : default_threads();
-    if (want != e->pool->size()) e->pool.reset(new ThreadPool(want));
+    if (want != e->pool->size()) {
+        e->pool.reset(new ThreadPool(want));
+        // the batched LU hands its host loops to the pool: give it the new one (the old one is gone)
+        if (e->blu) e->blu->set_parallel(pool_for, e->pool.get());
+    }
     e->cfg = *c;
     return D4_OK;
 }

@@ -194,6 +194,15 @@ void d4_ctx::stage_begin()
     pin_used = 0;
 }
 
+// A staged entry point's scope: it starts with an empty arena, and an error
+// return (no successful sync() since) drops the downloads it queued, so that
+// no later sync() copies them into the caller's arrays of this failed call.
+struct StageScope {
+    d4_ctx* c;
+    explicit StageScope(d4_ctx* ctx) : c(ctx) { c->stage_begin(); }
+    ~StageScope() { c->stage_begin(); }
+};
+
 int d4_ctx::stage(size_t bytes, size_t* off, const char* where)
 {
     const size_t at = (pin_used + 255) & ~(size_t)255;
@@ -429,7 +438,7 @@ int d4_speed_pass(d4_ctx* ctx, int n, const double* xy, double tau, double* v, d
     if (!ctx || n < 0 || (n > 0 && (!xy || !v))) return D4_ERR_INVALID;
     if (n == 0) return D4_OK;
     const size_t b = sizeof(double) * (size_t)n;
-    ctx->stage_begin();
+    StageScope staged(ctx);
     D4_TRY(ctx->ensure(ctx->xy, 2 * b, "positions"));
     D4_TRY(ctx->ensure(ctx->v0, b, "speeds"));
     double* dtol = nullptr;
@@ -505,7 +514,7 @@ int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* 
     *nnz = 0;
     if (n == 0) return D4_OK;
     const size_t b = sizeof(double) * (size_t)n;
-    ctx->stage_begin();
+    StageScope staged(ctx);
     D4_TRY(ctx->ensure(ctx->xy, 2 * b, "positions"));
     D4_TRY(ctx->ensure(ctx->v0, b, "speeds A"));
     D4_TRY(ctx->ensure(ctx->v1, b, "speeds B"));
@@ -598,7 +607,7 @@ int d4_jacobian_rescale(d4_ctx* ctx, int n, const int* Ap, const int* Ai, double
     if (n == 0) return D4_OK;
     const int nnz = Ap[n];
     const size_t b = sizeof(double) * (size_t)n;
-    ctx->stage_begin();
+    StageScope staged(ctx);
     D4_TRY(ctx->ensure(ctx->Ap, sizeof(int) * ((size_t)n + 1), "Ap"));
     D4_TRY(ctx->ensure(ctx->Ai, sizeof(int) * (size_t)std::max(nnz, 1), "Ai"));
     D4_TRY(ctx->ensure(ctx->Ax, sizeof(double) * (size_t)std::max(nnz, 1), "Ax"));

@@ -69,6 +69,7 @@ struct ThreadRes {
 
 ThreadRes* thread_res(int device)
 {
+    if (t_host_worker) return nullptr;  // no HIP from the ensemble's host workers (d4_solver.h)
     static std::mutex m;
     static std::map<std::pair<std::thread::id, int>, ThreadRes*> reg;
     std::lock_guard<std::mutex> l(m);

@@ -31,6 +31,13 @@
 
 namespace d4host {
 
+// True on the ensemble's host worker threads.  Those threads only move host
+// memory and run host solvers; HIP and rocBLAS are called from the thread
+// that steps the ensemble, so that every stream dependency is one that thread
+// orders (an event or a synchronization).  The GPU LU refuses to run on a
+// worker thread.
+inline thread_local bool t_host_worker = false;
+
 // factor a square CSC matrix once, then solve A x = b any number of times
 class LinearSolver {
 public:

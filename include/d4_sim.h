@@ -36,7 +36,15 @@ typedef struct {
     double strain_limit;     /* <= 0: none */
     double speed_threshold;  /* cutoff change threshold (simulation.cpp:215-219); <= 0: none */
     int solver;              /* linear solves (UMFPACK's place, simulation.cpp:635-659):
-                                0 host sparse LU, 1 dense LU on the GPU (rocSOLVER getrf/getrs) */
+                                0 host sparse LU (left-looking, minimum-degree order, threshold
+                                  pivoting u = 0.1 preferring the diagonal);
+                                1 dense LU on the GPU: the matrix in reverse Cuthill-McKee order
+                                  where that halves the work, padded to a multiple of 64, factored
+                                  first by a blocked LU without row interchanges inside its block
+                                  envelope; kept when every multiplier is at most 1000 (threshold
+                                  u = 0.001, the diagonal preferred), otherwise factored again with
+                                  partial pivoting by rocSOLVER getrf; solves by this library's
+                                  triangular-solve kernel */
 } d4_sim_params;
 
 typedef struct {

@@ -204,3 +204,40 @@ def test_ensemble_over_32768_columns_equals_plain_path(ctx):
     e.step(2)
     _check_equal(e, off, _plain(ctx, off, x, y, b, params, 2))
     e.close()
+
+
+def test_batch_jacobian_pass_growing_layout_on_shared_context(ctx, pkg):
+    """A batched Jacobian pass over far more columns than the first one on the same context: the device
+    CSC is sized for the new layout's diagonals (total) beside the pair list, and each realization still
+    equals the single-realization pass."""
+    for sizes, seed, rc in (([64, 100], 41, 0.05), ([4000] * 36, 43, 0.002)):
+        off, xy = _ragged(sizes, seed)
+        params = [pkg.jac_params(h=1e-3, tau_a=0.0, tau_b=0.05, n=n, cutoff_mult=rc * np.sqrt(n), heaviside=True)
+                  for n in sizes]
+        print(f"[batch grow] R={len(sizes)} total={off[-1]}", flush=True)
+        vA, vB, (Ap, Ai, Ax), idx, dvec = ens.batch_jacobian_pass(ctx, off, xy, params)
+        for r in sorted({0, len(sizes) // 2, len(sizes) - 1}):
+            s = slice(off[r], off[r + 1])
+            one = ctx.jacobian_pass(xy[s], params[r])
+            base = Ap[off[r]]
+            np.testing.assert_array_equal(vA[s], one["vA"])
+            np.testing.assert_array_equal(Ap[off[r]:off[r + 1] + 1] - base, one["Ap"])
+            np.testing.assert_array_equal(Ai[base:Ap[off[r + 1]]] - off[r], one["Ai"])
+            np.testing.assert_array_equal(Ax[base:Ap[off[r + 1]]], one["Ax"])
+            np.testing.assert_array_equal(dvec[s], one["dvec"])
+
+
+def test_ensemble_host_thread_change_keeps_gpu_solver(ctx):
+    """Changing the host thread count after the batched GPU LU exists: the LU's host loops run on the new
+    pool, and the results stay those of the plain path."""
+    sizes = [300, 130, 96, 200]
+    off, x, y, b = _ens_inputs(sizes, 19)
+    params = [dict(p, solver=1) for p in _params(sizes)]
+    e = ens.Ensemble(ctx, off, x, y, b, params)
+    e.step(1)
+    e.set_config(host_threads=2)
+    e.step(2)
+    e.set_config(host_threads=3)
+    e.step(1)
+    _check_equal(e, off, _plain(ctx, off, x, y, b, params, 4))
+    e.close()
