@@ -40,10 +40,12 @@ def build(quiet=True):
 
 
 class Oracle:
-    def __init__(self, fast=False):
-        path = os.path.join(HERE, "liboracle_fast.so" if fast else "liboracle.so")
-        if not os.path.exists(path):
-            build()
+    def __init__(self, fast=False, path=None):
+        """fast: the build with the reference's flags; path: another build of d4_oracle.c (tests only)."""
+        if path is None:
+            path = os.path.join(HERE, "liboracle_fast.so" if fast else "liboracle.so")
+            if not os.path.exists(path):
+                build()
         L = C.CDLL(path)
         d, i, ch = C.c_double, C.c_int, C.c_char
         sig = {

@@ -15,18 +15,27 @@
  *   orc_dxy4(exact=1), orc_dyxy4            central finite differences of orc_xy4.
  *   orc_normalize / orc_weight / orc_weight_inv   reference sim_utils.cpp under
  *       oracle/_ref on edge tables, plus closed-form round trips.
- *   orc_speed_pass / orc_jacobian_pass / orc_jacobian_dense / orc_jacobian_from_prev
+ *   orc_speed_pass / orc_jacobian_pass / orc_jacobian_dense
  *       brute force at small n over the reference Field (oracle/_ref), exact
- *       invariants (antisymmetric pair sums, V symmetric, row sums, A*1 = 1).
+ *       invariants (antisymmetric pair sums, V symmetric, row sums, A*1 = 1);
+ *       the weights d_i equal the reference's own weight (oracle/_ref) of the
+ *       unscaled diagonal s_i, bit for bit.
+ *   orc_jacobian_from_prev                  V and s are linear in h, so the pass
+ *       at h rescaled by 1/2 must equal the pass at h/2 entry for entry (all
+ *       weight functions, both cutoff modes).
  *   orc_read_dconf / orc_cutoff             the reference SimulationData under
  *       oracle/_ref on init_config_gen/README_files/1000_64.dconf.
  *   orc_table_bilinear / orc_table_build / orc_speed_pass_table   the build's own
  *       table format: exact at the nodes (the node value itself), exact on
  *       bilinear functions (a table of a + b x + c y + d x y reproduces it),
  *       periodic wrap; the nodes themselves are orc_xy / orc_dxy.
- *   orc_run                                 parity unpinned against the
- *       reference itself (simulation.cpp needs UMFPACK, absent from the image);
- *       checked by invariants (strain = sum b dx, accepted-step counts) only.
+ *   orc_run                                 not run against the reference's own
+ *       loop (simulation.cpp needs UMFPACK, absent from the image).  Pinned by
+ *       what the mathematics fixes: accept-all runs converge at second order to
+ *       a DOP853 integration of dx/dt = v(x; tau(t)) built on the pinned speed
+ *       pass, under a stress that changes in time; copies with a planted
+ *       mistake (ORC_MUTANT below: theta = 1, a wrong stage time, the sign of
+ *       g) fail that check; plus invariants (strain = sum b dx, step counts).
  */
 #include "d4_oracle.h"
 
@@ -576,6 +585,18 @@ static void sys_solve(orc_sys* s)
             if (!((-1 < s->x[i]) && (s->x[i] < 1))) s->x[i] = 0;
 }
 
+/* Planted mistakes, for the tests that show the run loop's checks can fail (never defined in the normal
+ * builds; tests/test_oracle.py compiles a copy with -DORC_MUTANT=k):
+ *   1  theta_i = 1 in every g (the implicit end point alone: backward Euler);
+ *   2  stage III's second speed pass at t1/2 instead of t1 (a wrong stage time);
+ *   3  the sign of g in every solve. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+#define THP(d) (ORC_MUTANT == 1 ? 2.0 : 1 + (d))
+#define THM(d) (ORC_MUTANT == 1 ? 0.0 : 1 - (d))
+#define GSIGN (ORC_MUTANT == 3 ? -1.0 : 1.0)
+
 int orc_run(int n, double* xy, const orc_run_params* p, orc_run_stats* st)
 {
     /* PrecisionHandler setup (simulation.cpp:24-26, precision_handler.cpp:75-89) */
@@ -622,35 +643,35 @@ int orc_run(int n, double* xy, const orc_run_params* p, orc_run_stats* st)
         orc_jacobian_pass(n, disl, h, TAU(t0), TAU(t1), cutoff, cut_sq, one_per, p->heaviside, p->wfunc,
                           init, sp, S.Ap, S.Ai, S.Ax, S.cap, S.idx, S.dvec, tol, cap2);
         sys_factor(&S);
-        for (int i = 0; i < n; ++i) S.g[i] = -h * ((1 + S.dvec[i]) * sp[i] + (1 - S.dvec[i]) * init[i]) / 2;
+        for (int i = 0; i < n; ++i) S.g[i] = -GSIGN * h * (THP(S.dvec[i]) * sp[i] + THM(S.dvec[i]) * init[i]) / 2;
         sys_solve(&S);
         for (int i = 0; i < n; ++i) { big[2 * i] = disl[2 * i] - S.x[i]; big[2 * i + 1] = disl[2 * i + 1]; }
         orc_speed_pass(n, big, TAU(t1), sp2, tol, cap2);
         for (int i = 0; i < n; ++i)
-            S.g[i] = big[2 * i] - disl[2 * i] - h * ((1 + S.dvec[i]) * sp2[i] + (1 - S.dvec[i]) * init[i]) / 2;
+            S.g[i] = GSIGN * (big[2 * i] - disl[2 * i] - h * (THP(S.dvec[i]) * sp2[i] + THM(S.dvec[i]) * init[i]) / 2);
         sys_solve(&S);
         for (int i = 0; i < n; ++i) big[2 * i] -= S.x[i];
         /* stage II */
         orc_jacobian_from_prev(n, S.Ap, S.Ai, S.Ax, S.dvec, p->wfunc, t0, th, t1, TAU(t1), TAU(th), sp);
         sys_factor(&S);
-        for (int i = 0; i < n; ++i) S.g[i] = -h / 2 * ((1 + S.dvec[i]) * sp[i] + (1 - S.dvec[i]) * init[i]) / 2;
+        for (int i = 0; i < n; ++i) S.g[i] = -GSIGN * h / 2 * (THP(S.dvec[i]) * sp[i] + THM(S.dvec[i]) * init[i]) / 2;
         sys_solve(&S);
         for (int i = 0; i < n; ++i) { s1[2 * i] = disl[2 * i] - S.x[i]; s1[2 * i + 1] = disl[2 * i + 1]; }
         orc_speed_pass(n, s1, TAU(th), sp2, tol, cap2);
         for (int i = 0; i < n; ++i)
-            S.g[i] = s1[2 * i] - disl[2 * i] - h / 2 * ((1 + S.dvec[i]) * sp2[i] + (1 - S.dvec[i]) * init[i]) / 2;
+            S.g[i] = GSIGN * (s1[2 * i] - disl[2 * i] - h / 2 * (THP(S.dvec[i]) * sp2[i] + THM(S.dvec[i]) * init[i]) / 2);
         sys_solve(&S);
         for (int i = 0; i < n; ++i) s1[2 * i] -= S.x[i];
         /* stage III */
         orc_jacobian_pass(n, s1, h / 2, TAU(th), TAU(t1), cutoff, cut_sq, one_per, p->heaviside, p->wfunc,
                           init2, sp2, S.Ap, S.Ai, S.Ax, S.cap, S.idx, S.dvec, tol, cap2);
         sys_factor(&S);
-        for (int i = 0; i < n; ++i) S.g[i] = -h / 2 * ((1 + S.dvec[i]) * sp2[i] + (1 - S.dvec[i]) * init2[i]) / 2;
+        for (int i = 0; i < n; ++i) S.g[i] = -GSIGN * h / 2 * (THP(S.dvec[i]) * sp2[i] + THM(S.dvec[i]) * init2[i]) / 2;
         sys_solve(&S);
         for (int i = 0; i < n; ++i) { s2[2 * i] = s1[2 * i] - S.x[i]; s2[2 * i + 1] = s1[2 * i + 1]; }
-        orc_speed_pass(n, s2, TAU(t1), sp2, tol, cap2);
+        orc_speed_pass(n, s2, TAU(ORC_MUTANT == 2 ? th : t1), sp2, tol, cap2);
         for (int i = 0; i < n; ++i)
-            S.g[i] = s2[2 * i] - s1[2 * i] - h / 2 * ((1 + S.dvec[i]) * sp2[i] + (1 - S.dvec[i]) * init2[i]) / 2;
+            S.g[i] = GSIGN * (s2[2 * i] - s1[2 * i] - h / 2 * (THP(S.dvec[i]) * sp2[i] + THM(S.dvec[i]) * init2[i]) / 2);
         sys_solve(&S);
         for (int i = 0; i < n; ++i) s2[2 * i] -= S.x[i];
         /* stage IV: error (simulation.cpp:692-700, precision_handler.cpp:41-54) */
@@ -697,6 +718,9 @@ int orc_run(int n, double* xy, const orc_run_params* p, orc_run_stats* st)
     return 0;
 #undef TAU
 }
+#undef THP
+#undef THM
+#undef GSIGN
 
 /* ------------------------------------------------------------ .dconf read */
 

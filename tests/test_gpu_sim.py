@@ -89,6 +89,33 @@ def test_c1_fixture_each_step_from_oracle_trajectory(ctx, orc):
     assert worst <= TRAJ_TOL
 
 
+def test_c1_whole_run_within_the_reference_flag_spread(ctx, orc):
+    """C1's whole 100-step run from the fixture, compared after every step with the oracle (-O2) run for
+    as many steps. The same source built with the reference's flags (-Ofast -ffast-math) leaves 1e-9 of
+    the -O2 build within a few steps (the run is chaotic); the GPU path is held to 1e-9 at every step up
+    to the last one at which those two builds of the reference path still agree within it."""
+    import oracle
+    fast = oracle.Oracle(fast=True)
+    xyb = dconf.read_dconf(FIX)
+    xy0 = _sorted_xy(xyb[:, 0], xyb[:, 1], xyb[:, 2])
+    s = simmod.Simulation.from_arrays(ctx, xyb[:, 0], xyb[:, 1], xyb[:, 2], **C1)
+    okw = _oracle_kw(C1)
+    gpu, flags = [], []
+    for k in range(1, 101):
+        s.step()
+        a, _ = orc.run(xy0, **dict(okw, step_limit=k - 1))
+        f, _ = fast.run(xy0, **dict(okw, step_limit=k - 1))
+        gpu.append(dconf.max_periodic_dx(s.positions(), a))
+        flags.append(dconf.max_periodic_dx(f, a))
+    s.close()
+    K = next((k for k, d in enumerate(flags) if d > TRAJ_TOL), len(flags))
+    print(f"C1 whole run: the two builds of the reference path agree within 1e-9 for {K} steps; GPU vs -O2 "
+          f"after steps 1..{K}: max {max(gpu[:K]):.3e}; after 10/50/100 steps GPU {gpu[9]:.2e} {gpu[49]:.2e} "
+          f"{gpu[99]:.2e}, reference flags {flags[9]:.2e} {flags[49]:.2e} {flags[99]:.2e}", flush=True)
+    assert K >= 1
+    assert max(gpu[:K]) <= TRAJ_TOL
+
+
 def test_c1_fixture_adaptive_100_steps(ctx, orc):
     kw = dict(C1)
     del kw["prec"], kw["dipole_prec"]

@@ -324,27 +324,119 @@ def test_jacobian_pass_cutoff_modes(orc):
             assert A[j[kk], i[kk]] == raw * ((1 + d[j[kk]]) / 2)
 
 
-def test_jacobian_from_prev_closed_form(orc):
-    """simulation.cpp:546-608 with factor 1/2: d' = w(w^-1(d)/2), off-diagonals A/2 (1+d'_r)/(1+d_r)."""
-    n = 30
-    xy = random_config(n, 5)
-    cut = orc.cutoff(n, 1e20, False)
-    R = orc.jacobian_pass(xy, 1e-2, 0.1, 0.3, *cut, False, "m")
-    A0 = _dense_from_csc(R, n)
-    d0 = R["dvec"].copy()
-    sp = np.arange(n, dtype=np.float64)
-    Ax = R["Ax"].copy()
-    dv = d0.copy()
-    orc.jacobian_from_prev(R["Ap"], R["Ai"], Ax, dv, "m", 0.0, 0.5, 1.0, 0.3, 0.2, sp)
-    d1 = np.array([0.0 if w == 0 else orc.L.orc_weight(orc.L.orc_weight_inv(w, b"m") * 0.5, b"m") for w in d0])
-    assert np.array_equal(dv, d1)
-    f = (1 + d1) / (1 + d0)
-    R2 = dict(R, Ax=Ax)
-    A1 = _dense_from_csc(R2, n)
-    off = ~np.eye(n, dtype=bool)
-    assert np.allclose(A1[off], (A0 / 2 * f[:, None])[off], rtol=1e-15, atol=0)
-    assert np.allclose(np.diag(A1), 1 + (np.diag(A0) - 1) / 2 * f, rtol=1e-15, atol=0)
-    assert np.allclose(sp, np.arange(n) + (0.2 - 0.3) * bsign(n), rtol=0, atol=1e-15)
+def test_rescale_equals_the_half_step_pass(orc):
+    """simulation.cpp:546-608 against what the mathematics fixes: V = h b b dsigma and s = -sum V are
+    linear in h, so a pass at h rescaled with factor 1/2 (d' = w(w^-1(d)/2), entries scaled by
+    (1 + d'_row)/(1 + d_row), speeds moved from the stress at t1 to the one at t1/2) must be the pass at
+    h/2 itself, entry for entry, to rounding -- for every weight function and cutoff mode."""
+    h, ta, t1, th = 2e-3, 0.01, 0.05, 0.03
+    for wf in "cpm":
+        for n, mult, hv in ((40, 1e20, False), (300, 0.05 * np.sqrt(300), False), (300, 0.05 * np.sqrt(300), True)):
+            xy = random_config(n, 50 + n)
+            cut = orc.cutoff(n, mult, hv)
+            R = orc.jacobian_pass(xy, h, ta, t1, *cut, hv, wf)
+            H = orc.jacobian_pass(xy, h / 2, ta, th, *cut, hv, wf)
+            assert (R["dvec"] > 0).sum() > n // 4  # the weights are active on many rows
+            Ax, dv, sp = R["Ax"].copy(), R["dvec"].copy(), R["vB"].copy()
+            orc.jacobian_from_prev(R["Ap"], R["Ai"], Ax, dv, wf, 0.0, 0.5, 1.0, t1, th, sp)
+            assert np.array_equal(R["Ap"], H["Ap"]) and np.array_equal(R["Ai"], H["Ai"])
+            assert np.all(np.abs(Ax - H["Ax"]) <= 2e-15 * np.abs(H["Ax"])), (wf, n, hv)
+            assert np.all(np.abs(dv - H["dvec"]) <= 1e-13 * H["dvec"]), (wf, n, hv)
+            assert np.all(np.abs(sp - H["vB"]) <= 1e-14 * (np.abs(H["vB"]) + 1)), (wf, n, hv)
+
+
+def test_weights_are_the_reference_weight_of_the_diagonal(orc, ref):
+    """d_i = w(s_i) (simulation.cpp:500-510) with s_i the unscaled diagonal of M = V + diag(s), through
+    the reference's own sim_utils.cpp weight (oracle/_ref), bit for bit; and s_i = -sum_j V_ij."""
+    n = 200
+    xy = random_config(n, 61)
+    for wf in "cpm":
+        vA, vB, M, d = orc.jacobian_dense(xy, 3e-3, 0.0, 0.1, wf)
+        s = np.diag(M).copy()
+        w = np.array([ref.L.ref_weight(x, wf.encode()) for x in s])
+        assert (w > 0).sum() > n // 4 and (w == 0).sum() > 0
+        assert np.array_equal(d, w)
+        off = M - np.diag(s)
+        assert np.all(np.abs(s + off.sum(axis=1)) <= 1e-13 * np.abs(off).sum(axis=1))
+        cut = orc.cutoff(n, 1e20, False)
+        R = orc.jacobian_pass(xy, 3e-3, 0.0, 0.1, *cut, False, wf)
+        assert np.array_equal(R["dvec"], w)
+
+
+# ------------------------------------------------------------------ the run loop (simulation.cpp:88-276)
+
+def _convergence(o, tau0=0.3, rate=40.0, T=0.02, levels=(10, 20, 40, 80)):
+    """Accept-all fixed steps (huge -P, -p 0, -M = the step) of the loop at h = T/k against a DOP853
+    integration (rtol = atol = 1e-13) of dx_i/dt = v_i(x; tau(t)) built on the pinned speed pass, from a
+    well-separated 16-dislocation configuration under a stress that grows in time (-r rate -s tau0)."""
+    from scipy.integrate import solve_ivp
+    from helpers import sort_reference
+    rng = np.random.default_rng(3)
+    g = np.arange(4) * 0.25 - 0.375
+    X, Y = np.meshgrid(g, g)
+    xy = sort_reference(np.stack([X.ravel(), Y.ravel()], 1) + rng.uniform(-0.03, 0.03, (16, 2)))
+    ode = oracle_mod().Oracle()
+    sol = solve_ivp(lambda t, x: ode.speed_pass(np.stack([x, xy[:, 1]], 1), tau0 + rate * t), (0, T), xy[:, 0],
+                    method="DOP853", rtol=1e-13, atol=1e-13)
+    xe = sol.y[:, -1]
+    errs = []
+    for k in levels:
+        h = T / k
+        out, st = o.run(xy, dt0=h, max_dt=h, prec=1e10, dipole_prec=0.0, step_limit=k - 1, stress_kind=1,
+                        stress_init=tau0, stress_rate=rate)
+        assert st.ok_steps == k and st.failed_steps == 0
+        errs.append(np.abs(wrap(out[:, 0] - xe)).max())
+    return np.array(errs)
+
+
+def oracle_mod():
+    import oracle
+    return oracle
+
+
+def test_run_loop_converges_at_second_order(orc):
+    """The theta-step (theta_i = (1 + d_i)/2, d_i = w(s_i) = O(h^2) on these non-stiff rows) with step
+    doubling is a second-order method: halving h divides the end-point error by 4. A wrong theta weight,
+    a wrong stage time or a wrong sign in g makes it first order or no method at all (next test)."""
+    e = _convergence(orc)
+    ratios = e[:-1] / e[1:]
+    print("errors", e, "ratios", ratios)
+    assert np.all((ratios > 3.8) & (ratios < 4.2)) and e[-1] < 1e-6
+
+
+@pytest.mark.parametrize("mutant", [1, 2, 3])
+def test_run_loop_check_catches_planted_mistakes(orc, tmp_path, mutant):
+    """The same check against copies of the oracle with a planted mistake (ORC_MUTANT in d4_oracle.c):
+    theta = 1, a stage-III speed at the wrong time, the sign of g. Each one loses the second order."""
+    import subprocess
+    src = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "d4_oracle.c")
+    lib = str(tmp_path / f"liboracle_m{mutant}.so")
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-std=c11", f"-DORC_MUTANT={mutant}", "-o", lib, src, "-lm"],
+                   check=True)
+    e = _convergence(oracle_mod().Oracle(path=lib))
+    ratios = e[:-1] / e[1:]
+    good = _convergence(orc)
+    print("mutant", mutant, "errors", e, "ratios", ratios)
+    assert not np.all((ratios > 3.8) & (ratios < 4.2)) and e[-1] > 10 * good[-1]
+
+
+def test_c1_run_is_chaotic_under_the_reference_flags(orc):
+    """C1 (1000_64.dconf, 100 accept-all steps of 1e-3) with the oracle at -O2 against the same source at
+    the reference's flags (-Ofast -ffast-math, CMakeLists.txt:6): the two agree to rounding after one step
+    and separate beyond 1e-9 within a few steps, so no implementation that rounds differently from the
+    reference's build can meet 1e-9 over the whole run; the GPU test of C1 is held to 1e-9 for as long
+    as these two builds stay within it."""
+    fast = oracle_mod().Oracle(fast=True)
+    xy0 = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dconf_golden.npz"))["xy"]
+    devs = {}
+    for k in (1, 3, 5, 10, 100):
+        kw = dict(dt0=1e-3, max_dt=1e-3, prec=1e10, dipole_prec=0.0, step_limit=k - 1)
+        a, _ = orc.run(xy0, **kw)
+        b, _ = fast.run(xy0, **kw)
+        devs[k] = np.abs(wrap(a[:, 0] - b[:, 0])).max()
+    print("O2 vs reference flags, max |dx| after k steps:", devs)
+    assert devs[1] <= 1e-12
+    assert devs[10] > 1e-6 and devs[100] > 1e-3
 
 
 def test_run_invariants(orc, golden):
