@@ -66,6 +66,7 @@ SIGNATURES = {
     "d4_field_eval_dev": (C.c_int, [_vp, C.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
     "d4_speed_pass": (C.c_int, [_vp, C.c_int, _vp, _d, _vp, _vp, _d]),
     "d4_speed_pass_dev": (C.c_int, [_vp, C.c_int, _vp, _d, _vp, _vp, _d]),
+    "d4_speed_pass_dev_noalloc": (C.c_int, [_vp, C.c_int, _vp, _d, _vp, _vp, _d]),
     "d4_jacobian_pass": (C.c_int, [_vp, C.c_int, _vp, _P(JacParams), _vp, _vp, _vp, _vp, _vp, C.c_int,
                                    _pi, _vp, _vp, _vp]),
     "d4_jacobian_rescale": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_char, _d, _d, _vp]),

@@ -6,10 +6,39 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace d4;
+
+// Scratch of d4_speed_pass_dev_noalloc: the tile table and the tile partial sums of one speed pass
+// over at most D4_NOALLOC_MAX_N dislocations, in device memory the library holds from load to exit
+// (8.6 MB on each GPU it runs on).  Every call on a GPU shares that GPU's copy.
+namespace {
+constexpr int kNoAllocMaxTiles = ((D4_NOALLOC_MAX_N + kTile - 1) / kTile) * ((D4_NOALLOC_MAX_N + kTile - 1) / kTile + 1) / 2;
+constexpr size_t kNoAllocTileDoubles = (sizeof(TileDesc) * kNoAllocMaxTiles + 255) / 256 * 256 / sizeof(double);
+constexpr size_t kNoAllocDoubles = kNoAllocTileDoubles + 128 * (size_t)kNoAllocMaxTiles;
+}  // namespace
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+// this device's address of `scratch` (host code never passes the symbol itself to a kernel): looked up
+// once per device, with that device current
+static int noalloc_scratch_ptr(int device, void** out)
+{
+    static std::mutex m;
+    static std::vector<void*> per_device;
+    std::lock_guard<std::mutex> l(m);
+    if (device < 0) return D4_ERR_INVALID;
+    if ((int)per_device.size() <= device) per_device.resize(device + 1, nullptr);
+    if (!per_device[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return D4_ERR_HIP;
+        per_device[device] = p;
+    }
+    *out = per_device[device];
+    return D4_OK;
+}
 
 d4_ctx::~d4_ctx()
 {
@@ -292,7 +321,7 @@ int d4_ctx_create(int device, d4_ctx** out)
         return D4_ERR_HIP;
     }
     c->stream = c->own_stream;
-    if (c->ensure(c->coo_count, sizeof(int) * 4, "coo counter")) {
+    if (c->ensure(c->coo_count, sizeof(int) * 4, "coo counter") || noalloc_scratch_ptr(device, &c->noalloc_scratch)) {
         delete c;
         return D4_ERR_HIP;
     }
@@ -453,6 +482,46 @@ int d4_speed_pass(d4_ctx* ctx, int n, const double* xy, double tau, double* v, d
     D4_TRY(ctx->d2h_s(v, ctx->v0.p, b, "speed download"));
     if (dtol) D4_TRY(ctx->d2h_s(tol, dtol, b, "tolerance download"));
     return ctx->sync("speed pass");
+}
+
+int d4_speed_pass_dev_noalloc(d4_ctx* ctx, int n, const double* xy, double tau, double* v, double* tol,
+                              double tol_cap2)
+{
+    if (!ctx || n < 0 || (n > 0 && (!xy || !v))) return D4_ERR_INVALID;
+    if (n > D4_NOALLOC_MAX_N)
+        return ctx->fail(D4_ERR_INVALID, "d4_speed_pass_dev_noalloc: n = " + std::to_string(n) + " exceeds " +
+                                             std::to_string(D4_NOALLOC_MAX_N) + ", the size of its scratch");
+    if (n == 0) return D4_OK;
+    if (!ctx->noalloc_scratch) return ctx->fail(D4_ERR_HIP, "d4_speed_pass_dev_noalloc: no scratch on this device");
+    // the same tiles, slots and kernels as d4_speed_pass_dev, so the same bits
+    TileDesc* tiles = static_cast<TileDesc*>(ctx->noalloc_scratch);
+    double* part = static_cast<double*>(ctx->noalloc_scratch) + kNoAllocTileDoubles;
+    const int nb = (n + kTile - 1) / kTile;
+    RealParams rp = real_params(n, nullptr, tau);
+    rp.tol_cap2 = tol ? tol_cap2 : 0.0;
+    hipError_t e = launch_tile_table(tiles, n, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "noalloc tile table");
+    PassArgs a{};
+    a.xy = reinterpret_cast<const double2*>(xy);
+    a.n0 = n;
+    a.tiles = tiles;
+    a.ntiles = nb * (nb + 1) / 2;
+    a.rp0 = rp;
+    a.part = part;
+    a.tol = tol;
+    e = launch_pair_pass(kModeSpeed, a, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "noalloc speed pass");
+    ReduceArgs r{};
+    r.part = part;
+    r.nq = 1;
+    r.rp0 = rp;
+    r.R = 1;
+    r.total = n;
+    r.vA = v;
+    r.add_tau = 1;
+    e = launch_reduce(r, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "noalloc reduce");
+    return D4_OK;
 }
 
 int d4_jacobian_dense_dev(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, double* vA, double* vB,

@@ -68,6 +68,9 @@ struct d4_ctx {
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_marks;
+    // this device's copy of the allocation-free entry point's scratch array (d4_capi.hip), looked up
+    // once per device when a context is created on it
+    void* noalloc_scratch = nullptr;
     BatchPasses* batch = nullptr;  // tables and buffers of d4_batch_* (d4_batch.hip)
     d4host::BatchDenseLU* lu = nullptr;  // d4_lu_solve_batch's solver (d4_gpulu.hip)
     // tabulated field for the table pass modes (set around single_pass)

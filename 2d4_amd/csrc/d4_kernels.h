@@ -134,6 +134,8 @@ hipError_t launch_csc_count(const CooEntry* coo, int ncoo, int* cnt, int total, 
 hipError_t launch_scan(const int* cnt, int* Ap, int total, hipStream_t s, int* scratch);
 hipError_t launch_csc_fill(const CooEntry* coo, int ncoo, const int* Ap, int* cursor, int* Ai, double* Ax,
                            int total, hipStream_t s, const int* ncoo_dev = nullptr);
+// the single-realization tile table of n (row-major upper triangle, slot = index) built on the device
+hipError_t launch_tile_table(TileDesc* t, int n, hipStream_t s);
 // p[0..n) = v (a kernel, so it can sit in a captured graph)
 hipError_t launch_fill_int(int* p, int v, int n, hipStream_t s);
 // wfunc per realization from wf_arr (null: wf0 for all)

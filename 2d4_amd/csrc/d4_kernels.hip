@@ -589,6 +589,32 @@ __global__ void fill_int_kernel(int* p, int v, int n)
     if (k < n) p[k] = v;
 }
 
+// The upper-triangle tile table of one realization of n, row-major (slot = index), as
+// d4_ctx::single_tiles builds it on the host: tile t is (I, J) with I the largest row whose first
+// index I nb - I (I - 1) / 2 is at most t.
+__global__ void tile_table_kernel(TileDesc* t, int nb, int nt)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nt) return;
+    // first index of row I: I (2 nb - I + 1) / 2; estimate I from the quadratic, then correct it
+    const double b = 2.0 * nb + 1.0;
+    int I = (int)floor((b - sqrt(b * b - 8.0 * k)) / 2.0);
+    I = max(0, min(I, nb - 1));
+    while (I > 0 && I * (2 * nb - I + 1) / 2 > k) --I;
+    while (I + 1 < nb && (I + 1) * (2 * nb - I) / 2 <= k) ++I;
+    const int J = I + (k - I * (2 * nb - I + 1) / 2);
+    t[k] = TileDesc{0, I * kTile, J * kTile, k};
+}
+
+hipError_t launch_tile_table(TileDesc* t, int n, hipStream_t s)
+{
+    const int nb = (n + kTile - 1) / kTile;
+    const int nt = nb * (nb + 1) / 2;
+    if (nt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(tile_table_kernel, dim3((nt + 255) / 256), dim3(256), 0, s, t, nb, nt);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_int(int* p, int v, int n, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;

@@ -80,6 +80,17 @@ int d4_field_eval_dev(d4_ctx* ctx, int64_t n, const double* dx, const double* dy
 int d4_speed_pass(d4_ctx* ctx, int n, const double* xy, double tau, double* v, double* tol, double tol_cap2);
 int d4_speed_pass_dev(d4_ctx* ctx, int n, const double* xy, double tau, double* v, double* tol, double tol_cap2);
 
+/* d4_speed_pass_dev that allocates nothing (no counterpart in the reference).  For callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Inputs and outputs are the caller's device buffers, as for d4_speed_pass_dev, with the
+ * same results bit for bit.  Its tile table and partial sums live in a scratch array the library holds
+ * on every GPU it runs on (8.6 MB), so n is at most D4_NOALLOC_MAX_N (larger n: D4_ERR_INVALID).
+ * Every call on one GPU uses that GPU's one scratch array: calls that overlap on a GPU (two streams,
+ * two contexts or two threads) race and must be ordered by the caller. */
+#define D4_NOALLOC_MAX_N 8192
+int d4_speed_pass_dev_noalloc(d4_ctx* ctx, int n, const double* xy, double tau, double* v, double* tol,
+                              double tol_cap2);
+
 /* Parameters of a Jacobian pass (simulation.cpp:345-527 reads them from
  * SimulationData, simulation_data.h:181-197, and the stress protocol). */
 typedef struct {
