@@ -376,6 +376,17 @@ struct BatchPasses {
         int* ncoo = static_cast<int*>(ncoo_d.p);
         hipError_t e = launch_fill_int(ncoo, 0, 1, s);
         if (e == hipSuccess) e = launch_pair_pass(kModeJacCoo, pass_args(with_tol), s, cfg.pass_grid);
+        if (e == hipSuccess) {
+            CooDerivArgs da{};
+            da.xy = static_cast<const double2*>(xy_d.p);
+            da.off = static_cast<const int*>(off_d.p);
+            da.elem = static_cast<const int*>(elem_d.p);
+            da.rp = static_cast<const RealParams*>(rp_d.p);
+            da.coo = static_cast<CooEntry*>(coo_d.p);
+            da.count = ncoo;
+            da.cap = (int)coo_cap;
+            e = launch_coo_derivative(da, 0, s);
+        }
         if (e != hipSuccess) return ctx->hip_fail(e, "batch jacobian pass");
         ReduceArgs r = reduce_args(1);
         r.vA = static_cast<double*>(vA_d.p);

@@ -634,6 +634,15 @@ int d4_jacobian_pass(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* 
         D4_TRY(ctx->single_pass(kModeJacCoo, n, static_cast<double*>(ctx->xy.p), rp, dtol,
                                 static_cast<double*>(ctx->v0.p), static_cast<double*>(ctx->v1.p), nullptr, nullptr,
                                 nullptr, 0, nullptr, p->exact ? 1 : 0, 0, 1));
+        CooDerivArgs da{};
+        da.xy = static_cast<const double2*>(ctx->xy.p);
+        da.n0 = n;
+        da.rp0 = rp;
+        da.coo = static_cast<CooEntry*>(ctx->coo.p);
+        da.count = dcount;
+        da.cap = (int)coo_cap;
+        e = launch_coo_derivative(da, p->exact ? 1 : 0, ctx->stream);
+        if (e != hipSuccess) return ctx->hip_fail(e, "pair list derivative");
         // CSC assembly: counts (diagonal + both copies), scan, fill, sort/finish, scale
         e = launch_fill_int(dcnt, 1, n, ctx->stream);  // the diagonal of every column
         if (e == hipSuccess) e = launch_csc_count(dcoo, (int)coo_cap, dcnt, n, ctx->stream, dcount);

@@ -14,10 +14,11 @@
 // dx^2 + dy^2 <= 1e-6 (Field.cpp:176-188, 213-250).
 //
 // Arithmetic layout (the part that is this library's own): one reciprocal per
-// image wall, shared by the wall's sigma and d sigma terms,
+// near image wall (|k| <= 2), shared by the wall's sigma and d sigma terms,
 //   r = 1/D,  t = N r,  f = u t r,  f' = r^2 (N + 2 pi u S (c - 2 t)),
-// and one reciprocal of E shared by all hyperbolics; every division of the
-// reference becomes a multiply by one of these eleven reciprocals.
+// the five of them from one batch inversion, one reciprocal of E shared by all
+// hyperbolics, and the far walls (|k| >= 3) as series in 1/C, a constant
+// times E or 1/E; no division of the reference remains.
 // Where the reference's rounding sequence decides the result -- 1/E, the
 // hyperbolic identities, and C c - 1 and C - c of the central wall, which
 // cancel for near pairs -- the same sequence is kept (unfused, correctly
@@ -85,11 +86,11 @@ D4_HD double rcp(double x)
 // sim_utils.cpp:6-13 semantics: the unique value in [-0.5, 0.5) congruent to v
 // mod 1.  v - rint(v) is exact (a multiple of ulp(v) no larger than |v|) and
 // lies in [-0.5, 0.5]; only +0.5 needs the final -1, as the reference's
-// `while (n >= 0.5) n -= 1` gives.
+// `while (n >= 0.5) n -= 1` gives, and -(+0.5) is that value.
 D4_HD double wrap(double v)
 {
     double r = v - rint(v);
-    return r >= 0.5 ? r - 1.0 : r;
+    return r >= 0.5 ? -r : r;  // r = +0.5 exactly: -0.5 = r - 1
 }
 
 // dx^2 + dy^2 rounded as the reference rounds it (two products, one sum;
@@ -181,40 +182,40 @@ D4_HD void wall0(double dx, double C, double S, double c, double& sig, double& d
     }
 }
 
-// Seven reciprocals from one: Montgomery's batch inversion over a product tree
-// (6 products, one rcp(), 12 products back).  Each result is within a few ulp
-// of 1/x.  The products stay far inside binary64's range here (the seven wall
-// denominators lie between ~2e-5 and ~2e9, their product below ~1e34).
-D4_HD void batch_rcp7(const double* x, double* r)
+// Five reciprocals from one: Montgomery's batch inversion over a product tree
+// (4 products, one rcp(), 8 products back).  Each result is within a few ulp
+// of 1/x.  The products stay far inside binary64's range here (the five wall
+// denominators lie between ~2e-5 and ~3.3e6, their product below ~1e22).
+D4_HD void batch_rcp5(const double* x, double* r)
 {
-    const double p01 = x[0] * x[1], p23 = x[2] * x[3], p45 = x[4] * x[5];
-    const double p03 = p01 * p23, p46 = p45 * x[6];
-    const double iall = rcp(p03 * p46);
-    const double i03 = iall * p46, i46 = iall * p03;
-    const double i01 = i03 * p23, i23 = i03 * p01, i45 = i46 * x[6];
-    r[6] = i46 * p45;
+    const double p01 = x[0] * x[1], p23 = x[2] * x[3];
+    const double p03 = p01 * p23;
+    const double iall = rcp(p03 * x[4]);
+    const double i03 = iall * x[4];
+    r[4] = iall * p03;
+    const double i01 = i03 * p23, i23 = i03 * p01;
     r[0] = i01 * x[1];
     r[1] = i01 * x[0];
     r[2] = i23 * x[3];
     r[3] = i23 * x[2];
-    r[4] = i45 * x[5];
-    r[5] = i45 * x[4];
 }
 
 // sigma (and d sigma/dx if NEED_D) of one pair from (dx, dy) already wrapped
 // and the transcendental inputs E = 2^(kPi2Ln2Inv dx), R = 1/E, c = cos(2 pi dy).
-// Walls (index: offset u, weight w): 0: dx + 3; 1: dx - 3; 2: dx - 4s (1);
-// 3: dx + 4s (1 - |dx|); 4: dx - 5s (|dx|); 5, 6: dx +- 2; 7, 8: dx +- 1;
-// 9: dx (the central wall, or the series when dx^2 + dy^2 <= 1e-6).
+// Walls (offset u, weight w): dx +- 2, dx +- 1 and dx (the central wall, or the
+// series when dx^2 + dy^2 <= 1e-6) through one batch inversion; dx +- 3, dx - 4s
+// (1), dx + 4s (1 - |dx|), dx - 5s (|dx|) from series in rho = 1/C.
 //
-// The walls at +-4 and +-5 have C = cosh(2 pi k) E^(+-1) >= kCh4 e^-pi ~ 1.8e9,
-// so with rho = 1/C <= 5.6e-10 their terms are, exactly,
-//   f  = u rho (c - rho) / (1 - c rho)^2                      = u rho c + O(u rho^2)
-//   f' = rho [(c - rho) + 2 pi u s (c - 2t)] / (1 - c rho)^2  = rho c (1 - 2 pi s u) + O(36 rho^2)
+// For |k| >= 3, C = cosh(2 pi k) E^(+-1) >= kCh3 e^-pi ~ 3.3e6, so rho <= 3e-7 is
+// a constant times R or E, and the wall terms are the series
+//   f  = u rho (c - rho) / (1 - c rho)^2
+//      = u rho [c + rho (2c^2 - 1) + O(rho^2)]
+//   f' = rho [(c - rho) + 2 pi u s (c - 2t)] / (1 - c rho)^2
+//      = rho [c (1 - 2 pi u s) + rho (2c^2 - 1)(1 - 4 pi u s) + O(30 rho^2)]
 // (t = (c - rho)/(1 - c rho), s the sign of the "sinh", +1 for the reference's
-// cosh-as-sinh).  The remainders are below 1e-17 absolute, four orders under
-// the rounding of the near walls, so these three walls take the leading terms
-// and need no reciprocal; the other seven share one batch inversion.
+// cosh-as-sinh).  At +-3 two terms leave ~1e-19 out; at +-4 and +-5
+// (rho <= 5.6e-10) the first term alone leaves ~1e-18 out -- both four orders
+// under the rounding of the near walls, and no reciprocal is needed.
 // With NEED_D, want_d = false skips the derivative (a per-lane choice: where
 // lanes of a wavefront differ, the shared sigma work is done once).
 template <bool NEED_D, bool EXACT>
@@ -228,17 +229,15 @@ D4_HD void pair_field(double dx, double dy, double E, double R, double c, double
     const double ad = pos ? dx : -dx;
     const bool series = !(dist2(dx, dy) > 1e-6);
 
-    // the seven reciprocal walls, in batch order: +3, -3, +2, -2, +1, -1, 0
-    double C[7], D[7], N[7], r[7];
-    C[0] = kCh3 * E;
-    C[1] = kCh3 * R;
-    C[2] = h.Cp2;
-    C[3] = h.Cm2;
-    C[4] = h.Cp1;
-    C[5] = h.Cm1;
-    C[6] = 0.5 * h.P;
+    // the five reciprocal walls, in batch order: +2, -2, +1, -1, 0
+    double C[5], D[5], N[5], r[5];
+    C[0] = h.Cp2;
+    C[1] = h.Cm2;
+    C[2] = h.Cp1;
+    C[3] = h.Cm1;
+    C[4] = 0.5 * h.P;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
+    for (int k = 0; k < 4; ++k) {
         D[k] = C[k] - c;
         N[k] = fma(C[k], c, -1.0);
     }
@@ -246,32 +245,22 @@ D4_HD void pair_field(double dx, double dy, double E, double R, double c, double
         // central wall: C - c and C c - 1 rounded as the reference rounds them
         // (Field.cpp:178-179), since for near pairs both cancel
 #pragma clang fp contract(off)
-        D[6] = C[6] - c;
-        N[6] = C[6] * c - 1.0;
+        D[4] = C[4] - c;
+        N[4] = C[4] * c - 1.0;
     }
-    if (series) D[6] = 1.0;  // the series branch brings its own denominator
-    batch_rcp7(D, r);
+    if (series) D[4] = 1.0;  // the series branch brings its own denominator
+    batch_rcp5(D, r);
 
     double sig = 0.0, dsig = 0.0;
 #pragma unroll
-    for (int k = 0; k < 7; ++k) {
-        if (k == 6 && series) break;
-        const double uk = k == 0 ? dx + 3.0 : k == 1 ? dx - 3.0 : k == 2 ? dx + 2.0 : k == 3 ? dx - 2.0
-                        : k == 4 ? dx + 1.0 : k == 5 ? dx - 1.0 : dx;
+    for (int k = 0; k < 5; ++k) {
+        if (k == 4 && series) break;
+        const double uk = k == 0 ? dx + 2.0 : k == 1 ? dx - 2.0 : k == 2 ? dx + 1.0 : k == 3 ? dx - 1.0 : dx;
         const double r2 = r[k] * r[k];
         sig = fma(uk, N[k] * r2, sig);
         if (dd) {
-            // sinh of the wall: the cosh value itself at +-3 (the reference's
-            // cosh-as-sinh, Field.cpp:269-283) or its shift-signed copy (EXACT);
-            // the identities for |k| <= 2
-            double Sk;
-            if (k == 0) Sk = C[0];
-            else if (k == 1) Sk = EXACT ? -C[1] : C[1];
-            else if (k == 2) Sk = h.Sp2;
-            else if (k == 3) Sk = h.Sm2;
-            else if (k == 4) Sk = h.Sp1;
-            else if (k == 5) Sk = h.Sm1;
-            else Sk = 0.5 * h.M;
+            // sinh of the wall from the identities
+            const double Sk = k == 0 ? h.Sp2 : k == 1 ? h.Sm2 : k == 2 ? h.Sp1 : k == 3 ? h.Sm1 : 0.5 * h.M;
             const double q = fma(-2.0, N[k] * r[k], c);
             const double g = fma(kTwoPi * uk, Sk * q, N[k]);
             dsig = fma(g, r2, dsig);
@@ -282,13 +271,20 @@ D4_HD void pair_field(double dx, double dy, double E, double R, double c, double
         else wall0_series<false>(dx, dy, sig, dsig);
     }
 
-    // the walls at -4s (weight 1), +4s (1 - |dx|) and -5s (|dx|)
-    constexpr double kIch4 = 1.0 / kCh4, kIch5 = 1.0 / kCh5;
-    const double rho_p4 = kIch4 * R, rho_m4 = kIch4 * E, rho_p5 = kIch5 * R, rho_m5 = kIch5 * E;
+    // the walls at +-3: rho = R / cosh(6 pi) at +3, E / cosh(6 pi) at -3; q2 = 2c^2 - 1
+    constexpr double kIch3 = 1.0 / kCh3, kIch4 = 1.0 / kCh4, kIch5 = 1.0 / kCh5;
+    const double q2 = fma(2.0 * c, c, -1.0);
+    const double rp3 = kIch3 * R, rm3 = kIch3 * E;
+    const double up3 = dx + 3.0, um3 = dx - 3.0;
+    const double ap3 = fma(rp3, q2, c), am3 = fma(rm3, q2, c);
+    sig = fma(up3 * rp3, ap3, sig);
+    sig = fma(um3 * rm3, am3, sig);
+
+    // the walls at -4s (weight 1), +4s (1 - |dx|) and -5s (|dx|): rho c from E c or R c
+    const double Ef = pos ? E : R, Rf = pos ? R : E;
+    const double ec = Ef * c, rc = Rf * c;
     const double u2 = dx - 4.0 * sg, u3 = dx + 4.0 * sg, u4 = dx - 5.0 * sg;
-    const double rc2 = (pos ? rho_m4 : rho_p4) * c;
-    const double rc3 = (pos ? rho_p4 : rho_m4) * c;
-    const double rc4 = (pos ? rho_m5 : rho_p5) * c;
+    const double rc2 = kIch4 * ec, rc3 = kIch4 * rc, rc4 = kIch5 * ec;
     const double f3 = u3 * rc3, f4 = u4 * rc4;
     sig = fma(u2, rc2, sig);
     sig = fma(1.0 - ad, f3, sig);
@@ -296,7 +292,12 @@ D4_HD void pair_field(double dx, double dy, double E, double R, double c, double
     sig_out = kTwoPiSq * sig;
     if (dd) {
         // s = +1 (reference); EXACT: the sign of the shift
+        const double sp3 = 1.0, sm3 = EXACT ? -1.0 : 1.0;
         const double s2 = EXACT ? -sg : 1.0, s3 = EXACT ? sg : 1.0, s4 = EXACT ? -sg : 1.0;
+        const double bp3 = fma(c, fma(-kTwoPi * sp3, up3, 1.0), rp3 * q2 * fma(-2.0 * kTwoPi * sp3, up3, 1.0));
+        const double bm3 = fma(c, fma(-kTwoPi * sm3, um3, 1.0), rm3 * q2 * fma(-2.0 * kTwoPi * sm3, um3, 1.0));
+        dsig = fma(rp3, bp3, dsig);
+        dsig = fma(rm3, bm3, dsig);
         dsig = fma(rc2, fma(-kTwoPi * s2, u2, 1.0), dsig);
         dsig = fma((1.0 - ad) * rc3, fma(-kTwoPi * s3, u3, 1.0), dsig);
         dsig = fma(ad * rc4, fma(-kTwoPi * s4, u4, 1.0), dsig);

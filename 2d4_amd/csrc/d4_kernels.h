@@ -44,7 +44,8 @@ struct CooEntry {
 enum PassMode : int {
     kModeSpeed = 0,      // sigma only: speeds, tolerance
     kModeJacDense = 1,   // sigma + d sigma: speeds, V dense column-major, row sums
-    kModeJacCoo = 2,     // sigma all pairs, d sigma inside the storage radius -> COO list
+    kModeJacCoo = 2,     // sigma all pairs; the pairs inside the storage radius -> COO list, whose
+                         // d sigma launch_coo_derivative adds
     kModeVjp = 3,        // d sigma/dx (+ d sigma/dy): w-weighted pair sums
     kModeJacDenseCut = 4,  // as kModeJacDense with the storage radius of kModeJacCoo:
                            // V = 0 outside it (the reference's cutoff, dense storage)
@@ -80,6 +81,20 @@ struct PassArgs {
     int exact;                // mode 1/2/3: 1 = true derivative instead of the reference's
     int want_dy;              // mode 3: also d/dy
 };
+
+// The pair-list pass's second half: d sigma of each listed pair into its entry.
+struct CooDerivArgs {
+    const double2* xy;        // concatenated sorted positions
+    const int* off;           // R+1 realization offsets (null: one realization of n0)
+    const int* elem;          // realization of each element (null: one realization)
+    int n0;
+    const RealParams* rp;     // R (null: rp0)
+    RealParams rp0;
+    CooEntry* coo;
+    const int* count;         // pairs the pass listed (device)
+    int cap;                  // list capacity (the grid covers it)
+};
+hipError_t launch_coo_derivative(const CooDerivArgs& a, int exact, hipStream_t s);
 
 // number of accumulated quantities per mode
 inline int pass_nq(int mode)

@@ -44,6 +44,12 @@ __device__ __forceinline__ void coo_flush(const PassArgs& a, CooEntry* cbuf, int
     __syncthreads();  // every lane has read the buffer before it is refilled
 }
 
+// v with its sign flipped when neg: the product by b_i b_j = -1 (exact)
+__device__ __forceinline__ double flip_if(double v, bool neg)
+{
+    return __longlong_as_double(__double_as_longlong(v) ^ ((long long)neg << 63));
+}
+
 // One 64 x 64 tile of a pass, evaluated by one wavefront (the LDS arrays are
 // per-workgroup allocations of this function's instantiation).
 template <int MODE, bool EXACT>
@@ -51,7 +57,6 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
 {
     constexpr bool kTab = MODE == kModeSpeedTab || MODE == kModeJacDenseTab;
     constexpr bool kDense = MODE == kModeJacDense || MODE == kModeJacDenseCut || MODE == kModeJacDenseTab;
-    constexpr bool kCut = MODE == kModeJacCoo || MODE == kModeJacDenseCut;  // storage radius applies
     constexpr bool kCoo = MODE == kModeJacCoo;
     constexpr bool kVjp = MODE == kModeVjp;
     constexpr int NQ = (MODE == kModeSpeed || MODE == kModeJacCoo || MODE == kModeSpeedTab) ? 1 : 2;
@@ -77,6 +82,7 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
     const bool vi = i < n;
     const double2 pi = vi ? a.xy[base + i] : make_double2(0.0, 0.0);
     const double bi = i < half ? 1.0 : -1.0;
+    const bool ipos = i < half;
     {
         const int jj = td.J0 + lane;
         const double2 pj = jj < n ? a.xy[base + jj] : make_double2(0.0, 0.0);
@@ -108,37 +114,42 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                     atomic_min_nonneg(a.tol + base + j, d2);
                 }
                 const double bj = j < half ? 1.0 : -1.0;
+                const bool neg = ipos != (j < half);  // b_i b_j = -1
                 double E = 0.0, R = 0.0, c = 0.0;
                 if (!kTab) field_inputs(dx, dy, E, R, c);
                 double sig = 0.0, dsig = 0.0;
+                // source-side sums accumulate with the target's sign and are negated once at the end
                 if (MODE == kModeSpeedTab) {
                     sig = table_bilinear(a.tab_s, a.tab_n, dx, dy);
-                    const double f = sig * bi * bj;
+                    const double f = flip_if(sig, neg);
                     q0 += f;
-                    atomicAdd(&jq0[jl], -f);
+                    atomicAdd(&jq0[jl], f);
                 } else if (MODE == kModeJacDenseTab) {
                     sig = table_bilinear(a.tab_s, a.tab_n, dx, dy);
                     dsig = table_bilinear(a.tab_d, a.tab_n, dx, dy);
-                    const double f = sig * bi * bj;
+                    const double f = flip_if(sig, neg);
                     V = rp.h * bj * bi * dsig;
                     q0 += f;
-                    atomicAdd(&jq0[jl], -f);
-                    q1 -= V;
-                    atomicAdd(&jq1[jl], -V);
-                } else if (MODE == kModeSpeed) {
+                    atomicAdd(&jq0[jl], f);
+                    q1 += V;
+                    atomicAdd(&jq1[jl], V);
+                } else if (MODE == kModeSpeed || kCoo) {
+                    // the pair-list pass takes sigma here and d sigma for the listed pairs alone
+                    // (launch_coo_derivative), so its tiles do the speed pass's work
                     pair_field<false, false>(dx, dy, E, R, c, sig, dsig);
-                    const double f = sig * bi * bj;
+                    const double f = flip_if(sig, neg);
                     q0 += f;
-                    atomicAdd(&jq0[jl], -f);
+                    atomicAdd(&jq0[jl], f);
+                    if (kCoo) inr = d2 < rp.range_sq;
                 } else if (MODE == kModeJacDense) {
                     pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
-                    const double f = sig * bi * bj;
+                    const double f = flip_if(sig, neg);
                     V = rp.h * bj * bi * dsig;
                     q0 += f;
-                    atomicAdd(&jq0[jl], -f);
-                    q1 -= V;
-                    atomicAdd(&jq1[jl], -V);
-                } else if (kCut) {
+                    atomicAdd(&jq0[jl], f);
+                    q1 += V;
+                    atomicAdd(&jq1[jl], V);
+                } else if (MODE == kModeJacDenseCut) {
                     // sigma for every pair, d sigma only inside the storage radius:
                     // one evaluation whose derivative part runs for the lanes that need it
                     inr = d2 < rp.range_sq;
@@ -150,24 +161,22 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                             V *= exp(-(e * e) * rp.one_per);
                         }
                     }
-                    const double f = sig * bi * bj;
+                    const double f = flip_if(sig, neg);
                     q0 += f;
-                    atomicAdd(&jq0[jl], -f);
-                    if (MODE == kModeJacDenseCut) {
-                        q1 -= V;
-                        atomicAdd(&jq1[jl], -V);
-                    }
+                    atomicAdd(&jq0[jl], f);
+                    q1 += V;
+                    atomicAdd(&jq1[jl], V);
                 } else {  // kVjp
                     pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
                     const double bb = bi * bj;
                     const double coef = (wi - sw[jl]) * bb;
                     const double gx = coef * dsig;
                     q0 += gx;
-                    atomicAdd(&jq0[jl], -gx);
+                    atomicAdd(&jq0[jl], gx);
                     if (a.want_dy) {
                         const double gy = coef * pair_field_dy(dx, E, R, c, sin(kTwoPi * dy));
                         q1 += gy;
-                        atomicAdd(&jq1[jl], -gy);
+                        atomicAdd(&jq1[jl], gy);
                     }
                 }
             }
@@ -184,7 +193,7 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                         CooEntry e;
                         e.i = base + i;
                         e.j = base + j;
-                        e.v = V;
+                        e.v = 0.0;  // d sigma follows in launch_coo_derivative
                         cbuf[cfill + __popcll(mask & ((1ull << lane) - 1ull))] = e;
                     }
                     cfill += cnt;
@@ -226,22 +235,62 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
     }
     if (kCoo && cfill > 0) coo_flush(a, cbuf, cfill, lane);
     __syncthreads();
+    // quantity 0 and quantity 1 (the dense modes' -sum V, the VJP's y part) of the targets, then of the
+    // sources, whose sums were taken with the target's sign
     double* P = a.part + (size_t)td.slot * NQ * 128;
     P[lane] = q0;
-    P[64 + lane] = jq0[lane];
+    P[64 + lane] = -jq0[lane];
     if (NQ == 2) {
-        P[128 + lane] = q1;
-        P[192 + lane] = jq1[lane];
+        P[128 + lane] = kVjp ? q1 : -q1;
+        P[192 + lane] = -jq1[lane];
     }
 }
 
+
+// d sigma of the pairs a pair-list pass listed, one thread per entry (the entries past the
+// pass's count, or past the list's capacity, are left alone):
+// V = h b_i b_j d sigma(x_i - x_j, y_i - y_j), i < j, times exp(-(r - rc)^2 / rc^2) beyond rc when the
+// cutoff is not a step (simulation.cpp:461-470) -- the value the reference stores for the pair.
+template <bool EXACT>
+__global__ void coo_derivative_kernel(CooDerivArgs a)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= min(a.cap, *a.count)) return;
+    CooEntry& e = a.coo[k];
+    const int r = a.elem ? a.elem[e.i] : 0;
+    const int base = a.off ? a.off[r] : 0;
+    const int half = (a.off ? a.off[r + 1] - base : a.n0) >> 1;
+    const RealParams rp = a.rp ? a.rp[r] : a.rp0;
+    const double2 pi = a.xy[e.i], pj = a.xy[e.j];
+    const double dx = wrap(pi.x - pj.x), dy = wrap(pi.y - pj.y);
+    const double d2 = dist2(dx, dy);
+    double E, R, c, sig = 0.0, dsig = 0.0;
+    field_inputs(dx, dy, E, R, c);
+    pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
+    const double bi = e.i - base < half ? 1.0 : -1.0, bj = e.j - base < half ? 1.0 : -1.0;
+    double V = rp.h * bj * bi * dsig;
+    if (!rp.heaviside && d2 > rp.cut_sq) {  // simulation.cpp:469-470
+        const double t = sqrt(d2) - rp.cutoff;
+        V *= exp(-(t * t) * rp.one_per);
+    }
+    e.v = V;
+}
+
+hipError_t launch_coo_derivative(const CooDerivArgs& a, int exact, hipStream_t s)
+{
+    if (a.cap <= 0) return hipSuccess;
+    const dim3 g((a.cap + 255) / 256);
+    if (exact) hipLaunchKernelGGL(coo_derivative_kernel<true>, g, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(coo_derivative_kernel<false>, g, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 
 // 64 threads (one wavefront) per block; at least 4 waves per SIMD (<= 128
 // VGPRs) for the sigma-only pass, 3 (<= 168 VGPRs) for the dense pass, 2 for
 // the pair-list, dense-with-cutoff and adjoint passes (their extra live values
 // would spill at 168).  PERSIST: the grid strides over the tile table.
 template <int MODE, bool EXACT, bool PERSIST>
-__global__ __launch_bounds__(64, (MODE == kModeSpeed || MODE == kModeSpeedTab)
+__global__ __launch_bounds__(64, (MODE == kModeSpeed || MODE == kModeSpeedTab || MODE == kModeJacCoo)
                                      ? 4
                                      : ((MODE == kModeJacDense || MODE == kModeJacDenseTab) ? 3 : 2)) void pair_pass_kernel(
     PassArgs a)
