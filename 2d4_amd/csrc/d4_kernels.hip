@@ -820,21 +820,50 @@ hipError_t launch_rescale(const RescaleArgs& a, hipStream_t s)
 
 // ------------------------------------------------------ dense final scale
 
-__global__ void dense_scale_kernel(double* M, int64_t ld, int n, const double* dvec)
+// Each thread owns two consecutive rows (one 16-byte access per column when VEC: n and ld even, M
+// 16-byte aligned) and keeps their factors (1 + d)/2 for the kScaleCols columns of its block.
+// Unfused, as simulation.cpp:516-522 rounds it: A = M (1 + d_row)/2, then + 1 on the diagonal.
+constexpr int kScaleCols = 8;
+
+template <bool VEC>
+__global__ __launch_bounds__(256) void dense_scale_kernel(double* M, int64_t ld, int n, const double* dvec)
 {
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    const int col = blockIdx.y;
+#pragma clang fp contract(off)
+    const int row = 2 * (blockIdx.x * blockDim.x + threadIdx.x);
     if (row >= n) return;
-    double* p = M + (int64_t)col * ld + row;
-    double v = *p * ((1 + dvec[row]) / 2);
-    if (row == col) v += 1;
-    *p = v;
+    const bool two = row + 1 < n;
+    const double f0 = (1 + dvec[row]) / 2, f1 = two ? (1 + dvec[row + 1]) / 2 : 0.0;
+    const int c0 = blockIdx.y * kScaleCols, c1 = min(c0 + kScaleCols, n);
+    for (int c = c0; c < c1; ++c) {
+        double* p = M + (int64_t)c * ld + row;
+        if (VEC) {
+            double2 v = *reinterpret_cast<const double2*>(p);
+            v.x = v.x * f0;
+            v.y = v.y * f1;
+            if (row == c) v.x = v.x + 1;
+            if (row + 1 == c) v.y = v.y + 1;
+            *reinterpret_cast<double2*>(p) = v;
+        } else {
+            double v = p[0] * f0;
+            if (row == c) v = v + 1;
+            p[0] = v;
+            if (two) {
+                double w = p[1] * f1;
+                if (row + 1 == c) w = w + 1;
+                p[1] = w;
+            }
+        }
+    }
 }
 
 hipError_t launch_dense_scale(double* M, int64_t ld, int n, const double* dvec, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(dense_scale_kernel, dim3((n + 255) / 256, n), dim3(256), 0, s, M, ld, n, dvec);
+    const dim3 g((n + 511) / 512, (n + kScaleCols - 1) / kScaleCols);
+    if (n % 2 == 0 && ld % 2 == 0 && reinterpret_cast<uintptr_t>(M) % 16 == 0)
+        hipLaunchKernelGGL(dense_scale_kernel<true>, g, dim3(256), 0, s, M, ld, n, dvec);
+    else
+        hipLaunchKernelGGL(dense_scale_kernel<false>, g, dim3(256), 0, s, M, ld, n, dvec);
     return hipGetLastError();
 }
 

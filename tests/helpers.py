@@ -110,3 +110,40 @@ def pair_allowance(orc, dx, dy, E, c, deriv=False):
     return TOL * base + 4 * (np.abs(sE - s0) + np.abs(sc - s0)), s0
 
 
+
+
+def _row_sum(n, i, j, vi, vj):
+    out = np.zeros(n)
+    np.add.at(out, i, vi)
+    np.add.at(out, j, vj)
+    return out
+
+
+def speed_allowance(orc, xy):
+    """Per-row bound for a speed pass: the sum of the row's per-pair bounds (B) plus 1e-13 of its
+    absolute pair sum (a different, fixed summation order)."""
+    n = xy.shape[0]
+    i, j, dx, dy = all_pairs(xy)
+    e, c = orc.inputs(dx, dy)
+    a, s = pair_allowance(orc, dx, dy, e, c)
+    return _row_sum(n, i, j, a, a) + 1e-13 * _row_sum(n, i, j, np.abs(s), np.abs(s))
+
+
+def jacobian_allowance(orc, xy, h, R):
+    """Per-entry bound for a CSC Jacobian pass R (the oracle's) from the pair bounds on d sigma, the
+    weights' and the diagonal's sums over the column (the bound test_gpu_passes applies)."""
+    n = xy.shape[0]
+    i, j, dx, dy = all_pairs(xy)
+    e, c = orc.inputs(dx, dy)
+    ad, _ = pair_allowance(orc, dx, dy, e, c, deriv=True)
+    aV = np.zeros((n, n))
+    aV[i, j] = aV[j, i] = h * ad
+    col_allow = aV.sum(axis=0)
+    d = R["dvec"]
+    rows = R["Ai"]
+    cols = np.repeat(np.arange(n), np.diff(R["Ap"]))
+    Vabs = np.abs(R["Ax"])
+    bound = (1 + d[rows]) / 2 * aV[rows, cols] + Vabs * 0.5 * col_allow[rows] + 1e-13 * Vabs
+    diag = rows == cols
+    bound[diag] = col_allow[cols[diag]] + np.abs(R["Ax"][diag]) * (1e-13 + 0.5 * col_allow[cols[diag]])
+    return bound, 0.5 * col_allow + 1e-13

@@ -241,3 +241,36 @@ def test_ensemble_host_thread_change_keeps_gpu_solver(ctx):
     e.step(1)
     _check_equal(e, off, _plain(ctx, off, x, y, b, params, 4))
     e.close()
+
+
+def test_batch_passes_against_the_oracle(ctx, orc, pkg):
+    """Each realization of a batched speed pass and batched Jacobian pass against the oracle's
+    single-realization passes (orc_speed_pass, orc_jacobian_pass), to the bounds of the plain passes:
+    per row the sum of its pairs' bounds (B), per CSC entry the bound from the pairs' d sigma bounds;
+    the CSC structure bit for bit."""
+    from helpers import jacobian_allowance, speed_allowance
+    sizes = [2, 63, 200, 301, 64, 512]
+    off, xy = _ragged(sizes, 71)
+    tau = np.linspace(0.0, 0.1, len(sizes))
+    v = ens.batch_speed_pass(ctx, off, xy, tau)
+    kinds = [dict(cutoff_mult=1e20, heaviside=False, wfunc="c"), dict(cutoff_mult=0.05 * np.sqrt(63), heaviside=True),
+             dict(cutoff_mult=0.9, heaviside=False, wfunc="m"), dict(cutoff_mult=0.1 * np.sqrt(301), heaviside=True),
+             dict(cutoff_mult=1e20, heaviside=False, wfunc="p"), dict(cutoff_mult=0.05 * np.sqrt(512), heaviside=True)]
+    params = [pkg.jac_params(h=1e-3, tau_a=0.02, tau_b=0.05, n=n, **kinds[r]) for r, n in enumerate(sizes)]
+    vA, vB, (Ap, Ai, Ax), idx, dvec = ens.batch_jacobian_pass(ctx, off, xy, params)
+    for r, n in enumerate(sizes):
+        s = slice(off[r], off[r + 1])
+        x = xy[s]
+        allow = speed_allowance(orc, x)
+        assert np.all(np.abs(v[s] - orc.speed_pass(x, tau[r])) <= allow + 1e-15), r
+        p = params[r]
+        R = orc.jacobian_pass(x, p.h, p.tau_a, p.tau_b, p.cutoff, p.cut_sq, p.one_per_cut_sq, bool(p.heaviside),
+                              p.wfunc.decode())
+        base = Ap[off[r]]
+        np.testing.assert_array_equal(Ap[off[r]:off[r + 1] + 1] - base, R["Ap"])
+        np.testing.assert_array_equal(Ai[base:Ap[off[r + 1]]] - off[r], R["Ai"])
+        np.testing.assert_array_equal(idx[s] - base, R["indexes"])
+        bound, dbound = jacobian_allowance(orc, x, p.h, R)
+        assert np.all(np.abs(Ax[base:Ap[off[r + 1]]] - R["Ax"]) <= bound), r
+        assert np.all(np.abs(dvec[s] - R["dvec"]) <= dbound), r
+        assert np.all(np.abs(vA[s] - R["vA"]) <= allow + 1e-15) and np.all(np.abs(vB[s] - R["vB"]) <= allow + 1e-15)

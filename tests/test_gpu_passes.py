@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from helpers import (TOL, all_pairs, bsign, pair_allowance, random_config, scale_dsigma, scale_sigma,
-                     sort_reference, wrap)
+                     sort_reference, speed_allowance, wrap)
 
 pytestmark = pytest.mark.gpu
 GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -20,22 +20,6 @@ GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 def _dconf64():
     return np.load(os.path.join(GOLDEN_DIR, "dconf_golden.npz"))["xy"]
-
-
-def _row_sum(n, i, j, vi, vj):
-    out = np.zeros(n)
-    np.add.at(out, i, vi)
-    np.add.at(out, j, vj)
-    return out
-
-
-def speed_allowance(orc, xy):
-    """per-row bound for a speed pass (sum of per-pair bounds (B) + reordering)."""
-    n = xy.shape[0]
-    i, j, dx, dy = all_pairs(xy)
-    e, c = orc.inputs(dx, dy)
-    a, s = pair_allowance(orc, dx, dy, e, c)
-    return _row_sum(n, i, j, a, a) + 1e-13 * _row_sum(n, i, j, np.abs(s), np.abs(s))
 
 
 # ------------------------------------------------------------------ field
@@ -229,7 +213,7 @@ def test_jacobian_rescale_vs_oracle(pkg, ctx, orc):
 
 # ------------------------------------------------------------------ Jacobian (dense)
 
-@pytest.mark.parametrize("n", [64, 300])
+@pytest.mark.parametrize("n", [64, 300, 301])
 def test_jacobian_dense_vs_oracle(pkg, ctx, orc, n):
     xy = _dconf64() if n == 64 else random_config(n, 11)
     h = 1e-3
