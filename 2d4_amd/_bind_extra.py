@@ -58,6 +58,14 @@ SIGNATURES = {
     "d4_ens_get_config": (C.c_int, [_vp, _P(EnsConfig)]),
     "d4_ens_autotune": (C.c_int, [_vp, _P(EnsConfig), _P(_d)]),
     "d4_ens_timing": (C.c_int, [_vp, _vp]),
+    # include/d4_dstep.h
+    "d4_dstep_create": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp, C.c_int, _P(_vp)]),
+    "d4_dstep_destroy": (None, [_vp]),
+    "d4_dstep_last_error": (C.c_char_p, [_vp]),
+    "d4_dstep_set_graph": (C.c_int, [_vp, C.c_int]),
+    "d4_dstep_run": (C.c_int, [_vp, C.c_int, _pi]),
+    "d4_dstep_positions": (C.c_int, [_vp, _vp, C.c_int]),
+    "d4_dstep_stats": (C.c_int, [_vp, _vp]),
 }
 
 

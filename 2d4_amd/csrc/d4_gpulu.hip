@@ -595,6 +595,153 @@ bool valid_csc(const BatchDenseLU::Matrix& m)
     return true;
 }
 
+// The launch plan of the blocked no-pivot LU over matrices fd (largest first): per step p the number of
+// active matrices (a prefix), and offsets into pre of their trailing block counts T (tbo), of the panel
+// items' prefix sums (pre2, n2 items, padded to fours per matrix) and of the update tiles' prefix sums
+// (pre3, n3 tiles).  last[i]: matrix i's envelope (its last nonzero block per step) or null (dense).
+struct LuStep {
+    int act, tbo, pre2, n2, pre3, n3;
+};
+
+bool plan_lu_steps(const std::vector<FacDesc>& fd, const std::vector<const int*>& last, std::vector<LuStep>& steps,
+                   std::vector<int>& pre)
+{
+    const int Ka = (int)fd.size();
+    const int P = Ka ? fd[0].npad / kFB : 0;
+    steps.assign(P, LuStep{});
+    pre.clear();
+    std::vector<int> T(Ka);
+    for (int p = 0; p < P; ++p) {
+        int act = 0;
+        while (act < Ka && fd[act].npad > kFB * p) ++act;
+        LuStep& sp = steps[p];
+        sp.act = act;
+        sp.tbo = (int)pre.size();
+        for (int i = 0; i < act; ++i) {
+            T[i] = last[i] ? std::max(0, last[i][p] - p) : fd[i].npad / kFB - p - 1;
+            pre.push_back(T[i]);
+        }
+        sp.pre2 = (int)pre.size();
+        pre.push_back(0);
+        for (int i = 0; i < act; ++i) pre.push_back(pre.back() + (2 * T[i] + 3) / 4 * 4);
+        sp.n2 = pre.back();
+        sp.pre3 = (int)pre.size();
+        pre.push_back(0);
+        for (int i = 0; i < act; ++i) {
+            const long long t2 = (long long)T[i] * T[i];
+            if (pre.back() + t2 > INT_MAX) return false;
+            pre.push_back(pre.back() + (int)t2);
+        }
+        sp.n3 = pre.back();
+    }
+    return true;
+}
+
+int launch_lu_steps(const std::vector<LuStep>& steps, const FacDesc* fdesc, const int* pre, double* A, hipStream_t st)
+{
+    for (size_t p = 0; p < steps.size(); ++p) {
+        const LuStep& sp = steps[p];
+        const int j0 = kFB * (int)p;
+        hipLaunchKernelGGL(lu_diag_kernel, dim3(sp.act), dim3(kFB), 0, st, fdesc, A, j0);
+        if (sp.n2 > 0)
+            hipLaunchKernelGGL(lu_panel_kernel, dim3(sp.n2 / 4), dim3(256), 0, st, fdesc, pre + sp.pre2,
+                               pre + sp.tbo, sp.act, A, j0);
+        if (sp.n3 > 0)
+            hipLaunchKernelGGL(lu_update_kernel, dim3(sp.n3), dim3(256), 0, st, fdesc, pre + sp.pre3, pre + sp.tbo,
+                               sp.act, A, j0);
+    }
+    return hipGetLastError() != hipSuccess ? -1 : 0;
+}
+
+// Partial pivoting LU (LAPACK getf2's order: the first row of largest magnitude in the column, the rows
+// swapped across every column, multipliers by division) of each matrix whose flag is set, one workgroup
+// per matrix, in place in its store; perm (npad entries per matrix, at its solve descriptor's perm_off)
+// receives its solve's row order, the identity for the matrices whose flag is clear, and singular[k]
+// whether a zero pivot was met.  The rare fallback of a factorization whose multipliers exceed the
+// threshold: a latency chain of n steps, but only launches, so it can sit in a captured graph.
+constexpr int kPivThreads = 1024;
+
+__global__ __launch_bounds__(kPivThreads) void lu_pivoted_kernel(const MatDesc* __restrict__ desc,
+                                                                 const SolveDesc* __restrict__ sdesc,
+                                                                 double* __restrict__ A, const int* __restrict__ flag,
+                                                                 int* __restrict__ perm, int* __restrict__ singular)
+{
+    extern __shared__ int piv[];  // npad swaps, then the permutation
+    __shared__ double rv[kPivThreads / 64];
+    __shared__ int ri[kPivThreads / 64];
+    const MatDesc m = desc[blockIdx.x];
+    const int n = m.n, npad = m.npad, tid = threadIdx.x;
+    int* pm = perm + sdesc[blockIdx.x].perm_off;
+    if (!flag[blockIdx.x]) {
+        for (int i = tid; i < npad; i += kPivThreads) pm[i] = i;
+        if (tid == 0) singular[blockIdx.x] = 0;
+        return;
+    }
+    double* a = A + m.a_off;
+    const size_t ld = (size_t)npad;
+    int sing = 0;
+    for (int k = 0; k < n; ++k) {
+        // the pivot: largest |a_rk|, r >= k, the first one on ties
+        double bv = -1.0;
+        int bi = n;
+        for (int r = k + tid; r < n; r += kPivThreads) {
+            const double v = std::fabs(a[(size_t)k * ld + r]);
+            if (v > bv) { bv = v; bi = r; }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ov = __shfl_down(bv, o);
+            const int oi = __shfl_down(bi, o);
+            if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        }
+        if ((tid & 63) == 0) { rv[tid >> 6] = bv; ri[tid >> 6] = bi; }
+        __syncthreads();
+        if (tid == 0) {
+            for (int w = 1; w < kPivThreads / 64; ++w)
+                if (rv[w] > rv[0] || (rv[w] == rv[0] && ri[w] < ri[0])) { rv[0] = rv[w]; ri[0] = ri[w]; }
+            piv[k] = ri[0] < n ? ri[0] : k;
+        }
+        __syncthreads();
+        const int p = piv[k];
+        if (p != k)
+            for (int c = tid; c < n; c += kPivThreads) {
+                const double t = a[(size_t)c * ld + k];
+                a[(size_t)c * ld + k] = a[(size_t)c * ld + p];
+                a[(size_t)c * ld + p] = t;
+            }
+        __syncthreads();
+        const double d = a[(size_t)k * ld + k];
+        if (d == 0.0) {
+            sing = 1;
+            __syncthreads();
+            continue;
+        }
+        for (int r = k + 1 + tid; r < n; r += kPivThreads) a[(size_t)k * ld + r] /= d;
+        __syncthreads();
+        for (int c = k + 1; c < n; ++c) {
+            const double akc = a[(size_t)c * ld + k];
+            if (akc == 0.0) continue;
+            for (int r = k + 1 + tid; r < n; r += kPivThreads) a[(size_t)c * ld + r] -= a[(size_t)k * ld + r] * akc;
+        }
+        __syncthreads();
+    }
+    // the swap sequence into the permutation (rows beyond n stay in place)
+    int* q = piv + npad;
+    for (int i = tid; i < npad; i += kPivThreads) q[i] = i;
+    __syncthreads();
+    if (tid == 0)
+        for (int i = 0; i < n; ++i) {
+            const int j = piv[i];
+            if (j != i) {
+                const int t = q[i];
+                q[i] = q[j];
+                q[j] = t;
+            }
+        }
+    __syncthreads();
+    for (int i = tid; i < npad; i += kPivThreads) pm[i] = q[i];
+    if (tid == 0) singular[blockIdx.x] = sing;
+}
+
 class BatchLU : public BatchDenseLU {
 public:
     explicit BatchLU(int device) : device_(device)
@@ -970,53 +1117,15 @@ private:
                 fd.push_back({desc_h_[k].a_off, desc_h_[k].npad, 0});
                 last.push_back(desc_h_[k].env_off >= 0 ? plan_[k].env.last.data() : nullptr);
             }
-        const int Ka = (int)fd.size();
-        if (Ka == 0) return 0;
-        const int P = fd[0].npad / kFB;
-        struct Step {
-            int act, tbo, pre2, n2, pre3, n3;
-        };
-        std::vector<Step> steps(P);
-        std::vector<int> pre, T(Ka);
-        for (int p = 0; p < P; ++p) {
-            int act = 0;
-            while (act < Ka && fd[act].npad > kFB * p) ++act;
-            Step& sp = steps[p];
-            sp.act = act;
-            sp.tbo = (int)pre.size();
-            for (int i = 0; i < act; ++i) {
-                T[i] = last[i] ? std::max(0, last[i][p] - p) : fd[i].npad / kFB - p - 1;
-                pre.push_back(T[i]);
-            }
-            sp.pre2 = (int)pre.size();
-            pre.push_back(0);
-            for (int i = 0; i < act; ++i) pre.push_back(pre.back() + (2 * T[i] + 3) / 4 * 4);
-            sp.n2 = pre.back();
-            sp.pre3 = (int)pre.size();
-            pre.push_back(0);
-            for (int i = 0; i < act; ++i) {
-                const long long t2 = (long long)T[i] * T[i];
-                if (pre.back() + t2 > INT_MAX) return -1;
-                pre.push_back(pre.back() + (int)t2);
-            }
-            sp.n3 = pre.back();
-        }
-        if (!fdesc_.ensure(Ka) || !fpre_.ensure(pre.size())) return -1;
-        if (hipMemcpyAsync(fdesc_.p, fd.data(), sizeof(FacDesc) * Ka, hipMemcpyHostToDevice, st) != hipSuccess ||
+        if (fd.empty()) return 0;
+        std::vector<LuStep> steps;
+        std::vector<int> pre;
+        if (!plan_lu_steps(fd, last, steps, pre)) return -1;
+        if (!fdesc_.ensure(fd.size()) || !fpre_.ensure(pre.size())) return -1;
+        if (hipMemcpyAsync(fdesc_.p, fd.data(), sizeof(FacDesc) * fd.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
             hipMemcpyAsync(fpre_.p, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice, st) != hipSuccess)
             return -1;
-        for (int p = 0; p < P; ++p) {
-            const Step& sp = steps[p];
-            const int j0 = kFB * p;
-            hipLaunchKernelGGL(lu_diag_kernel, dim3(sp.act), dim3(kFB), 0, st, fdesc_.p, A_.p, j0);
-            if (sp.n2 > 0)
-                hipLaunchKernelGGL(lu_panel_kernel, dim3(sp.n2 / 4), dim3(256), 0, st, fdesc_.p,
-                                   fpre_.p + sp.pre2, fpre_.p + sp.tbo, sp.act, A_.p, j0);
-            if (sp.n3 > 0)
-                hipLaunchKernelGGL(lu_update_kernel, dim3(sp.n3), dim3(256), 0, st, fdesc_.p, fpre_.p + sp.pre3,
-                                   fpre_.p + sp.tbo, sp.act, A_.p, j0);
-        }
-        return hipGetLastError() != hipSuccess ? -1 : 0;
+        return launch_lu_steps(steps, fdesc_.p, fpre_.p, A_.p, st);
     }
 
     bool ensure_side()
@@ -1146,7 +1255,129 @@ private:
     bool singular_ = false, unpivoted_ = false;
 };
 
+// The dense LU of a fixed batch of matrices by launches alone (see DenseLUGraph in d4_solver.h): the
+// plans, descriptors and device arrays are made once here, so factor and solve allocate nothing, copy
+// nothing from the host and never wait for the device.
+class GraphLU : public DenseLUGraph {
+public:
+    GraphLU() = default;
+    ~GraphLU() override
+    {
+        for (void* p : {(void*)fd_d_, (void*)pre_d_, (void*)md_d_, (void*)sd_d_, (void*)perm_d_, (void*)scratch_d_})
+            if (p) (void)hipFree(p);
+    }
+
+    int init(int device, int K, const int* n, const long long* b_off)
+    {
+        if (K <= 0) return -1;
+        if (hipSetDevice(device) != hipSuccess) return -2;
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lds = 65536;
+        K_ = K;
+        std::vector<MatDesc> md(K);
+        std::vector<SolveDesc> sd(K);
+        a_off_.resize(K);
+        long long a = 0, poff = 0, btot = 0;
+        for (int k = 0; k < K; ++k) {
+            const int npad = gpu_lu_padded(n[k]);
+            if (n[k] <= 0 || npad * 8 > lds) return -1;  // the fallback keeps 2 npad ints in LDS
+            MatDesc& m = md[k];
+            m = MatDesc{};
+            m.a_off = a_off_[k] = a;
+            m.n = n[k];
+            m.npad = npad;
+            m.ipiv_off = m.ord_off = m.env_off = -1;
+            sd[k] = SolveDesc{a, n[k], npad, (int)b_off[k], (int)poff, -1, -1};
+            a += (long long)npad * npad;
+            poff += npad;
+            btot = std::max(btot, b_off[k] + n[k]);
+            npad_max_ = std::max(npad_max_, npad);
+        }
+        if (poff > INT_MAX || btot > INT_MAX) return -1;
+        store_ = a;
+        btot_ = btot;
+        // the factorization plan: matrices largest first, each dense
+        std::vector<int> order(K);
+        for (int k = 0; k < K; ++k) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return md[x].npad > md[y].npad; });
+        std::vector<FacDesc> fd;
+        for (int k : order) fd.push_back({md[k].a_off, md[k].npad, 0});
+        std::vector<const int*> last(K, nullptr);
+        std::vector<int> pre;
+        if (!plan_lu_steps(fd, last, steps_, pre)) return -1;
+        lds_order_ = std::max(0, (lds / 8 - 2 * kSB * kSB) / 256 * 256);
+        const size_t scratch = npad_max_ <= lds_order_ ? 1 : (size_t)btot;
+        if (hipMalloc(&fd_d_, sizeof(FacDesc) * fd.size()) != hipSuccess ||
+            hipMalloc(&pre_d_, sizeof(int) * std::max<size_t>(pre.size(), 1)) != hipSuccess ||
+            hipMalloc(&md_d_, sizeof(MatDesc) * K) != hipSuccess || hipMalloc(&sd_d_, sizeof(SolveDesc) * K) != hipSuccess ||
+            hipMalloc(&perm_d_, sizeof(int) * (size_t)poff) != hipSuccess ||
+            hipMalloc(&scratch_d_, sizeof(double) * scratch) != hipSuccess)
+            return -2;
+        if (hipMemcpy(fd_d_, fd.data(), sizeof(FacDesc) * fd.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            (!pre.empty() && hipMemcpy(pre_d_, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+            hipMemcpy(md_d_, md.data(), sizeof(MatDesc) * K, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(sd_d_, sd.data(), sizeof(SolveDesc) * K, hipMemcpyHostToDevice) != hipSuccess)
+            return -2;
+        return 0;
+    }
+
+    long long a_off(int k) const override { return a_off_[k]; }
+    long long store_size() const override { return store_; }
+
+    int factor_unpivoted(double* store, int* flag, hipStream_t st) override
+    {
+        if (d4::launch_fill_int(flag, 0, K_, st) != hipSuccess) return -2;
+        if (launch_lu_steps(steps_, fd_d_, pre_d_, store, st)) return -2;
+        for (int k0 = 0; k0 < K_; k0 += kGridY)
+            hipLaunchKernelGGL(multiplier_check_kernel, dim3(npad_max_, std::min(kGridY, K_ - k0)), dim3(256), 0, st,
+                               md_d_ + k0, store, flag + k0, nullptr);
+        return hipGetLastError() != hipSuccess ? -2 : 0;
+    }
+
+    int factor_pivoted(double* store, const int* flag, int* singular, hipStream_t st) override
+    {
+        hipLaunchKernelGGL(lu_pivoted_kernel, dim3(K_), dim3(kPivThreads), sizeof(int) * 2 * npad_max_, st, md_d_,
+                           sd_d_, store, flag, perm_d_, singular);
+        return hipGetLastError() != hipSuccess ? -2 : 0;
+    }
+
+    int solve(const double* store, double* b, hipStream_t st) override
+    {
+        if (npad_max_ <= lds_order_)
+            hipLaunchKernelGGL(lu_solve_kernel<true>, dim3(K_), dim3(kSolveThreads),
+                               sizeof(double) * (2 * kSB * kSB + npad_max_), st, sd_d_, store, perm_d_, b, scratch_d_,
+                               nullptr);
+        else
+            hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(K_), dim3(kSolveThreads), sizeof(double) * 2 * kSB * kSB,
+                               st, sd_d_, store, perm_d_, b, scratch_d_, nullptr);
+        return hipGetLastError() != hipSuccess ? -2 : 0;
+    }
+
+private:
+    static constexpr int kGridY = 65535;
+    int K_ = 0, npad_max_ = 0, lds_order_ = 0;
+    long long store_ = 0, btot_ = 0;
+    std::vector<long long> a_off_;
+    std::vector<LuStep> steps_;
+    FacDesc* fd_d_ = nullptr;
+    int* pre_d_ = nullptr;
+    MatDesc* md_d_ = nullptr;
+    SolveDesc* sd_d_ = nullptr;
+    int* perm_d_ = nullptr;
+    double* scratch_d_ = nullptr;
+};
+
 }  // namespace
+
+DenseLUGraph* new_dense_lu_graph(int device, int K, const int* n, const long long* b_off)
+{
+    GraphLU* g = new GraphLU();
+    if (g->init(device, K, n, b_off) != 0) {
+        delete g;
+        return nullptr;
+    }
+    return g;
+}
 
 LinearSolver* new_gpu_dense_lu(int device) { return new GpuDenseLU(device); }
 BatchDenseLU* new_batch_dense_lu(int device) { return new BatchLU(device); }

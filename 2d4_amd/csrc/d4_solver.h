@@ -29,6 +29,10 @@
 #include <cstdint>
 #include <vector>
 
+// HIP's stream handle (hip_runtime_api.h declares the same), so that host-only builds of this header
+// need no HIP headers
+typedef struct ihipStream_t* hipStream_t;
+
 namespace d4host {
 
 // True on the ensemble's host worker threads.  Those threads only move host
@@ -185,5 +189,25 @@ public:
     virtual int solve_dense_dev(int n, const double* b, double* x) = 0;
 };
 BatchDenseLU* new_batch_dense_lu(int device);
+
+// The same dense LU for a batch of matrices whose orders are fixed once, by launches alone: no
+// allocation, no host copy and no wait after creation, so that a caller can capture factor and solve in
+// a HIP graph.  Matrix k is column-major of order gpu_lu_padded(n[k]) at store + a_off(k) (the caller
+// assembles it there, identity beyond n[k]).  Natural order (no reordering), threshold partial pivoting
+// with u = 0.001: factor_unpivoted() factors every matrix without interchanges and sets flag[k] where a
+// pivot is not finite and nonzero or a multiplier exceeds 1000; the caller then assembles the flagged
+// matrices again and factor_pivoted() factors them with partial pivoting (singular[k]: a zero pivot was
+// met); solve() overwrites each right-hand side, packed at b_off[k], with its solution.  Null if the
+// batch does not fit (an order above the device's LDS bound of the fallback) or the device fails.
+class DenseLUGraph {
+public:
+    virtual ~DenseLUGraph() = default;
+    virtual long long a_off(int k) const = 0;
+    virtual long long store_size() const = 0;  // doubles
+    virtual int factor_unpivoted(double* store, int* flag, hipStream_t st) = 0;
+    virtual int factor_pivoted(double* store, const int* flag, int* singular, hipStream_t st) = 0;
+    virtual int solve(const double* store, double* b, hipStream_t st) = 0;
+};
+DenseLUGraph* new_dense_lu_graph(int device, int K, const int* n, const long long* b_off);
 
 }  // namespace d4host
