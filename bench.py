@@ -221,6 +221,87 @@ def implicit_step_bench(pkg, ctx, gpu_steps=50, cpu_steps=10, with_cpu=True, see
     return out
 
 
+def device_step_bench(pkg, ctx, iters=32):
+    """The implicit step resident on the device (include/d4_dstep.h: every stage a launch, one iteration
+    a HIP graph replay) on the N = 1024 workload of implicit_step_bench (heaviside cutoff 0.05,
+    tau = 0.1, adaptive), for one realization and for a batch of 64, beside the host-driven batched
+    ensemble (include/d4_batch.h, GPU LU) on the same 64 realizations."""
+    dstep = importlib.import_module("2d4_amd.dstep")
+    ens = importlib.import_module("2d4_amd.ensemble")
+    n = 1024
+    out = {"workload": "N=1024 per realization, uniform, heaviside cutoff 0.05, tau=0.1, adaptive (C2)",
+           "matrix": "dense n x n per realization, GPU dense LU in natural order (u = 0.001 threshold pivoting)"}
+    for R in (1, 64):
+        rng = np.random.default_rng(77)
+        x = rng.uniform(-0.5, 0.5, R * n)
+        y = rng.uniform(-0.5, 0.5, R * n)
+        b = np.tile(np.where(np.arange(n) % 2 == 0, -1.0, 1.0), R)
+        off = (np.arange(R + 1) * n).astype(np.int32)
+        params = dict(cutoff_mult=0.05 * np.sqrt(n), heaviside=True, stress=("fixed", 0.1, 0.0))
+        s = dstep.DeviceSimulation(ctx, off, x, y, b, params)
+        s.run(2)
+        t0 = time.perf_counter()
+        s.run(iters)
+        t = time.perf_counter() - t0
+        st = s.stats()
+        s.set_graph(False)
+        t1 = time.perf_counter()
+        s.run(8)
+        te = (time.perf_counter() - t1) / 8
+        s.close()
+        row = {"ms_per_iteration": t * 1e3 / iters, "realization_steps_per_s": R * iters / t,
+               "eager_ms_per_iteration": te * 1e3, "iterations": iters,
+               "accepted": int(sum(q["successful_steps"] for q in st)), "failed": int(sum(q["failed_steps"] for q in st))}
+        if R > 1:
+            e = ens.Ensemble(ctx, off, x, y, b, [dict(params, solver=1) for _ in range(R)])
+            e.step(2)
+            t0 = time.perf_counter()
+            e.step(8)
+            row["host_driven_ensemble_realization_steps_per_s"] = R * 8 / (time.perf_counter() - t0)
+            e.close()
+        out[f"R{R}"] = row
+    return out
+
+
+def c5_reference_step_check(T=4, n=2048, eps=1e-9, coord=7, seed=2048):
+    """The C5 configuration (N = 2048 uniform, tau = 0.2, fixed dt = 1e-3: -P 1e10 -p 0 -M 1e-3
+    --initial-stepsize 1e-3) run through the reference's own step (the device-resident loop, pinned to
+    the oracle's loop step by step): the accumulated strain, the largest wrapped position change per
+    step and the central difference of the strain in x_coord, with the start and its two perturbations
+    by +-eps run as one batch of three realizations."""
+    dstep = importlib.import_module("2d4_amd.dstep")
+    pkg = importlib.import_module("2d4_amd")
+    ctx = pkg.Context(0)
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-0.5, 0.5, n)
+    y = rng.uniform(-0.5, 0.5, n)
+    b = np.where(np.arange(n) % 2 == 0, -1.0, 1.0)
+    xs = np.concatenate([x, x, x])
+    xs[n + coord] += eps
+    xs[2 * n + coord] -= eps
+    p = dict(init_step_size=1e-3, max_step_size=1e-3, prec=1e10, dipole_prec=0.0, stress=("fixed", 0.2, 0.0))
+    s = dstep.DeviceSimulation(ctx, [0, n, 2 * n, 3 * n], xs, np.tile(y, 3), np.tile(b, 3), p)
+    prev = s.positions()[:n, 0]
+    rows = []
+    t0 = time.perf_counter()
+    for k in range(1, T + 1):
+        s.run(1)
+        pos = s.positions()
+        st = s.stats()
+        a = pos[:n, 0]
+        step = float(np.abs((a - prev + 0.5) % 1 - 0.5).max())
+        dev = float(np.abs((pos[n:2 * n, 0] - a + 0.5) % 1 - 0.5).max())
+        rows.append({"step": k, "strain": st[0]["strain"], "max_wrapped_step_change": step,
+                     "max_deviation_of_the_perturbed_run": dev,
+                     "central_difference_dstrain_dx": (st[1]["strain"] - st[2]["strain"]) / (2 * eps),
+                     "unpivoted_factorizations": st[0]["unpivoted_factorizations"],
+                     "factorizations": st[0]["factorizations"], "seconds": time.perf_counter() - t0})
+        log(f"[bench] C5 reference step {k}: {rows[-1]}")
+        prev = a
+    s.close()
+    return {"n": n, "tau": 0.2, "dt": 1e-3, "eps": eps, "coordinate": coord, "steps": rows}
+
+
 def adjoint_bench(n, T, dt, every):
     """Strain gradient through T backward-Euler steps (2d4_amd/adjoint.py) at
     N = n: times the forward pass, the checkpointed forward and the backward."""
@@ -288,6 +369,8 @@ def main():
                     help="only the batched ensemble (no plain comparison, no pair-pass headline), for profiling")
     ap.add_argument("--adjoint", type=str, default="",
                     help="only the adjoint measurement: N,T,dt (e.g. 2048,64,1e-5), checkpoints every 16 steps")
+    ap.add_argument("--c5-check", type=int, default=0,
+                    help="only the C5 configuration through the reference's own step for this many steps")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -305,6 +388,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calib_copy_bytes": nel * 8}), flush=True)
+        return
+    if args.c5_check:
+        print(json.dumps({"c5_reference_step": c5_reference_step_check(args.c5_check)}), flush=True)
         return
     if args.adjoint:
         an, aT, adt = args.adjoint.split(",")
@@ -416,9 +502,12 @@ def main():
     if not args.no_ens:
         ensemble = ensemble_bench(pkg, ctx, args.ens_steps, args.ens_r)
     step = None
+    dstep_res = None
     if not args.no_step:
         log("[bench] implicit step, N=1024 (GPU, then the CPU loop)")
         step = implicit_step_bench(pkg, ctx, with_cpu=not args.no_cpu)
+        log("[bench] device-resident implicit step, N=1024, 1 and 64 realizations")
+        dstep_res = device_step_bench(pkg, ctx)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_pair_dense_n16384.json")
@@ -461,6 +550,7 @@ def main():
         "cpu_baseline": cpu,
         "ensemble": ensemble,
         "implicit_step": step,
+        "device_step": dstep_res,
         "extra": extra,
     }
     print(json.dumps(line), flush=True)
