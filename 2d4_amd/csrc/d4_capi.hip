@@ -44,8 +44,8 @@ d4_ctx::~d4_ctx()
 {
     if (batch) batch_passes_free(batch);
     if (lu) batch_lu_free(lu);
-    for (auto* b : {&part, &tiles, &xy, &v0, &v1, &tol, &coo, &coo_count, &cnt, &cursor, &Ap, &Ai, &Ax, &idx, &dvec,
-                    &nw, &aux0, &aux1, &aux2})
+    for (auto* b : {&part, &tiles, &tiles_off, &xy, &v0, &v1, &tol, &coo, &coo_count, &cnt, &cursor, &Ap, &Ai, &Ax, &idx, &dvec,
+                    &nw, &aux0, &aux1, &aux2, &tab_rec})
         b->release();
     for (auto& m : ev_marks) {
         ev_pool.push_back(m.first);
@@ -120,22 +120,44 @@ int d4_ctx::mark_end(hipEvent_t b)
 }
 
 // Upper-triangle tile table of one realization of n (row-major, slot = index).
-int d4_ctx::single_tiles(int n, int* ntiles)
+int d4_ctx::single_tiles(int n, int* ntiles, bool by_offset)
 {
     const int nb = (n + kTile - 1) / kTile;
     const int nt = nb * (nb + 1) / 2;
     *ntiles = nt;
-    if (tiles_n == n) return D4_OK;
+    DevBuf& buf = by_offset ? tiles_off : tiles;
+    int& cached = by_offset ? tiles_off_n : tiles_n;
+    if (cached == n) return D4_OK;
+    // slot = the tile's row-major index in the upper triangle, whatever the launch order: partial sums
+    // land in the same places and the reduction adds them in the same order, so both orders give the
+    // same bits
+    auto slot_of = [nb](int I, int J) { return I * nb - I * (I - 1) / 2 + (J - I); };
     std::vector<TileDesc> t;
     t.reserve(nt);
-    int slot = 0;
-    for (int I = 0; I < nb; ++I)
-        for (int J = I; J < nb; ++J) t.push_back(TileDesc{0, I * kTile, J * kTile, slot++});
-    int rc = ensure(tiles, sizeof(TileDesc) * (size_t)nt, "tile table");
+    if (!by_offset) {
+        for (int I = 0; I < nb; ++I)
+            for (int J = I; J < nb; ++J) t.push_back(TileDesc{0, I * kTile, J * kTile, slot_of(I, J)});
+    } else {
+        // tiles listed by their offset J - I, and the list cut into kXcds contiguous runs, run x taking
+        // the launch positions p with p % kXcds == x (the order in which workgroups are handed to the
+        // XCDs): each XCD walks its run in order, so the tiles it holds at once lie on one or two offsets
+        std::vector<TileDesc> diag;
+        diag.reserve(nt);
+        for (int d = 0; d < nb; ++d)
+            for (int I = 0; I + d < nb; ++I) diag.push_back(TileDesc{0, I * kTile, (I + d) * kTile, slot_of(I, I + d)});
+        t.resize(nt);
+        int start = 0;
+        for (int x = 0; x < kXcds; ++x) {
+            const int cnt = nt / kXcds + (x < nt % kXcds ? 1 : 0);
+            for (int k = 0; k < cnt; ++k) t[(size_t)k * kXcds + x] = diag[(size_t)start + k];
+            start += cnt;
+        }
+    }
+    int rc = ensure(buf, sizeof(TileDesc) * (size_t)nt, "tile table");
     if (rc) return rc;
-    hipError_t e = hipMemcpy(tiles.p, t.data(), sizeof(TileDesc) * (size_t)nt, hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpy(buf.p, t.data(), sizeof(TileDesc) * (size_t)nt, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "tile table upload");
-    tiles_n = n;
+    cached = n;
     return D4_OK;
 }
 
@@ -146,7 +168,11 @@ int d4_ctx::single_pass(int mode, int n, const double* d_xy, const RealParams& r
                         int exact, int want_dy, int add_tau)
 {
     int ntiles = 0;
-    int rc = single_tiles(n, &ntiles);
+    // the tabulated field's passes read cells at the pairs' separations: launched by tile offset, the
+    // tiles an XCD holds at once read the same few table rows (positions sorted along y, as
+    // SimulationData sorts them, give every tile of one offset about the same separation in y)
+    const bool by_offset = mode == kModeSpeedTab || mode == kModeJacDenseTab;
+    int rc = single_tiles(n, &ntiles, by_offset);
     if (rc) return rc;
     const int nq = pass_nq(mode);
     rc = ensure(part, sizeof(double) * 128 * (size_t)nq * (size_t)ntiles, "partials");
@@ -155,7 +181,7 @@ int d4_ctx::single_pass(int mode, int n, const double* d_xy, const RealParams& r
     a.xy = reinterpret_cast<const double2*>(d_xy);
     a.off = nullptr;
     a.n0 = n;
-    a.tiles = static_cast<const TileDesc*>(tiles.p);
+    a.tiles = static_cast<const TileDesc*>(by_offset ? tiles_off.p : tiles.p);
     a.ntiles = ntiles;
     a.rp = nullptr;
     a.rp0 = rp;
@@ -762,7 +788,11 @@ int d4_speed_pass_table_dev(d4_ctx* ctx, int n, const double* xy, const double* 
 {
     if (!ctx || n < 0 || !table_size_ok(tab_n) || (n > 0 && (!xy || !v || !tab_s))) return D4_ERR_INVALID;
     if (n == 0) return D4_OK;
-    ctx->tab_s = tab_s;
+    // the table as cell records (one 32-byte record per pair instead of four scattered nodes)
+    D4_TRY(ctx->ensure(ctx->tab_rec, sizeof(double) * 4 * (size_t)tab_n * tab_n, "table cell records"));
+    hipError_t e = launch_table_records(tab_n, tab_s, nullptr, static_cast<double*>(ctx->tab_rec.p), ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "table cell records");
+    ctx->tab_s = static_cast<const double*>(ctx->tab_rec.p);
     ctx->tab_d = nullptr;
     ctx->tab_n = tab_n;
     RealParams rp = real_params(n, nullptr, tau);
@@ -779,8 +809,12 @@ int d4_jacobian_dense_table_dev(d4_ctx* ctx, int n, const double* xy, const d4_j
     if (!ctx || !p || n < 0 || !table_size_ok(tab_n) || (n > 0 && (!xy || !vA || !M || ld < n || !tab_s || !tab_d)))
         return D4_ERR_INVALID;
     if (n == 0) return D4_OK;
-    ctx->tab_s = tab_s;
-    ctx->tab_d = tab_d;
+    // sigma's and d sigma's corners of a cell in one 64-byte record
+    D4_TRY(ctx->ensure(ctx->tab_rec, sizeof(double) * 8 * (size_t)tab_n * tab_n, "table cell records"));
+    hipError_t e = launch_table_records(tab_n, tab_s, tab_d, static_cast<double*>(ctx->tab_rec.p), ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "table cell records");
+    ctx->tab_s = static_cast<const double*>(ctx->tab_rec.p);
+    ctx->tab_d = nullptr;
     ctx->tab_n = tab_n;
     RealParams rp = real_params(n, p, 0.0);
     rp.tol_cap2 = 0.0;

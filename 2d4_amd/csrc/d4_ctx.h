@@ -62,8 +62,10 @@ struct d4_ctx {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     std::string err;
-    int tiles_n = -1;  // n of the cached single-realization tile table
-    DevBuf part, tiles, xy, v0, v1, tol, coo, coo_count, cnt, cursor, Ap, Ai, Ax, idx, dvec, nw, aux0, aux1, aux2;
+    int tiles_n = -1;      // n of the cached single-realization tile table (row-major)
+    int tiles_off_n = -1;  // n of the cached table in offset order (single_tiles)
+    DevBuf part, tiles, tiles_off, xy, v0, v1, tol, coo, coo_count, cnt, cursor, Ap, Ai, Ax, idx, dvec, nw, aux0, aux1, aux2,
+        tab_rec;
     // pair-kernel timing: event pairs recorded around launches while profiling is on
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -74,7 +76,7 @@ struct d4_ctx {
     BatchPasses* batch = nullptr;  // tables and buffers of d4_batch_* (d4_batch.hip)
     d4host::BatchDenseLU* lu = nullptr;  // d4_lu_solve_batch's solver (d4_gpulu.hip)
     // tabulated field for the table pass modes (set around single_pass)
-    const double* tab_s = nullptr;
+    const double* tab_s = nullptr;  // cell records (launch_table_records)
     const double* tab_d = nullptr;
     int tab_n = 0;
     hipEvent_t take_event();
@@ -85,7 +87,7 @@ struct d4_ctx {
     int fail(int code, const std::string& msg);
     int hip_fail(hipError_t e, const char* where);
     int ensure(DevBuf& b, size_t bytes, const char* what);
-    int single_tiles(int n, int* ntiles);
+    int single_tiles(int n, int* ntiles, bool by_offset = false);
     int single_pass(int mode, int n, const double* d_xy, const d4::RealParams& rp, double* d_tol, double* d_vA,
                     double* d_vB, double* d_q1, double* d_dvec, double* d_M, int64_t ld, const double* d_w, int exact,
                     int want_dy, int add_tau);

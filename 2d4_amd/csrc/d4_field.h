@@ -459,4 +459,27 @@ D4_HD double table_bilinear(const double* tab, int n, double dx, double dy)
     return (1.0 - ax) * ((1.0 - ay) * t00 + ay * t01) + ax * ((1.0 - ay) * t10 + ay * t11);
 }
 
+// The same interpolation from cell records: record (ky * n + kx) holds the cell's corner values
+// {t(kx, ky), t(kx+1, ky), t(kx, ky+1), t(kx+1, ky+1)} (wrapped), so one pair reads one 32-byte record
+// (two loads in one cache line) instead of four scattered nodes; the arithmetic is table_bilinear's, so
+// the value is the same bits.  The cell of (dx, dy) and its weights are returned for a second quantity
+// stored in the same record (t = rec + 4).
+D4_HD double bilinear_cell(const double* t, double ax, double ay)
+{
+#pragma clang fp contract(off)
+    return (1.0 - ax) * ((1.0 - ay) * t[0] + ay * t[2]) + ax * ((1.0 - ay) * t[1] + ay * t[3]);
+}
+
+D4_HD const double* table_cell(const double* rec, int stride, int n, double dx, double dy, double& ax, double& ay)
+{
+#pragma clang fp contract(off)
+    const double u = (dx + 0.5) * n;
+    const double w = (dy + 0.5) * n;
+    const double fu = floor(u), fw = floor(w);
+    ax = u - fu;
+    ay = w - fw;
+    const int i0 = ((int)fu) & (n - 1), j0 = ((int)fw) & (n - 1);
+    return rec + ((size_t)j0 * n + i0) * stride;
+}
+
 }  // namespace d4

@@ -9,6 +9,7 @@
 namespace d4 {
 
 constexpr int kTile = 64;  // one wavefront per 64 x 64 block of unordered pairs
+constexpr int kXcds = 8;   // MI355X: workgroups are handed to its 8 XCDs round-robin by workgroup index
 
 // One tile of a (possibly ragged, batched) pass: realization r, block rows
 // [I0, I0+64) x block columns [J0, J0+64) of that realization, I0 <= J0.
@@ -74,7 +75,8 @@ struct PassArgs {
     int coo_cap;
     // VJP input (mode 3)
     const double* w;
-    // tabulated field (modes 5, 6): tab_n x tab_n node values over [-1/2, 1/2)^2
+    // tabulated field (modes 5, 6): tab_n x tab_n cells over [-1/2, 1/2)^2 as cell records in tab_s
+    // (launch_table_records): sigma's four corners (mode 5: 4 doubles), then d sigma's (mode 6: 8 doubles)
     const double* tab_s;
     const double* tab_d;
     int tab_n;
@@ -109,6 +111,9 @@ hipError_t launch_pair_pass(int mode, const PassArgs& a, hipStream_t s, int grid
 // Node values of the tabulated field: tab[ky * n + kx] = sigma (and d sigma/dx)
 // at (dx, dy) = (-1/2 + kx/n, -1/2 + ky/n), from the analytic field (0 at the origin).
 hipError_t launch_table_build(int n, double* tab_s, double* tab_d, hipStream_t s);
+// cell records of node tables: rec[(ky n + kx) stride + q] = tab_s corners q = 0..3 ((kx, ky), (kx+1, ky),
+// (kx, ky+1), (kx+1, ky+1), wrapped), then with tab_d (stride 8) tab_d's corners
+hipError_t launch_table_records(int n, const double* tab_s, const double* tab_d, double* rec, hipStream_t s);
 // bilinear values of the tables for m independent pairs
 hipError_t launch_table_eval(int64_t m, const double* dx, const double* dy, const double* tab_s, const double* tab_d,
                              int tab_n, double* s_out, double* d_out, hipStream_t s);

@@ -50,6 +50,26 @@ __device__ __forceinline__ double flip_if(double v, bool neg)
     return __longlong_as_double(__double_as_longlong(v) ^ ((long long)neg << 63));
 }
 
+// the value of the neighbouring lane (lanes 2k and 2k + 1 swap), by DPP quad_perm [1, 0, 3, 2]; every
+// lane of the wavefront must be active
+__device__ __forceinline__ unsigned long long swap_adjacent_u64(unsigned long long v)
+{
+    constexpr int kQuadSwap = 1 | (0 << 2) | (3 << 4) | (2 << 6);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)v, kQuadSwap, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), kQuadSwap, 0xF, 0xF, false);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
+__device__ __forceinline__ double swap_adjacent(double v)
+{
+    return __longlong_as_double((long long)swap_adjacent_u64((unsigned long long)__double_as_longlong(v)));
+}
+
+__device__ __forceinline__ double2 swap_adjacent(double2 v)
+{
+    return make_double2(swap_adjacent(v.x), swap_adjacent(v.y));
+}
+
 // One 64 x 64 tile of a pass, evaluated by one wavefront (the LDS arrays are
 // per-workgroup allocations of this function's instantiation).
 template <int MODE, bool EXACT>
@@ -99,6 +119,80 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
     const bool track_tol = a.tol != nullptr && rp.tol_cap2 > 0.0;
 
     for (int ph = 0; ph < 4; ++ph) {
+        if constexpr (kTab) {
+            // tabulated field, four steps at a time with every record load issued before any is used.
+            // A record is S doubles, read as S / 4 pairs of 16-byte chunks; lanes 2k and 2k + 1 read
+            // them together -- the even lane the first chunk of a pair, the odd lane the second, first
+            // of the even lane's record, then of the odd lane's -- and swap what the other needs, so
+            // each load instruction touches 32 cache lines instead of 64 (the vector L1 looks up one
+            // line per clock, which bounds these passes).  Every lane loads: addresses of lanes past
+            // the tile's edge are cells of the table too (table_cell wraps its indexes).
+            constexpr int S = MODE == kModeSpeedTab ? 4 : 8;  // doubles per cell record
+            constexpr int NP = S / 4;                          // chunk pairs per record
+            const bool odd = lane & 1;
+            for (int t0 = 0; t0 < 16; t0 += 4) {
+                double2 xa[4][NP], xb[4][NP];
+                double axk[4], ayk[4];
+                bool actk[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int jl = lane ^ (ph * 16 + t0 + k);
+                    const int j = td.J0 + jl;
+                    actk[k] = vi && j < n && (!diag || lane < jl);
+                    const double dx = wrap(pi.x - sx[jl]);
+                    const double dy = wrap(pi.y - sy[jl]);
+                    if (actk[k] && track_tol) {
+                        const double d2 = dist2(dx, dy);
+                        if (d2 < rp.tol_cap2) {
+                            atomic_min_nonneg(a.tol + base + i, d2);
+                            atomic_min_nonneg(a.tol + base + j, d2);
+                        }
+                    }
+                    const unsigned long long own = reinterpret_cast<unsigned long long>(
+                        table_cell(a.tab_s, S, a.tab_n, dx, dy, axk[k], ayk[k]));
+                    const unsigned long long other = swap_adjacent_u64(own);
+                    const unsigned long long half_off = odd ? 16ull : 0ull;
+                    const char* ra = reinterpret_cast<const char*>((odd ? other : own) + half_off);  // even lane's record
+                    const char* rb = reinterpret_cast<const char*>((odd ? own : other) + half_off);  // odd lane's record
+#pragma unroll
+                    for (int m = 0; m < NP; ++m) {
+                        xa[k][m] = *reinterpret_cast<const double2*>(ra + 32 * m);
+                        xb[k][m] = *reinterpret_cast<const double2*>(rb + 32 * m);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    // chunks 2m and 2m + 1 of this lane's record: the even lane holds chunk 2m of its own
+                    // (xa) and the odd lane chunk 2m + 1 of its own (xb); each sends the other's
+                    double2 ch[2 * NP];
+#pragma unroll
+                    for (int m = 0; m < NP; ++m) {
+                        const double2 got = swap_adjacent(odd ? xa[k][m] : xb[k][m]);
+                        ch[2 * m] = odd ? got : xa[k][m];
+                        ch[2 * m + 1] = odd ? xb[k][m] : got;
+                    }
+                    const int tt = t0 + k;
+                    const int jl = lane ^ (ph * 16 + tt);
+                    const int j = td.J0 + jl;
+                    double V = 0.0;
+                    if (actk[k]) {
+                        const double c4[4] = {ch[0].x, ch[0].y, ch[1].x, ch[1].y};
+                        const double f = flip_if(bilinear_cell(c4, axk[k], ayk[k]), ipos != (j < half));
+                        q0 += f;
+                        atomicAdd(&jq0[jl], f);
+                        if (MODE == kModeJacDenseTab) {
+                            const double d4v[4] = {ch[2 * NP - 2].x, ch[2 * NP - 2].y, ch[2 * NP - 1].x,
+                                                   ch[2 * NP - 1].y};
+                            const double bj = j < half ? 1.0 : -1.0;
+                            V = rp.h * bj * bi * bilinear_cell(d4v, axk[k], ayk[k]);
+                            q1 += V;
+                            atomicAdd(&jq1[jl], V);
+                        }
+                    }
+                    if (kDense) stage[((lane >> 4) * 16 + (lane & 15)) * 17 + ((lane & 15) ^ tt)] = V;
+                }
+            }
+        } else
         for (int tt = 0; tt < 16; ++tt) {
             const int jl = lane ^ (ph * 16 + tt);
             const int j = td.J0 + jl;
@@ -116,24 +210,10 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                 const double bj = j < half ? 1.0 : -1.0;
                 const bool neg = ipos != (j < half);  // b_i b_j = -1
                 double E = 0.0, R = 0.0, c = 0.0;
-                if (!kTab) field_inputs(dx, dy, E, R, c);
+                field_inputs(dx, dy, E, R, c);
                 double sig = 0.0, dsig = 0.0;
                 // source-side sums accumulate with the target's sign and are negated once at the end
-                if (MODE == kModeSpeedTab) {
-                    sig = table_bilinear(a.tab_s, a.tab_n, dx, dy);
-                    const double f = flip_if(sig, neg);
-                    q0 += f;
-                    atomicAdd(&jq0[jl], f);
-                } else if (MODE == kModeJacDenseTab) {
-                    sig = table_bilinear(a.tab_s, a.tab_n, dx, dy);
-                    dsig = table_bilinear(a.tab_d, a.tab_n, dx, dy);
-                    const double f = flip_if(sig, neg);
-                    V = rp.h * bj * bi * dsig;
-                    q0 += f;
-                    atomicAdd(&jq0[jl], f);
-                    q1 += V;
-                    atomicAdd(&jq1[jl], V);
-                } else if (MODE == kModeSpeed || kCoo) {
+                if (MODE == kModeSpeed || kCoo) {
                     // the pair-list pass takes sigma here and d sigma for the listed pairs alone
                     // (launch_coo_derivative), so its tiles do the speed pass's work
                     pair_field<false, false>(dx, dy, E, R, c, sig, dsig);
@@ -215,17 +295,19 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                 // interior tile: every entry valid
 #pragma unroll 4
                 for (int k = 0; k < 16; ++k) {
-                    *p1 = st1[k * 17];
-                    *p2 = st2[k];
+                    // the matrix is written once and read by a later launch: streamed past the L2, where
+                    // the tabulated field's cells and the positions stay
+                    __builtin_nontemporal_store(st1[k * 17], p1);
+                    __builtin_nontemporal_store(st2[k], p2);
                     p1 += ld;
                     p2 += ld;
                 }
             } else {
                 for (int k = 0; k < 16; ++k) {
                     const int i1 = ib + k, j1 = jb + q;
-                    if (i1 < n && j1 < n && (!diag || i1 < j1)) *p1 = st1[k * 17];
+                    if (i1 < n && j1 < n && (!diag || i1 < j1)) __builtin_nontemporal_store(st1[k * 17], p1);
                     const int i2 = ib + q, j2 = jb + k;
-                    if (i2 < n && j2 < n && (!diag || i2 < j2)) *p2 = st2[k];
+                    if (i2 < n && j2 < n && (!diag || i2 < j2)) __builtin_nontemporal_store(st2[k], p2);
                     p1 += ld;
                     p2 += ld;
                 }
@@ -351,6 +433,35 @@ __global__ void table_build_kernel(int n, double* tab_s, double* tab_d)
     }
     tab_s[k] = s;
     if (tab_d) tab_d[k] = d;
+}
+
+__global__ void table_records_kernel(int n, const double* tab_s, const double* tab_d, double* rec)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)n * n) return;
+    const int kx = (int)(k % n), ky = (int)(k / n);
+    const int kx1 = (kx + 1) & (n - 1), ky1 = (ky + 1) & (n - 1);
+    const int64_t c00 = (int64_t)ky * n + kx, c10 = (int64_t)ky * n + kx1, c01 = (int64_t)ky1 * n + kx,
+                  c11 = (int64_t)ky1 * n + kx1;
+    double* r = rec + k * (tab_d ? 8 : 4);
+    r[0] = tab_s[c00];
+    r[1] = tab_s[c10];
+    r[2] = tab_s[c01];
+    r[3] = tab_s[c11];
+    if (tab_d) {
+        r[4] = tab_d[c00];
+        r[5] = tab_d[c10];
+        r[6] = tab_d[c01];
+        r[7] = tab_d[c11];
+    }
+}
+
+hipError_t launch_table_records(int n, const double* tab_s, const double* tab_d, double* rec, hipStream_t s)
+{
+    const int64_t nn = (int64_t)n * n;
+    hipLaunchKernelGGL(table_records_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, n, tab_s, tab_d,
+                       rec);
+    return hipGetLastError();
 }
 
 hipError_t launch_table_build(int n, double* tab_s, double* tab_d, hipStream_t s)
