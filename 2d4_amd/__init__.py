@@ -72,6 +72,8 @@ SIGNATURES = {
     "d4_jacobian_rescale": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_char, _d, _d, _vp]),
     "d4_jacobian_dense_dev": (C.c_int, [_vp, C.c_int, _vp, _P(JacParams), _vp, _vp, _vp, C.c_int64, _vp, _vp]),
     "d4_jacobian_dense_scale_dev": (C.c_int, [_vp, C.c_int, _vp, C.c_int64, _vp]),
+    "d4_jacobian_dense_cut_dev": (C.c_int, [_vp, C.c_int, _vp, _P(JacParams), _vp, _vp, _vp, C.c_int64, _vp, _vp]),
+    "d4_theta_pair_vjp_dev": (C.c_int, [_vp, C.c_int, _vp, _P(JacParams), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "d4_jacobian_dense": (C.c_int, [_vp, C.c_int, _vp, _P(JacParams), _vp, _vp, _vp, _vp, _vp, C.c_int]),
     "d4_vjp_dev": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, _pd, C.c_int]),
     "d4_table_build_dev": (C.c_int, [_vp, C.c_int, _vp, _vp]),
@@ -254,7 +256,10 @@ class Context:
                                             tol.data_ptr() if tol is not None else None, float(tol_cap2)))
         return v
 
-    def jacobian_dense_t(self, xy, params, M=None, tol=None, scale=False):
+    def jacobian_dense_t(self, xy, params, M=None, tol=None, scale=False, cutoff=False):
+        """(vA, vB, M, d): M[col, row] = V + diag(s) (the row scaling and +1 too with scale=True); with
+        cutoff=True V keeps only the pairs inside the storage radius, tapered, as the reference stores them
+        (d4_jacobian_dense_cut_dev), else every pair (d4_jacobian_dense_dev)."""
         import torch
         n = xy.shape[0]
         dev = xy.device
@@ -264,12 +269,32 @@ class Context:
         if M is None:
             M = torch.empty((n, n), dtype=torch.float64, device=dev)  # M[col, row]
         self.use_torch_stream()
-        self.check(self.L.d4_jacobian_dense_dev(self.h, n, xy.data_ptr(), C.byref(params), vA.data_ptr(),
-                                                vB.data_ptr(), M.data_ptr(), n, d.data_ptr(),
-                                                tol.data_ptr() if tol is not None else None))
+        fn = self.L.d4_jacobian_dense_cut_dev if cutoff else self.L.d4_jacobian_dense_dev
+        self.check(fn(self.h, n, xy.data_ptr(), C.byref(params), vA.data_ptr(), vB.data_ptr(), M.data_ptr(), n,
+                      d.data_ptr(), tol.data_ptr() if tol is not None else None))
         if scale:
-            self.check(self.L.d4_jacobian_dense_scale_dev(self.h, n, M.data_ptr(), n, d.data_ptr()))
+            self.dense_scale_t(M, d)
         return vA, vB, M, d
+
+    def dense_scale_t(self, M, d):
+        """M[col, row] -> the reference's final matrix I + diag((1+d)/2) M, in place."""
+        n = M.shape[0]
+        self.use_torch_stream()
+        self.check(self.L.d4_jacobian_dense_scale_dev(self.h, n, M.data_ptr(), M.stride(0), d.data_ptr()))
+        return M
+
+    def theta_pair_vjp_t(self, xy, params, a, c, u, w, t, want_dy=True):
+        """(gx, gy): the pair term of the implicit step's vector-Jacobian product (d4_theta_pair_vjp_dev)."""
+        import torch
+        n = xy.shape[0]
+        gx = torch.empty(n, dtype=torch.float64, device=xy.device)
+        gy = torch.empty(n, dtype=torch.float64, device=xy.device) if want_dy else None
+        vs = [v.contiguous() for v in (a, c, u, w, t)]
+        self.use_torch_stream()
+        self.check(self.L.d4_theta_pair_vjp_dev(self.h, n, xy.data_ptr(), C.byref(params),
+                                                *[v.data_ptr() for v in vs], gx.data_ptr(),
+                                                gy.data_ptr() if gy is not None else None))
+        return gx, gy
 
     def vjp_t(self, xy, w, exact=True, want_dy=True):
         """(gx, gy, gtau) of L = sum_i w_i v_i(x, y, tau)."""

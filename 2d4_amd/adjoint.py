@@ -1,7 +1,6 @@
 """Gradients through the pair-speed map and through implicit steps.
 
-No counterpart in the reference (which has no derivatives beyond its own
-Jacobian); the forward map differentiated here is defined by this module:
+The reference computes no derivatives beyond its own Jacobian; the maps differentiated here are:
 
   * PairSpeed: v_i(x, y, tau) = sum_{j != i} b_i b_j sigma(x_i - x_j, y_i - y_j)
     + tau b_i, the speed pass of simulation.cpp:287-333 (b_i = +1 for the first
@@ -9,27 +8,33 @@ Jacobian); the forward map differentiated here is defined by this module:
     hand-written VJP kernel (d4_vjp_dev): for L = sum_i w_i v_i,
     dL/dx_j = sum_{i != j} (w_j - w_i) b_i b_j d sigma/dx (x_i - x_j, y_i - y_j),
     the same with d sigma/dy for dL/dy_j, and dL/dtau = sum_i b_i w_i.
-  * ImplicitStep: one backward-Euler step x' = x + dt v(x', y, tau), solved by
-    Newton's method with the dense Jacobian from the pair kernel
-    (M = -dt dv/dx, so the Newton matrix is I + M), each Newton system solved
-    by the library's GPU dense LU (d4_lu_factor_dense_dev).  Its backward follows from
-    the implicit function theorem: mu = (I + M)^{-T} lambda at the converged x',
-    dL/dx = mu, dL/dy = dt (dv/dy)^T mu, dL/dtau = dt b^T mu -- the last two
-    from the VJP kernel.
-  * implicit_step: ImplicitStep over dt, or -- where Newton does not converge,
-    as near a pair so close that dt |dv/dx| >> 1 and a step moves dislocations
-    farther than they are apart -- a run of shorter implicit sub-steps (halved
-    on failure, doubled after success) that ends exactly at dt; each sub-step
-    is its own ImplicitStep in the autograd graph, and the same sub-steps are
-    taken again when a checkpointed segment is recomputed.
-  * accumulated_strain: sum_i b_i (x_T,i - x_0,i) after T steps (the plastic
-    strain the reference accumulates, simulation.cpp:713-720, summed over steps),
-    with checkpoint/recompute every `checkpoint_every` steps.
-
-Every derivative here is the true derivative of sigma (exact = True), not the
-reference's Jacobian entry, which replaces sinh by cosh for the |shift| >= 3
-images (Field.cpp:269-283); with exact = False the VJP kernel is the transpose
-of the reference's own Jacobian instead.
+  * ThetaHalfStep: one half of the reference's implicit step, the map the simulation runs
+    (simulation.cpp:98-154).  At z, with k the half step: the pass gives the speeds v0, v1 at the
+    half step's two times, V (k b_i b_j times the reference's own d sigma, Field.cpp:314-331, inside
+    the storage radius), s = -V 1 and d = weight(s) (sim_utils.cpp:36-53);
+    A = I + diag((1 + d)/2)(V + diag(s)) (simulation.cpp:516-522);
+        w1 = z - A^-1 g1,  g1 = -k ((1 + d) v1 + (1 - d) v0) / 2,
+        w2 = w1 - A^-1 g2, g2 = w1 - z - k ((1 + d) v(w1) + (1 - d) v0) / 2
+    (calcGSolveAndUpdate, simulation.cpp:610-621).  A step of h is two half steps of h/2: stage II's
+    matrix, rescaled from stage I's (simulation.cpp:546-608), is the pass at h/2, and stage I's large
+    step only feeds the error estimate.  The backward pass is this map's exact vector-Jacobian
+    product: two transposed solves with A, the VJP kernel at z and at w1, the derivative of weight(),
+    and how A's entries move with the positions (d4_theta_pair_vjp_dev, the reference's d sigma
+    differentiated by dual numbers).
+  * theta_step: one accepted step of h.  The step sizes are taken as given: the reference chooses
+    them from an error estimate (precision_handler.cpp), and gradients here are those of the map
+    with its step sequence held fixed.
+  * ImplicitStep: a backward-Euler step x' = x + dt v(x', y, tau), solved by Newton's method with
+    the dense Jacobian from the pair kernel, each Newton system by the library's GPU dense LU
+    (d4_lu_factor_dense_dev); no reference counterpart (kept as an alternative map).  Its backward
+    follows from the implicit function theorem: mu = (I + M)^{-T} lambda at the converged x',
+    dL/dx = mu, dL/dy = dt (dv/dy)^T mu, dL/dtau = dt b^T mu.
+  * implicit_step: ImplicitStep over dt, or -- where Newton does not converge, as near a pair so
+    close that dt |dv/dx| >> 1 -- a run of shorter implicit sub-steps (halved on failure, doubled
+    after success) that ends exactly at dt.
+  * accumulated_strain: sum_i b_i (x_T,i - x_0,i) after T steps (the plastic strain the reference
+    accumulates, simulation.cpp:167-171), with checkpoint/recompute every `checkpoint_every` steps;
+    method "theta" (the reference's step, the default) or "backward_euler".
 """
 from importlib import import_module
 
@@ -64,6 +69,107 @@ class PairSpeed(torch.autograd.Function):
         x, y = ctx.saved_tensors
         gx, gy, gt = ctx.d4.vjp_t(_xy(x, y), gv.contiguous(), exact=ctx.exact, want_dy=True)
         return gx, gy, torch.tensor(gt, dtype=torch.float64, device=x.device), None, None
+
+
+_WEIGHT_C = 0.412073  # sim_utils.cpp:47
+
+
+def _weight_prime(s, wfunc):
+    """d weight(s) / ds for sim_utils.cpp:36-53's weight functions (0 where s <= 0, where weight is 0)."""
+    pos = s > 0
+    sp = torch.where(pos, s, torch.zeros_like(s))
+    if wfunc == "c":  # (s / (s + 1))^2
+        g = 2.0 * sp / (sp + 1.0) ** 3
+    elif wfunc == "p":  # s / (s + 1)
+        g = 1.0 / (sp + 1.0) ** 2
+    elif wfunc == "m":  # q / (q + 1.5), q = s + c s^2
+        q = sp + _WEIGHT_C * sp * sp
+        g = 1.5 * (1.0 + 2.0 * _WEIGHT_C * sp) / (q + 1.5) ** 2
+    else:
+        raise ValueError(f"unknown weight function {wfunc!r}")
+    return torch.where(pos, g, torch.zeros_like(s))
+
+
+class ThetaOptions:
+    """The reference's pass settings of an implicit step: cutoff (from -u as SimulationData::updateCutOff,
+    simulation_data.cpp:253-262; None: every pair), heaviside (-h) and the weight function (-w)."""
+
+    def __init__(self, cutoff_mult=None, heaviside=False, wfunc="c"):
+        self.cutoff_mult, self.heaviside, self.wfunc = cutoff_mult, bool(heaviside), wfunc
+
+    def params(self, n, k, tau0, tau1):
+        return _pkg().jac_params(h=k, tau_a=tau0, tau_b=tau1, n=n, cutoff_mult=self.cutoff_mult,
+                                 heaviside=self.heaviside, wfunc=self.wfunc, exact=False)
+
+
+class ThetaHalfStep(torch.autograd.Function):
+    """w2 = H(z): one half step of the reference's implicit step (see the module docstring)."""
+
+    @staticmethod
+    def _pass(d4, z, y, p):
+        """(v0, v1, Mdot, d) at z: Mdot[col, row] = V + diag(s), V as the reference stores it."""
+        return d4.jacobian_dense_t(_xy(z, y), p, cutoff=True)
+
+    @staticmethod
+    def forward(ctx, z, y, tau0, tau1, k, d4, opts):
+        n = z.numel()
+        t0, t1 = float(tau0), float(tau1)
+        p = opts.params(n, k, t0, t1)
+        v0, v1, A, d = ThetaHalfStep._pass(d4, z, y, p)
+        d4.dense_scale_t(A, d)  # A[col, row]: the tensor read row-major is A^T
+        singular, _ = d4.lu_factor_dense_t(A, transpose=True)
+        if singular:
+            raise RuntimeError("singular implicit-step matrix")
+        g1 = -k * ((1 + d) * v1 + (1 - d) * v0) / 2
+        u = d4.lu_solve_dense_t(g1)
+        w1 = z - u
+        v2 = d4.speed_pass_t(_xy(w1, y), t1)
+        g2 = w1 - z - k * ((1 + d) * v2 + (1 - d) * v0) / 2
+        w = d4.lu_solve_dense_t(g2)
+        ctx.save_for_backward(z, y, w1, u, w, v0, v1, v2, d)
+        ctx.d4, ctx.k, ctx.opts, ctx.p = d4, k, opts, p
+        return w1 - w
+
+    @staticmethod
+    def backward(ctx, wb2):
+        z, y, w1, u, w, v0, v1, v2, d = ctx.saved_tensors
+        d4, k, p = ctx.d4, ctx.k, ctx.p
+        n = z.numel()
+        _, _, Md, d_again = ThetaHalfStep._pass(d4, z, y, p)
+        s = Md.diagonal().clone()  # s_i = -sum_j V_ij
+        Mdot = Md.t()  # (row, col) view of V + diag(s)
+        Mu, Mw = torch.mv(Mdot, u), torch.mv(Mdot, w)
+        A = d4.dense_scale_t(Md, d_again)
+        singular, _ = d4.lu_factor_dense_t(A, transpose=False)  # A^T: its solves give A^-T
+        if singular:
+            raise RuntimeError("singular implicit-step matrix")
+        Dh = (1 + d) / 2
+        r2 = -d4.lu_solve_dense_t(wb2.contiguous())  # adjoint of g2
+        vb2 = -k * Dh * r2
+        gx1, gy1, _ = d4.vjp_t(_xy(w1, y), vb2.contiguous(), exact=True, want_dy=True)
+        wb1 = wb2 + r2 + gx1
+        r1 = -d4.lu_solve_dense_t(wb1.contiguous())  # adjoint of g1
+        vb1 = -k * Dh * r1
+        vb0 = -k * (1 - d) / 2 * (r1 + r2)
+        gx0, gy0, _ = d4.vjp_t(_xy(z, y), (vb0 + vb1).contiguous(), exact=True, want_dy=True)
+        # the matrix: Abar = -(r1 u^T + r2 w^T); through diag((1 + d)/2), weight(s) and s = -V 1
+        db = -k / 2 * ((v2 - v0) * r2 + (v1 - v0) * r1) - (r1 * Mu + r2 * Mw) / 2
+        t = _weight_prime(s, ctx.opts.wfunc) * db - Dh * (r1 * u + r2 * w)
+        gpx, gpy = d4.theta_pair_vjp_t(_xy(z, y), p, Dh * r1, Dh * r2, u, w, t)
+        b = _signs(n, z.device)
+        xb = wb1 - r2 + gx0 + gpx
+        yb = gy1 + gy0 + gpy
+        tb0 = (b * vb0).sum()
+        tb1 = (b * (vb1 + vb2)).sum()
+        return xb, yb, tb0, tb1, None, None, None
+
+
+def theta_step(x, y, tau, h, d4, opts=None):
+    """One accepted step of h of the reference's loop (simulation.cpp:88-171) at constant stress tau:
+    two half steps of h/2."""
+    opts = opts if opts is not None else ThetaOptions()
+    x = ThetaHalfStep.apply(x, y, tau, tau, h / 2, d4, opts)
+    return ThetaHalfStep.apply(x, y, tau, tau, h / 2, d4, opts)
 
 
 def _newton_matrix(d4, x, y, tau, dt):
@@ -188,17 +294,23 @@ def implicit_step(x, y, tau, dt, d4, tol=1e-12, max_iter=30, max_halvings=24):
     return x
 
 
-def _segment(x, y, tau, steps, dt, d4, tol, max_iter):
+def _segment(x, y, tau, steps, dt, d4, tol, max_iter, method, opts):
     for _ in range(steps):
-        x = implicit_step(x, y, tau, dt, d4, tol, max_iter)
+        if method == "theta":
+            x = theta_step(x, y, tau, dt, d4, opts)
+        else:
+            x = implicit_step(x, y, tau, dt, d4, tol, max_iter)
     return x
 
 
-def accumulated_strain(x0, y, tau, T, dt, d4=None, checkpoint_every=16, tol=1e-12, max_iter=30):
-    """sum_i b_i (x_T,i - x_0,i) after T backward-Euler steps of dt from sorted
-    positions (x0, y) at stress tau (a 0-d tensor to differentiate by it).
-    Only every `checkpoint_every`-th state is kept for the backward pass; the
-    steps in between are recomputed."""
+def accumulated_strain(x0, y, tau, T, dt, d4=None, checkpoint_every=16, tol=1e-12, max_iter=30, method="theta",
+                       opts=None):
+    """sum_i b_i (x_T,i - x_0,i) after T steps of dt from sorted positions (x0, y) at stress tau (a 0-d
+    tensor to differentiate by it): the reference's implicit step (method "theta", with ThetaOptions
+    opts) or backward Euler ("backward_euler", Newton to tol).  Only every `checkpoint_every`-th state
+    is kept for the backward pass; the steps in between are recomputed."""
+    if method not in ("theta", "backward_euler"):
+        raise ValueError(f"unknown method {method!r}")
     d4 = d4 if d4 is not None else _pkg().default_context()
     if not torch.is_tensor(tau):
         tau = torch.tensor(float(tau), dtype=torch.float64, device=x0.device)
@@ -208,9 +320,9 @@ def accumulated_strain(x0, y, tau, T, dt, d4=None, checkpoint_every=16, tol=1e-1
     while done < T:
         k = min(seg, T - done)
         if checkpoint_every > 0 and torch.is_grad_enabled():
-            x = checkpoint(_segment, x, y, tau, k, dt, d4, tol, max_iter, use_reentrant=False)
+            x = checkpoint(_segment, x, y, tau, k, dt, d4, tol, max_iter, method, opts, use_reentrant=False)
         else:
-            x = _segment(x, y, tau, k, dt, d4, tol, max_iter)
+            x = _segment(x, y, tau, k, dt, d4, tol, max_iter, method, opts)
         done += k
     return (_signs(x0.numel(), x0.device) * (x - x0)).sum()
 

@@ -20,7 +20,8 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-mcode-obje
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wall", "-Wno-unused-command-line-argument"]
 
-SOURCES = ["d4_kernels.hip", "d4_capi.hip", "d4_batch.hip", "d4_gpulu.hip", "d4_dstep.hip", "d4_sim.cpp", "d4_solver.cpp"]
+SOURCES = ["d4_kernels.hip", "d4_capi.hip", "d4_batch.hip", "d4_gpulu.hip", "d4_dstep.hip", "d4_adjoint.hip",
+           "d4_sim.cpp", "d4_solver.cpp"]
 
 
 def _hipcc():

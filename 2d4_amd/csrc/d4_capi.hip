@@ -561,6 +561,38 @@ int d4_jacobian_dense_dev(d4_ctx* ctx, int n, const double* xy, const d4_jac_par
                             1);
 }
 
+int d4_jacobian_dense_cut_dev(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, double* vA, double* vB,
+                              double* M, int64_t ld, double* dvec, double* tol)
+{
+    if (!ctx || !p || n < 0 || (n > 0 && (!xy || !vA || !M || ld < n))) return D4_ERR_INVALID;
+    if (n == 0) return D4_OK;
+    RealParams rp = real_params(n, p, 0.0);
+    if (!tol) rp.tol_cap2 = 0.0;
+    return ctx->single_pass(kModeJacDenseCut, n, xy, rp, tol, vA, vB, nullptr, dvec, M, ld, nullptr, p->exact ? 1 : 0,
+                            0, 1);
+}
+
+int d4_theta_pair_vjp_dev(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, const double* a,
+                          const double* c, const double* u, const double* w, const double* t, double* gx, double* gy)
+{
+    if (!ctx || !p || n < 0 || (n > 0 && (!xy || !a || !c || !u || !w || !t || !gx))) return D4_ERR_INVALID;
+    if (n == 0) return D4_OK;
+    ThetaVjpArgs va{};
+    va.n = n;
+    va.xy = reinterpret_cast<const double2*>(xy);
+    va.rp = real_params(n, p, 0.0);
+    va.a = a;
+    va.c = c;
+    va.u = u;
+    va.w = w;
+    va.t = t;
+    va.gx = gx;
+    va.gy = gy;
+    hipError_t e = launch_theta_pair_vjp(va, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "implicit-step pair VJP launch");
+    return D4_OK;
+}
+
 int d4_jacobian_dense_scale_dev(d4_ctx* ctx, int n, double* M, int64_t ld, const double* dvec)
 {
     if (!ctx || n < 0 || (n > 0 && (!M || !dvec || ld < n))) return D4_ERR_INVALID;

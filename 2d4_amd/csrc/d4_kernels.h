@@ -156,6 +156,18 @@ hipError_t launch_csc_fill(const CooEntry* coo, int ncoo, const int* Ap, int* cu
                            int total, hipStream_t s, const int* ncoo_dev = nullptr);
 // the single-realization tile table of n (row-major upper triangle, slot = index) built on the device
 hipError_t launch_tile_table(TileDesc* t, int n, hipStream_t s);
+
+// the pair term of the implicit step's vector-Jacobian product (d4_adjoint.hip); rp: h = the step's k,
+// the storage radius and taper
+struct ThetaVjpArgs {
+    int n;
+    const double2* xy;
+    RealParams rp;
+    const double *a, *c, *u, *w, *t;
+    double* gx;
+    double* gy;  // may be null
+};
+hipError_t launch_theta_pair_vjp(const ThetaVjpArgs& a, hipStream_t s);
 // p[0..n) = v (a kernel, so it can sit in a captured graph)
 hipError_t launch_fill_int(int* p, int v, int n, hipStream_t s);
 // wfunc per realization from wf_arr (null: wf0 for all)

@@ -302,9 +302,9 @@ def c5_reference_step_check(T=4, n=2048, eps=1e-9, coord=7, seed=2048):
     return {"n": n, "tau": 0.2, "dt": 1e-3, "eps": eps, "coordinate": coord, "steps": rows}
 
 
-def adjoint_bench(n, T, dt, every):
-    """Strain gradient through T backward-Euler steps (2d4_amd/adjoint.py) at
-    N = n: times the forward pass, the checkpointed forward and the backward."""
+def adjoint_bench(n, T, dt, every, method="theta"):
+    """Strain gradient through T implicit steps (2d4_amd/adjoint.py: the reference's step, or backward
+    Euler) at N = n: times the forward pass, the checkpointed forward and the backward."""
     import torch
     adj = importlib.import_module("2d4_amd.adjoint")
     pkg = importlib.import_module("2d4_amd")
@@ -316,20 +316,20 @@ def adjoint_bench(n, T, dt, every):
     with torch.no_grad():
         xs = x
         for k in range(0, T, every):
-            xs = adj._segment(xs, y, tau, min(every, T - k), dt, ctx, 1e-12, 30)
+            xs = adj._segment(xs, y, tau, min(every, T - k), dt, ctx, 1e-12, 30, method, None)
             log(f"[bench] adjoint forward {min(k + every, T)} steps: {time.perf_counter() - t0:.1f} s {adj.stats}")
     t1 = time.perf_counter()
     fwd_stats = dict(adj.stats)
     x0 = x.clone().requires_grad_()
     yy = y.clone().requires_grad_()
     tt = tau.clone().requires_grad_()
-    L = adj.accumulated_strain(x0, yy, tt, T, dt, ctx, checkpoint_every=every)
+    L = adj.accumulated_strain(x0, yy, tt, T, dt, ctx, checkpoint_every=every, method=method)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     L.backward()
     torch.cuda.synchronize()
     t3 = time.perf_counter()
-    return {"n": n, "T": T, "dt": dt, "checkpoint_every": every, "tau": 0.2,
+    return {"n": n, "T": T, "dt": dt, "checkpoint_every": every, "tau": 0.2, "method": method,
             "forward_s": t1 - t0, "forward_checkpointed_s": t2 - t1, "backward_s": t3 - t2,
             "forward_stats": fwd_stats, "L": L.item(), "max_abs_dL_dx0": x0.grad.abs().max().item(),
             "max_abs_dL_dy": yy.grad.abs().max().item(), "dL_dtau": tt.grad.item(),
@@ -368,7 +368,8 @@ def main():
     ap.add_argument("--ens-only", action="store_true",
                     help="only the batched ensemble (no plain comparison, no pair-pass headline), for profiling")
     ap.add_argument("--adjoint", type=str, default="",
-                    help="only the adjoint measurement: N,T,dt (e.g. 2048,64,1e-5), checkpoints every 16 steps")
+                    help="only the adjoint measurement: N,T,dt[,method] (e.g. 2048,256,1e-3; method theta, the "
+                         "reference's step, or backward_euler), checkpoints every 16 steps")
     ap.add_argument("--c5-check", type=int, default=0,
                     help="only the C5 configuration through the reference's own step for this many steps")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -393,8 +394,10 @@ def main():
         print(json.dumps({"c5_reference_step": c5_reference_step_check(args.c5_check)}), flush=True)
         return
     if args.adjoint:
-        an, aT, adt = args.adjoint.split(",")
-        print(json.dumps({"adjoint": adjoint_bench(int(an), int(aT), float(adt), 16)}), flush=True)
+        parts = args.adjoint.split(",")
+        method = parts[3] if len(parts) > 3 else "theta"
+        print(json.dumps({"adjoint": adjoint_bench(int(parts[0]), int(parts[1]), float(parts[2]), 16, method)}),
+              flush=True)
         return
     pkg = importlib.import_module("2d4_amd")
     dev = torch.device("cuda:0")

@@ -133,6 +133,12 @@ int d4_jacobian_rescale(d4_ctx* ctx, int n, const int* Ap, const int* Ai, double
  * M = -h dv/dx.  vA, vB, dvec, tol as d4_jacobian_pass. */
 int d4_jacobian_dense_dev(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, double* vA, double* vB,
                           double* M, int64_t ld, double* dvec, double* tol);
+/* The reference's Jacobian pass with its cutoff (simulation.cpp:345-515) into dense storage: as
+ * d4_jacobian_dense_dev, but V_ij is stored only for pairs inside the storage radius (d^2 < rc^2 with
+ * the heaviside cutoff, d^2 < 49 rc^2 otherwise) and tapered by exp(-(r - rc)^2 / rc^2) beyond rc
+ * (simulation.cpp:458-470), zero elsewhere: the reference's CSC Ax before its row scaling, densely. */
+int d4_jacobian_dense_cut_dev(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, double* vA, double* vB,
+                              double* M, int64_t ld, double* dvec, double* tol);
 /* simulation.cpp:516-522 on a dense M from d4_jacobian_dense_dev:
  * M[j,i] *= (1 + d_j)/2, M[i,i] += 1 -> the reference's final A. */
 int d4_jacobian_dense_scale_dev(d4_ctx* ctx, int n, double* M, int64_t ld, const double* dvec);
@@ -147,6 +153,17 @@ int d4_jacobian_dense(d4_ctx* ctx, int n, const double* xy, const d4_jac_params*
  * whose transpose this then is). */
 int d4_vjp_dev(d4_ctx* ctx, int n, const double* xy, const double* w, double* gx, double* gy, double* gtau,
                int flags);
+
+/* The pair term of the vector-Jacobian product of the reference's implicit step (no reference
+ * counterpart; the adjoint of how simulation.cpp:345-527's matrix entries
+ * V_ij = k b_i b_j xy_diff_x(x_i - x_j, y_i - y_j) * taper depend on the positions, Field.cpp:314-331's
+ * formula differentiated as the reference evaluates it).  p: h = the step's k, cutoff, heaviside.
+ * With P_lj = -(a_l u_j + a_j u_l + c_l w_j + c_j w_l) - t_l - t_j over pairs inside the storage radius:
+ *   gx_l = sum_{j > l} k b_l b_j dm/ddx (x_l - x_j, y_l - y_j) P_lj - sum_{j < l} k b_l b_j dm/ddx (x_j - x_l,
+ *          y_j - y_l) P_lj  (each pair differentiated where the reference evaluates it, lower index first),
+ *   gy_l the same with dm/ddy (gy may be NULL).  Device pointers, n entries each. */
+int d4_theta_pair_vjp_dev(d4_ctx* ctx, int n, const double* xy, const d4_jac_params* p, const double* a,
+                          const double* c, const double* u, const double* w, const double* t, double* gx, double* gy);
 
 /* Tabulated field (no reference counterpart: the reference evaluates the
  * analytic Field::xy per pair).  A table is n x n doubles (n a power of two),

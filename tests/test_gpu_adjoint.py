@@ -1,7 +1,8 @@
-"""Adjoint of the pair-speed map (VJP kernel) and gradients through implicit
+"""Adjoint of the pair-speed map (VJP kernel) and gradients through backward-Euler
 steps (2d4_amd/adjoint.py), checked against the transpose of the Jacobian the
 reference assembles, against the oracle's closed form of d sigma/dy, and
-against central finite differences of the library's own forward map."""
+against central finite differences of the library's own forward map.  The
+reference's own step is covered by test_gpu_theta_adjoint.py."""
 from importlib import import_module
 
 import numpy as np
@@ -100,12 +101,13 @@ def _grad_and_fd(ctx, n, T, coords_x, coords_y, seed, checkpoint_every=16, dt=2e
     x0 = _dev(xy[:, 0]).requires_grad_()
     y = _dev(xy[:, 1]).requires_grad_()
     tau = torch.tensor(tau0, dtype=torch.float64, device="cuda:0", requires_grad=True)
-    L = adj.accumulated_strain(x0, y, tau, T, dt, ctx, checkpoint_every=checkpoint_every)
+    L = adj.accumulated_strain(x0, y, tau, T, dt, ctx, checkpoint_every=checkpoint_every, method="backward_euler")
     L.backward()
 
     def loss(xv, yv, tv):
         with torch.no_grad():
-            return adj.accumulated_strain(_dev(xv), _dev(yv), tv, T, dt, ctx, checkpoint_every=0).item()
+            return adj.accumulated_strain(_dev(xv), _dev(yv), tv, T, dt, ctx, checkpoint_every=0,
+                                          method="backward_euler").item()
 
     h = 1e-6
     fd = []
