@@ -22,8 +22,9 @@ The reference computes no derivatives beyond its own Jacobian; the maps differen
     and how A's entries move with the positions (d4_theta_pair_vjp_dev, the reference's d sigma
     differentiated by dual numbers).
   * theta_step: one accepted step of h.  The step sizes are taken as given: the reference chooses
-    them from an error estimate (precision_handler.cpp), and gradients here are those of the map
-    with its step sequence held fixed.
+    them from an error estimate (precision_handler.cpp); reference_schedule runs the reference's loop
+    (the library's host Simulation) to record them, and gradients are those of the map with its step
+    sequence held fixed.
   * ImplicitStep: a backward-Euler step x' = x + dt v(x', y, tau), solved by Newton's method with
     the dense Jacobian from the pair kernel, each Newton system by the library's GPU dense LU
     (d4_lu_factor_dense_dev); no reference counterpart (kept as an alternative map).  Its backward
@@ -295,36 +296,86 @@ def implicit_step(x, y, tau, dt, d4, tol=1e-12, max_iter=30, max_halvings=24):
 
 
 def _segment(x, y, tau, steps, dt, d4, tol, max_iter, method, opts):
-    for _ in range(steps):
+    """`steps` steps of dt, or the step sizes listed in `steps`."""
+    sizes = steps if isinstance(steps, (list, tuple)) else [dt] * steps
+    for h in sizes:
         if method == "theta":
-            x = theta_step(x, y, tau, dt, d4, opts)
+            x = theta_step(x, y, tau, h, d4, opts)
         else:
-            x = implicit_step(x, y, tau, dt, d4, tol, max_iter)
+            x = implicit_step(x, y, tau, h, d4, tol, max_iter)
     return x
 
 
 def accumulated_strain(x0, y, tau, T, dt, d4=None, checkpoint_every=16, tol=1e-12, max_iter=30, method="theta",
-                       opts=None):
-    """sum_i b_i (x_T,i - x_0,i) after T steps of dt from sorted positions (x0, y) at stress tau (a 0-d
-    tensor to differentiate by it): the reference's implicit step (method "theta", with ThetaOptions
-    opts) or backward Euler ("backward_euler", Newton to tol).  Only every `checkpoint_every`-th state
-    is kept for the backward pass; the steps in between are recomputed."""
+                       opts=None, steps=None):
+    """sum_i b_i (x_T,i - x_0,i) after T steps of dt -- or after the steps whose sizes `steps` lists (then T
+    and dt are ignored; reference_schedule gives the reference's own) -- from sorted positions (x0, y) at
+    stress tau (a 0-d tensor to differentiate by it): the reference's implicit step (method "theta", with
+    ThetaOptions opts) or backward Euler ("backward_euler", Newton to tol).  Only every
+    `checkpoint_every`-th state is kept for the backward pass; the steps in between are recomputed."""
     if method not in ("theta", "backward_euler"):
         raise ValueError(f"unknown method {method!r}")
     d4 = d4 if d4 is not None else _pkg().default_context()
     if not torch.is_tensor(tau):
         tau = torch.tensor(float(tau), dtype=torch.float64, device=x0.device)
+    sizes = list(steps) if steps is not None else [dt] * T
     x = x0
     done = 0
-    seg = checkpoint_every if checkpoint_every > 0 else T
-    while done < T:
-        k = min(seg, T - done)
+    seg = checkpoint_every if checkpoint_every > 0 else len(sizes)
+    while done < len(sizes):
+        chunk = sizes[done:done + seg]
         if checkpoint_every > 0 and torch.is_grad_enabled():
-            x = checkpoint(_segment, x, y, tau, k, dt, d4, tol, max_iter, method, opts, use_reentrant=False)
+            x = checkpoint(_segment, x, y, tau, chunk, dt, d4, tol, max_iter, method, opts, use_reentrant=False)
         else:
-            x = _segment(x, y, tau, k, dt, d4, tol, max_iter, method, opts)
-        done += k
+            x = _segment(x, y, tau, chunk, dt, d4, tol, max_iter, method, opts)
+        done += len(chunk)
     return (_signs(x0.numel(), x0.device) * (x - x0)).sum()
+
+
+def reference_schedule(x, y, tau, t_end, d4=None, opts=None, max_steps=1000000, progress=None, **sim_params):
+    """The step sizes the reference's loop accepts from sorted positions (x, y) at constant stress tau until
+    its time passes t_end: Simulation::run with its step doubling, error estimate and step-size control
+    (simulation.cpp:88-276, precision_handler.cpp:34-63), run by the library's host Simulation (d4_sim, its
+    passes on the GPU).  sim_params as 2d4_amd.sim (init_step_size, max_step_size, prec, dipole_prec, solver).
+    Returns (sizes, final sorted positions of that run); accumulated_strain(..., steps=sizes) replays the
+    accepted steps as a differentiable map.  progress(iterations, accepted, sim_time), if given, is called
+    after every iteration."""
+    import numpy as np
+    simmod = import_module("2d4_amd.sim")
+    d4 = d4 if d4 is not None else _pkg().default_context()
+    opts = opts if opts is not None else ThetaOptions()
+    xn = np.asarray(x.detach().cpu() if torch.is_tensor(x) else x, dtype=np.float64)
+    yn = np.asarray(y.detach().cpu() if torch.is_tensor(y) else y, dtype=np.float64)
+    n = xn.size
+    b = np.where(np.arange(n) < n // 2, 1.0, -1.0)
+    extra = {}
+    if opts.cutoff_mult is not None:
+        extra["cutoff_mult"] = float(opts.cutoff_mult)
+    sim = simmod.Simulation.from_arrays(d4, xn, yn, b, heaviside=opts.heaviside, wfunc=opts.wfunc,
+                                        stress=("fixed", float(tau), 0.0), time_limit=float(t_end), **extra,
+                                        **sim_params)
+    try:
+        start = sim.positions()
+        if not (np.array_equal(start[:, 0], xn) and np.array_equal(start[:, 1], yn)):
+            raise ValueError("positions must be in the reference's sorted order (b = +1 first, y descending)")
+        sizes = []
+        iterations = 0
+        while True:
+            st = sim.stats()
+            h = st["step_size"]  # the size this iteration tries
+            accepted, stopped = sim.step()
+            if stopped:
+                break
+            iterations += 1
+            if accepted:
+                sizes.append(h)
+                if len(sizes) > max_steps:
+                    raise RuntimeError(f"more than {max_steps} accepted steps before t = {t_end}")
+            if progress is not None:
+                progress(iterations, len(sizes), st["sim_time"] + (h if accepted else 0.0))
+        return sizes, sim.positions()
+    finally:
+        sim.close()
 
 
 def separated_config(n, seed, min_dist, device=None):

@@ -303,34 +303,52 @@ def c5_reference_step_check(T=4, n=2048, eps=1e-9, coord=7, seed=2048):
 
 
 def adjoint_bench(n, T, dt, every, method="theta"):
-    """Strain gradient through T implicit steps (2d4_amd/adjoint.py: the reference's step, or backward
-    Euler) at N = n: times the forward pass, the checkpointed forward and the backward."""
+    """Strain gradient through implicit steps (2d4_amd/adjoint.py) at N = n: T steps of dt of the reference's
+    step ("theta") or of backward Euler ("backward_euler"), or ("adaptive") the steps the reference's own
+    loop accepts until time T dt with its step at most dt (adjoint.reference_schedule: default precision,
+    GPU dense LU).  Times the forward pass, the checkpointed forward and the backward."""
     import torch
     adj = importlib.import_module("2d4_amd.adjoint")
     pkg = importlib.import_module("2d4_amd")
     ctx = pkg.Context(0)
     x, y = adj.separated_config(n, n, 0.5 / np.sqrt(n), device="cuda:0")
     tau = torch.tensor(0.2, dtype=torch.float64, device="cuda:0")
+    step_method = "backward_euler" if method == "backward_euler" else "theta"
+    sizes, schedule = [dt] * T, {}
+    if method == "adaptive":
+        t0 = time.perf_counter()
+        last = [t0]
+
+        def progress(it, acc, t):
+            if time.perf_counter() - last[0] > 10:
+                last[0] = time.perf_counter()
+                log(f"[bench] adjoint schedule: {it} iterations, {acc} accepted, t = {t:.4e} of {T * dt:.4e}")
+
+        sizes, _ = adj.reference_schedule(x, y, 0.2, T * dt, ctx, max_step_size=dt, solver=1, progress=progress)
+        schedule = {"accepted_steps": len(sizes), "min_step": min(sizes), "max_step": max(sizes),
+                    "schedule_s": time.perf_counter() - t0}
+        log(f"[bench] adjoint schedule: {schedule}")
     adj.reset_stats()
     t0 = time.perf_counter()
     with torch.no_grad():
         xs = x
-        for k in range(0, T, every):
-            xs = adj._segment(xs, y, tau, min(every, T - k), dt, ctx, 1e-12, 30, method, None)
-            log(f"[bench] adjoint forward {min(k + every, T)} steps: {time.perf_counter() - t0:.1f} s {adj.stats}")
+        for k in range(0, len(sizes), every):
+            xs = adj._segment(xs, y, tau, sizes[k:k + every], dt, ctx, 1e-12, 30, step_method, None)
+            log(f"[bench] adjoint forward {min(k + every, len(sizes))} steps: {time.perf_counter() - t0:.1f} s")
     t1 = time.perf_counter()
     fwd_stats = dict(adj.stats)
     x0 = x.clone().requires_grad_()
     yy = y.clone().requires_grad_()
     tt = tau.clone().requires_grad_()
-    L = adj.accumulated_strain(x0, yy, tt, T, dt, ctx, checkpoint_every=every, method=method)
+    L = adj.accumulated_strain(x0, yy, tt, T, dt, ctx, checkpoint_every=every, method=step_method, steps=sizes)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     L.backward()
     torch.cuda.synchronize()
     t3 = time.perf_counter()
-    return {"n": n, "T": T, "dt": dt, "checkpoint_every": every, "tau": 0.2, "method": method,
-            "forward_s": t1 - t0, "forward_checkpointed_s": t2 - t1, "backward_s": t3 - t2,
+    return {"n": n, "T": T, "dt": dt, "checkpoint_every": every, "tau": 0.2, "method": method, **schedule,
+            "steps": len(sizes), "forward_s": t1 - t0, "forward_checkpointed_s": t2 - t1, "backward_s": t3 - t2,
+            "ms_per_step_forward": (t1 - t0) * 1e3 / len(sizes), "ms_per_step_backward": (t3 - t2) * 1e3 / len(sizes),
             "forward_stats": fwd_stats, "L": L.item(), "max_abs_dL_dx0": x0.grad.abs().max().item(),
             "max_abs_dL_dy": yy.grad.abs().max().item(), "dL_dtau": tt.grad.item(),
             "config": "uniform positions with a minimum separation of 0.5/sqrt(N), sorted"}
@@ -368,8 +386,9 @@ def main():
     ap.add_argument("--ens-only", action="store_true",
                     help="only the batched ensemble (no plain comparison, no pair-pass headline), for profiling")
     ap.add_argument("--adjoint", type=str, default="",
-                    help="only the adjoint measurement: N,T,dt[,method] (e.g. 2048,256,1e-3; method theta, the "
-                         "reference's step, or backward_euler), checkpoints every 16 steps")
+                    help="only the adjoint measurement: N,T,dt[,method] (e.g. 2048,256,1e-3; method theta: T steps of "
+                         "dt of the reference's step, adaptive: the steps the reference's loop accepts until T dt, "
+                         "backward_euler), checkpoints every 16 steps")
     ap.add_argument("--c5-check", type=int, default=0,
                     help="only the C5 configuration through the reference's own step for this many steps")
     ap.add_argument("--calib-copy-mib", type=int, default=0,

@@ -210,3 +210,46 @@ def test_checkpointing_does_not_change_gradients(ctx):
     b = _strain_grad(ctx, xy, T, dt, OPTS["taper"], checkpoint_every=0)
     for u, v in zip(a, b):
         np.testing.assert_allclose(u, v, rtol=1e-12, atol=0.0)
+
+
+def test_reference_schedule_replay(ctx):
+    # the step sizes the reference's loop accepts (its error control, through the library's host
+    # Simulation), replayed through the differentiable map: the same end point, and gradients of the
+    # replayed map (schedule held fixed) against central differences of it
+    n = 32
+    xy = np.stack(adj.separated_config(n, 97, 0.5 / np.sqrt(n)), axis=1)
+    t_end = 2e-3
+    sizes, end = adj.reference_schedule(xy[:, 0], xy[:, 1], TAU, t_end, ctx, init_step_size=1e-5,
+                                        max_step_size=1e-3)
+    print(f"{len(sizes)} accepted steps, from {min(sizes):.3e} to {max(sizes):.3e}, total {sum(sizes):.6e}")
+    assert len(sizes) >= 3 and sum(sizes) >= t_end
+    x0 = _dev(xy[:, 0]).requires_grad_()
+    y = _dev(xy[:, 1]).requires_grad_()
+    tau = torch.tensor(TAU, dtype=torch.float64, device="cuda:0", requires_grad=True)
+    L = adj.accumulated_strain(x0, y, tau, 0, 0.0, ctx, checkpoint_every=4, steps=sizes)
+    L.backward()
+    with torch.no_grad():
+        xt = x0.detach()
+        for h in sizes:
+            xt = adj.theta_step(xt, y.detach(), tau.detach(), h, ctx)
+    d = xt.cpu().numpy() - end[:, 0]
+    d -= np.rint(d)
+    print(f"replay vs the run: max |dx| {np.abs(d).max():.3e}")
+    assert np.abs(d).max() <= 1e-9
+
+    def loss(a):
+        with torch.no_grad():
+            return adj.accumulated_strain(_dev(a[:, 0]), _dev(a[:, 1]), TAU, 0, 0.0, ctx, checkpoint_every=0,
+                                          steps=sizes).item()
+
+    h = 1e-6
+    g = x0.grad.cpu().numpy()
+    for k in (3, 17, 29):
+        def dd(hh):
+            p_, m_ = xy.copy(), xy.copy()
+            p_[k, 0] += hh
+            m_[k, 0] -= hh
+            return (loss(p_) - loss(m_)) / (2 * hh)
+        f = (4 * dd(h / 2) - dd(h)) / 3
+        print(f"x[{k}] adjoint {g[k]:.9e} fd {f:.9e}")
+        assert abs(f - g[k]) <= 1e-6 * max(abs(g[k]), 1e-3 * np.abs(g).max())
