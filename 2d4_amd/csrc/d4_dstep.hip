@@ -90,6 +90,9 @@ struct DsArgs {
     const int* npad;
     const int* flag;
     const int* sing;
+    const int* lu_aux;  // the LU's orders and envelopes (ord then inv; last, top, first per block)
+    const int* ordoff;  // per realization: its order in lu_aux, or -1 (natural)
+    const int* envoff;  // per realization: its envelope in lu_aux, or -1 (whole matrix)
 };
 
 __device__ RealParams pass_params(const DsArgs& a, int r, double h, double tA, double tB)
@@ -137,7 +140,10 @@ __global__ void ds_begin_kernel(DsArgs a)
 // The matrix the LU factors, from the dense pair pass's M (column c of realization r per block, identity
 // beyond n_r), unfused as simulation.cpp:516-522 rounds it: A = M (1 + d_row)/2, + 1 on the diagonal;
 // rescaled (stage II, simulation.cpp:546-598): A/2 f_row off the diagonal, 1 + (A - 1)/2 f_row on it.
-// only_flagged: just the realizations whose unpivoted factorization was flagged.
+// In the realization's LU order (store row/column i = dislocation ord[i]) and, on the first pass, only
+// inside its envelope (blocks top .. last of each column's block: every entry the factorization, the
+// check and the solves read).  only_flagged: just the realizations whose unpivoted factorization was
+// flagged, whole (the pivoted factorization reads every entry).
 __global__ __launch_bounds__(256) void ds_assemble_kernel(DsArgs a, int rescaled, int only_flagged)
 {
 #pragma clang fp contract(off)
@@ -145,17 +151,26 @@ __global__ __launch_bounds__(256) void ds_assemble_kernel(DsArgs a, int rescaled
     const int np = a.npad[r];
     if (c >= np || (only_flagged && !a.flag[r])) return;
     const int base = a.off[r], n = a.off[r + 1] - base;
+    const int* ord = a.ordoff[r] >= 0 ? a.lu_aux + a.ordoff[r] : nullptr;
+    int r0 = 0, r1 = np;
+    if (!only_flagged && a.envoff[r] >= 0) {
+        const int nb = np / kTile, bc = c / kTile;
+        r0 = a.lu_aux[a.envoff[r] + nb + bc] * kTile;
+        r1 = min(np, (a.lu_aux[a.envoff[r] + bc] + 1) * kTile);
+    }
     double* col = a.store + a.aoff[r] + (size_t)c * np;
-    const double* mc = a.M + a.moff[r] + (size_t)c * n;
-    for (int row = threadIdx.x; row < np; row += blockDim.x) {
+    const int oc = (ord && c < n) ? ord[c] : c;
+    const double* mc = a.M + a.moff[r] + (size_t)oc * n;
+    for (int row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
         double v;
         if (row >= n || c >= n) {
             v = row == c ? 1.0 : 0.0;
         } else {
-            v = mc[row] * ((1 + a.dvec[base + row]) / 2);
+            const int orow = ord ? ord[row] : row;
+            v = mc[orow] * ((1 + a.dvec[base + orow]) / 2);
             if (row == c) v = v + 1;
             if (rescaled) {
-                const double f = a.fac[base + row];
+                const double f = a.fac[base + orow];
                 v = row == c ? 1 + (v - 1) / 2 * f : v / 2 * f;
             }
         }
@@ -586,7 +601,35 @@ int d4_dstep_create(d4_ctx* ctx, int R, const int* off, const double* x, const d
     s->nslots = nslots;
     std::vector<int> nr(R);
     for (int r = 0; r < R; ++r) nr[r] = off[r + 1] - off[r];
-    s->lu.reset(d4host::new_dense_lu_graph(ctx->device, R, nr.data(), b_off.data()));
+    // Each realization's LU order and envelope, fixed for the run.  Rows sorted by |y| (the distance from
+    // the line y = 0; y never changes): a pair that can hold a matrix entry is closer than the storage
+    // radius, so closer than it in periodic y, so its keys differ by less than the radius -- for any x
+    // (key_band_envelope).  Kept where the radius cannot grow during the run (no cutoff change,
+    // simulation.cpp:215-219) and the envelope needs at most half the update tiles of the whole matrix.
+    std::vector<std::vector<int>> ords(R);
+    std::vector<d4host::BlockEnvelope> envs(R);
+    for (int r = 0; r < R; ++r) {
+        const int n = nr[r], np = npad[r];
+        if (par[r].speed_threshold > 0) continue;
+        const double reach = std::sqrt((par[r].heaviside ? 1.0 : 49.0) * st[r].cut_sq);
+        if (!(reach < 0.5)) continue;  // the whole cell in y
+        std::vector<double> key(n);
+        for (int i = 0; i < n; ++i) key[i] = std::fabs(xy[off[r] + i].y);
+        std::vector<int> o(n);
+        for (int i = 0; i < n; ++i) o[i] = i;
+        std::stable_sort(o.begin(), o.end(), [&](int u, int v) { return key[u] < key[v]; });
+        std::vector<double> ks(n);
+        for (int i = 0; i < n; ++i) ks[i] = key[o[i]];
+        // the margin covers the rounding of d^2 = dx^2 + dy^2 in the pass's storage test
+        d4host::BlockEnvelope e = d4host::key_band_envelope(n, np, kTile, ks.data(), reach * (1 + 1e-12) + 1e-300);
+        const long long nb = np / kTile;
+        long long dense = 0;
+        for (long long q = 0; q < nb; ++q) dense += (nb - 1 - q) * (nb - 1 - q);
+        if (2 * e.tiles > dense) continue;
+        ords[r] = std::move(o);
+        envs[r] = std::move(e);
+    }
+    s->lu.reset(d4host::new_dense_lu_graph(ctx->device, R, nr.data(), b_off.data(), &ords, &envs));
     if (!s->lu) return ctx->fail(D4_ERR_INVALID, "device step: the LU batch does not fit this device");
     for (int r = 0; r < R; ++r) aoff[r] = s->lu->a_off(r);
     // the tile table (realization-major, each realization's upper triangle row by row) and reduce blocks
@@ -630,6 +673,10 @@ int d4_dstep_create(d4_ctx* ctx, int R, const int* off, const double* x, const d
     if (!rc) rc = e.alloc(a.store, (size_t)e.lu->store_size(), "LU store");
     if (!rc) rc = e.alloc(aoff_d, R, "LU offsets");
     if (!rc) rc = e.alloc(npad_d, R, "LU orders");
+    int* ordoff_d = nullptr;
+    int* envoff_d = nullptr;
+    if (!rc) rc = e.alloc(ordoff_d, R, "LU order offsets");
+    if (!rc) rc = e.alloc(envoff_d, R, "LU envelope offsets");
     if (!rc) rc = e.alloc(e.flag_d, R, "LU flags");
     if (!rc) rc = e.alloc(e.sing_d, R, "LU zero pivots");
     if (!rc) rc = e.alloc(e.tiles, tiles.size(), "tile table");
@@ -646,6 +693,14 @@ int d4_dstep_create(d4_ctx* ctx, int R, const int* off, const double* x, const d
     a.npad = npad_d;
     a.flag = e.flag_d;
     a.sing = e.sing_d;
+    a.lu_aux = e.lu->aux();
+    a.ordoff = ordoff_d;
+    a.envoff = envoff_d;
+    std::vector<int> ordoff(R), envoff(R);
+    for (int r = 0; r < R; ++r) {
+        ordoff[r] = e.lu->ord_off(r);
+        envoff[r] = e.lu->env_off(r);
+    }
     bool ok = true;
     auto up = [&](void* dst, const void* src, size_t bytes) {
         ok = ok && hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
@@ -658,6 +713,8 @@ int d4_dstep_create(d4_ctx* ctx, int R, const int* off, const double* x, const d
     for (double2* q : {a.x0, a.big, a.s1, a.s2}) up(q, xy.data(), sizeof(double2) * T);
     up(a.tol, tol.data(), sizeof(double) * T);
     up(moff_d, moff.data(), sizeof(long long) * R);
+    up(ordoff_d, ordoff.data(), sizeof(int) * R);
+    up(envoff_d, envoff.data(), sizeof(int) * R);
     up(aoff_d, aoff.data(), sizeof(long long) * R);
     up(npad_d, npad.data(), sizeof(int) * R);
     up(e.tiles, tiles.data(), sizeof(TileDesc) * tiles.size());

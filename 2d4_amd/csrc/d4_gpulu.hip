@@ -281,7 +281,8 @@ __device__ inline double lane_value(double v, int lane)  // v of `lane` (uniform
 // global scratch; the arithmetic is the same either way.  A reordered matrix
 // gathers b through ord and scatters x back through it; with an envelope the
 // updates stop at its bounds, past which the factors' columns are zero (a
-// zero sum leaves x_r as it is, so the bits are those of the full sweep).
+// zero sum leaves x_r as it is, so the bits are those of the full sweep); a
+// matrix whose full flag is set is swept whole.
 // Latency: the next diagonal block is staged (by waves 1..15, into the other
 // of two LDS buffers) while wave 0 solves the current one; wave 0 reads its
 // entries of the block kAhead steps before their use; a row update issues
@@ -291,7 +292,8 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
                                                                   const double* __restrict__ A,
                                                                   const int* __restrict__ perm, double* __restrict__ b,
                                                                   double* __restrict__ scratch,
-                                                                  const int* __restrict__ aux)
+                                                                  const int* __restrict__ aux,
+                                                                  const int* __restrict__ full)
 {
     constexpr int kAhead = 8, kBatch = 32;
     extern __shared__ double lds[];
@@ -303,7 +305,8 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
     double* Dbuf = lds;  // two kSB x kSB diagonal blocks, column-major
     double* x = kLds ? lds + 2 * kSB * kSB : scratch + m.b_off;
     const int* ord = m.ord_off >= 0 ? aux + m.ord_off : nullptr;
-    const int* env = m.env_off >= 0 ? aux + m.env_off : nullptr;
+    // full[k] set: matrix k was refactored with pivoting, whose factors are not held to its envelope
+    const int* env = (m.env_off >= 0 && !(full && full[blockIdx.x])) ? aux + m.env_off : nullptr;
     const int nb = m.npad / kSB;
     const int nblk = (n + kSB - 1) / kSB;
     // diagonal block number s of the two sweeps (0..2 nblk - 1) into buffer s & 1,
@@ -1192,11 +1195,11 @@ private:
         if (npad_max_ <= lds_order_)
             hipLaunchKernelGGL(lu_solve_kernel<true>, dim3(nsolve_), dim3(kSolveThreads),
                                sizeof(double) * (2 * kSB * kSB + npad_max_), st, sdesc_.p, A_.p, ipiv_.p, b_.p,
-                               b_.p + total, aux_.p);
+                               b_.p + total, aux_.p, nullptr);
         else
             hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(nsolve_), dim3(kSolveThreads),
                                sizeof(double) * 2 * kSB * kSB, st, sdesc_.p, A_.p, ipiv_.p, b_.p, b_.p + total,
-                               aux_.p);
+                               aux_.p, nullptr);
         return hipGetLastError() != hipSuccess ? -1 : 0;
     }
 
@@ -1263,11 +1266,13 @@ public:
     GraphLU() = default;
     ~GraphLU() override
     {
-        for (void* p : {(void*)fd_d_, (void*)pre_d_, (void*)md_d_, (void*)sd_d_, (void*)perm_d_, (void*)scratch_d_})
+        for (void* p : {(void*)fd_d_, (void*)pre_d_, (void*)md_d_, (void*)sd_d_, (void*)perm_d_, (void*)scratch_d_,
+                        (void*)aux_d_})
             if (p) (void)hipFree(p);
     }
 
-    int init(int device, int K, const int* n, const long long* b_off)
+    int init(int device, int K, const int* n, const long long* b_off, const std::vector<std::vector<int>>* ord,
+             const std::vector<BlockEnvelope>* env)
     {
         if (K <= 0) return -1;
         if (hipSetDevice(device) != hipSuccess) return -2;
@@ -1277,6 +1282,9 @@ public:
         std::vector<MatDesc> md(K);
         std::vector<SolveDesc> sd(K);
         a_off_.resize(K);
+        ord_off_.assign(K, -1);
+        env_off_.assign(K, -1);
+        std::vector<int> aux;
         long long a = 0, poff = 0, btot = 0;
         for (int k = 0; k < K; ++k) {
             const int npad = gpu_lu_padded(n[k]);
@@ -1286,23 +1294,48 @@ public:
             m.a_off = a_off_[k] = a;
             m.n = n[k];
             m.npad = npad;
-            m.ipiv_off = m.ord_off = m.env_off = -1;
-            sd[k] = SolveDesc{a, n[k], npad, (int)b_off[k], (int)poff, -1, -1};
+            m.ipiv_off = -1;
+            if (ord && !(*ord)[k].empty()) {  // ord then inv
+                const std::vector<int>& o = (*ord)[k];
+                if ((int)o.size() != n[k]) return -1;
+                ord_off_[k] = (int)aux.size();
+                aux.insert(aux.end(), o.begin(), o.end());
+                std::vector<int> inv(n[k], -1);
+                for (int i = 0; i < n[k]; ++i) {
+                    if (o[i] < 0 || o[i] >= n[k] || inv[o[i]] >= 0) return -1;
+                    inv[o[i]] = i;
+                }
+                aux.insert(aux.end(), inv.begin(), inv.end());
+            }
+            if (env && !(*env)[k].last.empty()) {  // last, top, first per block
+                const BlockEnvelope& e = (*env)[k];
+                if ((int)e.last.size() != npad / kFB) return -1;
+                env_off_[k] = (int)aux.size();
+                aux.insert(aux.end(), e.last.begin(), e.last.end());
+                aux.insert(aux.end(), e.top.begin(), e.top.end());
+                aux.insert(aux.end(), e.first.begin(), e.first.end());
+            }
+            m.ord_off = ord_off_[k];
+            m.env_off = env_off_[k];
+            sd[k] = SolveDesc{a, n[k], npad, (int)b_off[k], (int)poff, ord_off_[k], env_off_[k]};
             a += (long long)npad * npad;
             poff += npad;
             btot = std::max(btot, b_off[k] + n[k]);
             npad_max_ = std::max(npad_max_, npad);
         }
-        if (poff > INT_MAX || btot > INT_MAX) return -1;
+        if (poff > INT_MAX || btot > INT_MAX || aux.size() > (size_t)INT_MAX) return -1;
         store_ = a;
         btot_ = btot;
-        // the factorization plan: matrices largest first, each dense
+        // the factorization plan: matrices largest first, each within its envelope
         std::vector<int> order(K);
         for (int k = 0; k < K; ++k) order[k] = k;
         std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return md[x].npad > md[y].npad; });
         std::vector<FacDesc> fd;
-        for (int k : order) fd.push_back({md[k].a_off, md[k].npad, 0});
-        std::vector<const int*> last(K, nullptr);
+        std::vector<const int*> last;
+        for (int k : order) {
+            fd.push_back({md[k].a_off, md[k].npad, 0});
+            last.push_back(env_off_[k] >= 0 ? (*env)[k].last.data() : nullptr);
+        }
         std::vector<int> pre;
         if (!plan_lu_steps(fd, last, steps_, pre)) return -1;
         lds_order_ = std::max(0, (lds / 8 - 2 * kSB * kSB) / 256 * 256);
@@ -1311,26 +1344,32 @@ public:
             hipMalloc(&pre_d_, sizeof(int) * std::max<size_t>(pre.size(), 1)) != hipSuccess ||
             hipMalloc(&md_d_, sizeof(MatDesc) * K) != hipSuccess || hipMalloc(&sd_d_, sizeof(SolveDesc) * K) != hipSuccess ||
             hipMalloc(&perm_d_, sizeof(int) * (size_t)poff) != hipSuccess ||
-            hipMalloc(&scratch_d_, sizeof(double) * scratch) != hipSuccess)
+            hipMalloc(&scratch_d_, sizeof(double) * scratch) != hipSuccess ||
+            hipMalloc(&aux_d_, sizeof(int) * std::max<size_t>(aux.size(), 1)) != hipSuccess)
             return -2;
         if (hipMemcpy(fd_d_, fd.data(), sizeof(FacDesc) * fd.size(), hipMemcpyHostToDevice) != hipSuccess ||
             (!pre.empty() && hipMemcpy(pre_d_, pre.data(), sizeof(int) * pre.size(), hipMemcpyHostToDevice) != hipSuccess) ||
             hipMemcpy(md_d_, md.data(), sizeof(MatDesc) * K, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(sd_d_, sd.data(), sizeof(SolveDesc) * K, hipMemcpyHostToDevice) != hipSuccess)
+            hipMemcpy(sd_d_, sd.data(), sizeof(SolveDesc) * K, hipMemcpyHostToDevice) != hipSuccess ||
+            (!aux.empty() && hipMemcpy(aux_d_, aux.data(), sizeof(int) * aux.size(), hipMemcpyHostToDevice) != hipSuccess))
             return -2;
         return 0;
     }
 
     long long a_off(int k) const override { return a_off_[k]; }
     long long store_size() const override { return store_; }
+    const int* aux() const override { return aux_d_; }
+    int ord_off(int k) const override { return ord_off_[k]; }
+    int env_off(int k) const override { return env_off_[k]; }
 
     int factor_unpivoted(double* store, int* flag, hipStream_t st) override
     {
+        flag_ = flag;  // the solves sweep the matrices refactored with pivoting whole
         if (d4::launch_fill_int(flag, 0, K_, st) != hipSuccess) return -2;
         if (launch_lu_steps(steps_, fd_d_, pre_d_, store, st)) return -2;
         for (int k0 = 0; k0 < K_; k0 += kGridY)
             hipLaunchKernelGGL(multiplier_check_kernel, dim3(npad_max_, std::min(kGridY, K_ - k0)), dim3(256), 0, st,
-                               md_d_ + k0, store, flag + k0, nullptr);
+                               md_d_ + k0, store, flag + k0, aux_d_);
         return hipGetLastError() != hipSuccess ? -2 : 0;
     }
 
@@ -1346,10 +1385,10 @@ public:
         if (npad_max_ <= lds_order_)
             hipLaunchKernelGGL(lu_solve_kernel<true>, dim3(K_), dim3(kSolveThreads),
                                sizeof(double) * (2 * kSB * kSB + npad_max_), st, sd_d_, store, perm_d_, b, scratch_d_,
-                               nullptr);
+                               aux_d_, flag_);
         else
             hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(K_), dim3(kSolveThreads), sizeof(double) * 2 * kSB * kSB,
-                               st, sd_d_, store, perm_d_, b, scratch_d_, nullptr);
+                               st, sd_d_, store, perm_d_, b, scratch_d_, aux_d_, flag_);
         return hipGetLastError() != hipSuccess ? -2 : 0;
     }
 
@@ -1358,6 +1397,7 @@ private:
     int K_ = 0, npad_max_ = 0, lds_order_ = 0;
     long long store_ = 0, btot_ = 0;
     std::vector<long long> a_off_;
+    std::vector<int> ord_off_, env_off_;
     std::vector<LuStep> steps_;
     FacDesc* fd_d_ = nullptr;
     int* pre_d_ = nullptr;
@@ -1365,14 +1405,17 @@ private:
     SolveDesc* sd_d_ = nullptr;
     int* perm_d_ = nullptr;
     double* scratch_d_ = nullptr;
+    int* aux_d_ = nullptr;
+    const int* flag_ = nullptr;
 };
 
 }  // namespace
 
-DenseLUGraph* new_dense_lu_graph(int device, int K, const int* n, const long long* b_off)
+DenseLUGraph* new_dense_lu_graph(int device, int K, const int* n, const long long* b_off,
+                                 const std::vector<std::vector<int>>* ord, const std::vector<BlockEnvelope>* env)
 {
     GraphLU* g = new GraphLU();
-    if (g->init(device, K, n, b_off) != 0) {
+    if (g->init(device, K, n, b_off, ord, env) != 0) {
         delete g;
         return nullptr;
     }

@@ -154,20 +154,11 @@ std::vector<int> rcm_order(int n, const int* Ap, const int* Ai)
     return order;
 }
 
-BlockEnvelope block_envelope(int n, int npad, int bs, const int* Ap, const int* Ai, const int* inv)
+// first, last and top from f(i), the lowest column of row i's entries (f(i) <= i, npad entries)
+static BlockEnvelope envelope_from_rows(const std::vector<int>& f, int npad, int bs)
 {
     BlockEnvelope e;
     const int nb = npad / bs;
-    std::vector<int> f(npad);
-    for (int i = 0; i < npad; ++i) f[i] = i;
-    for (int c = 0; c < n; ++c) {
-        const int nc = inv ? inv[c] : c;
-        for (int k = Ap[c]; k < Ap[c + 1]; ++k) {
-            const int nr = inv ? inv[Ai[k]] : Ai[k];
-            const int lo = std::min(nr, nc), hi = std::max(nr, nc);
-            f[hi] = std::min(f[hi], lo);
-        }
-    }
     e.first.assign(nb, 0);
     e.last.assign(nb, 0);
     e.top.assign(nb, 0);
@@ -185,6 +176,31 @@ BlockEnvelope block_envelope(int n, int npad, int bs, const int* Ap, const int* 
     }
     for (int p = 0; p < nb; ++p) e.tiles += (long long)(e.last[p] - p) * (e.last[p] - p);
     return e;
+}
+
+BlockEnvelope block_envelope(int n, int npad, int bs, const int* Ap, const int* Ai, const int* inv)
+{
+    std::vector<int> f(npad);
+    for (int i = 0; i < npad; ++i) f[i] = i;
+    for (int c = 0; c < n; ++c) {
+        const int nc = inv ? inv[c] : c;
+        for (int k = Ap[c]; k < Ap[c + 1]; ++k) {
+            const int nr = inv ? inv[Ai[k]] : Ai[k];
+            const int lo = std::min(nr, nc), hi = std::max(nr, nc);
+            f[hi] = std::min(f[hi], lo);
+        }
+    }
+    return envelope_from_rows(f, npad, bs);
+}
+
+BlockEnvelope key_band_envelope(int n, int npad, int bs, const double* key, double reach)
+{
+    std::vector<int> f(npad);
+    for (int i = 0; i < npad; ++i) f[i] = i;
+    // row i can hold entries from the first row whose key is within reach below its own
+    for (int i = 0; i < n; ++i)
+        f[i] = std::min(i, (int)(std::lower_bound(key, key + n, key[i] - reach) - key));
+    return envelope_from_rows(f, npad, bs);
 }
 
 bool band_order(int n, int npad, int bs, const int* Ap, const int* Ai, std::vector<int>& ord, std::vector<int>& inv,

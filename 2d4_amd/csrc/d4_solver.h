@@ -131,6 +131,14 @@ struct BlockEnvelope {
 };
 BlockEnvelope block_envelope(int n, int npad, int bs, const int* Ap, const int* Ai, const int* inv);
 
+// The block envelope (as block_envelope) of every matrix whose rows and columns, sorted by a key
+// (key[0] <= key[1] <= ...), couple only where the keys differ by less than reach: a bound that holds for
+// any pattern with that property, so it can be fixed before the pattern is known.  The implicit step's
+// matrix couples dislocations closer than the storage radius, hence closer than it in y; with the key
+// |y - y0| (the distance from a line, periodic) the pairs that may couple differ in key by less than the
+// radius, for any x.
+BlockEnvelope key_band_envelope(int n, int npad, int bs, const double* key, double reach);
+
 // The symmetric order of a matrix for the GPU dense LU: reverse Cuthill-McKee
 // when its envelope needs at most half the trailing-update tiles of the given
 // order, else the given order.  Returns true and fills ord (ord[new] = old)
@@ -204,10 +212,21 @@ public:
     virtual ~DenseLUGraph() = default;
     virtual long long a_off(int k) const = 0;
     virtual long long store_size() const = 0;  // doubles
+    // the device array of orders and envelopes, and matrix k's places in it (-1: natural order / whole
+    // matrix), laid out as the CSC path's aux: ord (new -> old) then inv; last, top, first per block
+    virtual const int* aux() const = 0;
+    virtual int ord_off(int k) const = 0;
+    virtual int env_off(int k) const = 0;
     virtual int factor_unpivoted(double* store, int* flag, hipStream_t st) = 0;
     virtual int factor_pivoted(double* store, const int* flag, int* singular, hipStream_t st) = 0;
     virtual int solve(const double* store, double* b, hipStream_t st) = 0;
 };
-DenseLUGraph* new_dense_lu_graph(int device, int K, const int* n, const long long* b_off);
+// ord[k] (ord[new] = old; empty: the natural order) and env[k] (empty last: the whole matrix) are each
+// matrix's symmetric order and block envelope, fixed for every factorization; either pointer may be null.
+// The caller assembles each matrix in its order, at least inside its envelope (blocks top .. last of each
+// column's block), and in full before factor_pivoted for the matrices it flags.
+DenseLUGraph* new_dense_lu_graph(int device, int K, const int* n, const long long* b_off,
+                                 const std::vector<std::vector<int>>* ord = nullptr,
+                                 const std::vector<BlockEnvelope>* env = nullptr);
 
 }  // namespace d4host

@@ -230,7 +230,8 @@ def device_step_bench(pkg, ctx, iters=32):
     ens = importlib.import_module("2d4_amd.ensemble")
     n = 1024
     out = {"workload": "N=1024 per realization, uniform, heaviside cutoff 0.05, tau=0.1, adaptive (C2)",
-           "matrix": "dense n x n per realization, GPU dense LU in natural order (u = 0.001 threshold pivoting)"}
+           "matrix": "dense n x n per realization; GPU dense LU (u = 0.001 threshold pivoting) in a fixed order by |y| "
+                     "inside the band envelope that order fixes for the heaviside storage radius"}
     for R in (1, 64):
         rng = np.random.default_rng(77)
         x = rng.uniform(-0.5, 0.5, R * n)
@@ -260,6 +261,32 @@ def device_step_bench(pkg, ctx, iters=32):
             row["host_driven_ensemble_realization_steps_per_s"] = R * 8 / (time.perf_counter() - t0)
             e.close()
         out[f"R{R}"] = row
+    # the bench ensemble's own workload (ensemble_bench: R = 256, N_r even in [256, 4096], heaviside cutoff 0.05,
+    # tau = 0.05, seed 4242) as one captured graph per lockstep iteration
+    R, seed = 256, 4242
+    rng = np.random.default_rng(seed)
+    sizes = 2 * rng.integers(128, 2049, R)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    T = int(off[-1])
+    x = rng.uniform(-0.5, 0.5, T)
+    y = rng.uniform(-0.5, 0.5, T)
+    b = np.concatenate([np.where(np.arange(m) % 2 == 0, -1.0, 1.0) for m in sizes])
+    params = [dict(cutoff_mult=0.05 * float(np.sqrt(m)), heaviside=True, stress=("fixed", 0.05, 0.0)) for m in sizes]
+    t0 = time.perf_counter()
+    s = dstep.DeviceSimulation(ctx, off, x, y, b, params)
+    tc = time.perf_counter() - t0
+    s.run(1)  # the first iteration (graph capture) untimed, as the ensemble's
+    t0 = time.perf_counter()
+    s.run(2)
+    t = time.perf_counter() - t0
+    st = s.stats()
+    s.close()
+    out["ensemble_R256"] = {"workload": "the bench ensemble (R=256, N_r even in [256, 4096], heaviside cutoff 0.05, "
+                                        "tau=0.05, seed 4242)", "create_s": tc, "iterations": 2,
+                            "ms_per_iteration": t * 1e3 / 2, "realization_steps_per_s": R * 2 / t,
+                            "accepted": int(sum(q["successful_steps"] for q in st)),
+                            "failed": int(sum(q["failed_steps"] for q in st))}
+    log(f"[bench] device step, bench ensemble: {out['ensemble_R256']}")
     return out
 
 
@@ -379,6 +406,8 @@ def main():
                     help="with --ens-only: 1 = GPU dense LU (default), 0 = host sparse LU for every realization")
     ap.add_argument("--no-ens", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-step", action="store_true", help="skip the N=1024 implicit-step measurement")
+    ap.add_argument("--dstep-only", action="store_true",
+                    help="only the device-resident step measurement (C2 at 1 and 64 realizations, the bench ensemble)")
     ap.add_argument("--step-only", action="store_true",
                     help="only the N=1024 implicit-step measurement, without its CPU leg (for profiling)")
     ap.add_argument("--step-solver", type=int, default=-1, choices=(-1, 0, 1),
@@ -425,6 +454,9 @@ def main():
         solvers = (0, 1) if args.step_solver < 0 else (args.step_solver,)
         print(json.dumps({"implicit_step": implicit_step_bench(pkg, sctx, with_cpu=False, solvers=solvers)}),
               flush=True)
+        return
+    if args.dstep_only:
+        print(json.dumps({"device_step": device_step_bench(pkg, pkg.Context(0))}), flush=True)
         return
     if args.ens_only:
         ectx = pkg.Context(0)
@@ -528,8 +560,13 @@ def main():
     if not args.no_step:
         log("[bench] implicit step, N=1024 (GPU, then the CPU loop)")
         step = implicit_step_bench(pkg, ctx, with_cpu=not args.no_cpu)
-        log("[bench] device-resident implicit step, N=1024, 1 and 64 realizations")
+        log("[bench] device-resident implicit step, N=1024, 1 and 64 realizations; the bench ensemble")
         dstep_res = device_step_bench(pkg, ctx)
+        if ensemble is not None and "plain_steps_per_s" in ensemble and args.ens_r == 256:
+            # the same workload as one captured graph per lockstep iteration (the device-resident step)
+            g = dstep_res["ensemble_R256"]["realization_steps_per_s"]
+            ensemble["graph_step_steps_per_s"] = g
+            ensemble["graph_step_ratio_vs_plain"] = g / ensemble["plain_steps_per_s"]
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_pair_dense_n16384.json")

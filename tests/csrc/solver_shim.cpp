@@ -30,6 +30,19 @@ extern "C" int ts_band_order(int n, int npad, int bs, const int* Ap, const int* 
     return re ? 1 : 0;
 }
 
+// the envelope fixed by sorted keys that couple only within reach (host code)
+extern "C" void ts_key_band_envelope(int n, int npad, int bs, const double* key, double reach, int* first, int* last,
+                                     int* top, long long* tiles)
+{
+    const d4host::BlockEnvelope e = d4host::key_band_envelope(n, npad, bs, key, reach);
+    for (int b = 0; b < npad / bs; ++b) {
+        first[b] = e.first[b];
+        last[b] = e.last[b];
+        top[b] = e.top[b];
+    }
+    *tiles = e.tiles;
+}
+
 extern "C" void ts_rcm_order(int n, const int* Ap, const int* Ai, int* ord)
 {
     const std::vector<int> o = d4host::rcm_order(n, Ap, Ai);

@@ -87,3 +87,32 @@ def test_c4_ensemble_two_iterations(ctx, orc):
           f"vs the oracle's loop {dev:.3e}", flush=True)
     assert (rst.ok_steps, rst.failed_steps) == (gst["successful_steps"], gst["failed_steps"]) == (1, rst.failed_steps)
     assert dev <= 1e-9
+
+
+def test_c4_device_step_matches_host_driven_ensemble(ctx):
+    # the same workload as one captured graph per lockstep iteration (include/d4_dstep.h: dense step matrices,
+    # the LU in a fixed order by |y| inside the band that order fixes for the heaviside radius) against the
+    # host-driven ensemble (CSC, per-pattern reverse Cuthill-McKee order): another LU order, so the same
+    # accept/reject decisions and positions and strains to rounding, not bit for bit
+    dstep = import_module("2d4_amd.dstep")
+    sizes, off, x, y, b, params = _c4()
+    e = ens.Ensemble(ctx, off, x, y, b, params)
+    e.step(2)
+    pos_h, st_h = e.positions(), e.stats()
+    e.close()
+    print("[c4] device step", flush=True)
+    s = dstep.DeviceSimulation(ctx, off, x, y, b, [{k: v for k, v in p.items() if k != "solver"} for p in params])
+    s.run(2)
+    pos_d, st_d = s.positions(), s.stats()
+    s.close()
+    worst = 0.0
+    for r in range(len(sizes)):
+        sl = slice(off[r], off[r + 1])
+        assert (st_d[r]["successful_steps"], st_d[r]["failed_steps"]) == \
+            (st_h[r]["successful_steps"], st_h[r]["failed_steps"]), r
+        assert st_d[r]["sim_time"] == pytest.approx(st_h[r]["sim_time"], rel=1e-9, abs=1e-300)
+        assert st_d[r]["strain"] == pytest.approx(st_h[r]["strain"], rel=1e-8, abs=1e-12)
+        np.testing.assert_array_equal(pos_d[sl, 1], pos_h[sl, 1])
+        worst = max(worst, float(np.abs(wrap(pos_d[sl, 0] - pos_h[sl, 0])).max()))
+    print(f"[c4] device step vs host-driven ensemble: max |dx| {worst:.3e}", flush=True)
+    assert worst <= 1e-9

@@ -158,6 +158,42 @@ def test_cutoff_pattern_is_reordered_into_a_narrow_envelope(ts, n):
     assert _outside_envelope(_lu_nopivot(Ar), last) == 0
 
 
+@pytest.mark.parametrize("n,rc", [(400, 0.05), (2048, 0.05), (1000, 0.12)])
+def test_key_band_envelope_holds_for_any_x(ts, n, rc):
+    # rows sorted by |y|: the envelope from the keys alone (the device-resident step fixes it once per run)
+    # contains the envelope of the heaviside-cutoff pattern for every x, and the no-interchange LU of such a
+    # matrix stays inside it; and it costs about what reverse Cuthill-McKee's per-pattern envelope does
+    ts.ts_key_band_envelope.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_double] + [C.c_void_p] * 3 + \
+        [C.POINTER(C.c_longlong)]
+    rng = np.random.default_rng(n)
+    y = rng.uniform(-0.5, 0.5, n)
+    ordv = np.argsort(np.abs(y), kind="stable")
+    key = np.ascontiguousarray(np.abs(y)[ordv])
+    bs = 64
+    npad = (n + bs - 1) // bs * bs
+    nb = npad // bs
+    first, last, top = (np.empty(nb, np.int32) for _ in range(3))
+    tiles = C.c_longlong(0)
+    p = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
+    ts.ts_key_band_envelope(n, npad, bs, p(key), rc * (1 + 1e-12), p(first), p(last), p(top), C.byref(tiles))
+    assert np.all(np.diff(last) >= 0) and np.all(last >= np.arange(nb)) and np.all(top <= first)
+    for trial in range(3):
+        x = rng.uniform(-0.5, 0.5, n)
+        dx, dy = wrap(x[:, None] - x[None, :]), wrap(y[:, None] - y[None, :])
+        P = (dx * dx + dy * dy < rc * rc)[np.ix_(ordv, ordv)]
+        i, j = np.nonzero(P)
+        lo, hi = np.minimum(i, j), np.maximum(i, j)
+        assert np.all(last[lo // bs] >= hi // bs), "an entry outside the fixed envelope"
+        if trial == 0 and n <= 1000:
+            vals = np.where(P, rng.uniform(-1, 1, P.shape), 0.0) / P.sum(1).max()
+            Ar = np.eye(npad)
+            Ar[:n, :n] = vals + 2.0 * np.eye(n)
+            assert _outside_envelope(_lu_nopivot(Ar), last) == 0
+    if n >= 2048:
+        _, _, _, _, _, rcm_tiles = _band(ts, P.astype(float))
+        assert tiles.value <= 1.25 * rcm_tiles, (tiles.value, rcm_tiles)
+
+
 def test_narrow_natural_order_is_kept(ts):
     # a band of half-width 40 in its own order: reverse Cuthill-McKee cannot halve the tiles
     n = 700
