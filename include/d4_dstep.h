@@ -9,13 +9,17 @@
  * Same semantics as include/d4_sim.h's loop, with these differences of storage only:
  *  - the step matrix is dense (n x n, zero outside the reference's storage radius, simulation.cpp:461-470)
  *    instead of CSC;
- *  - its LU is the GPU dense LU in the natural order (threshold partial pivoting u = 0.001: blocked LU
- *    without interchanges, refactored with partial pivoting on the device where a multiplier exceeds
- *    1000), so results agree with the CSC path's to rounding, not bit for bit;
+ *  - its LU is the GPU dense LU (threshold partial pivoting u = 0.001: blocked LU without interchanges,
+ *    refactored with partial pivoting on the device where a multiplier exceeds 1000) in an order fixed for
+ *    the run: with a storage radius below half the cell and no speed-threshold cutoff change, rows sorted
+ *    by |y| inside the band envelope that order fixes for every possible pattern (y never changes, and a
+ *    matrix entry needs a pair closer than the radius); otherwise the natural order and the whole matrix.
+ *    Results agree with the CSC path's to rounding, not bit for bit;
  *  - the run's initial speed pass (simulation.cpp:46), whose speeds stage I recomputes before any use,
  *    is not made.
- * d4_sim_params.solver is ignored.  Sized for moderate n (the dense matrix and its LU store hold
- * 2 x 8 n^2 bytes per realization); the largest order is bounded by the device's LDS (n <= ~10000).
+ * d4_sim_params.solver is ignored.  The dense matrix and its LU store hold 2 x 8 n^2 bytes per
+ * realization (25 GB for 256 realizations of up to 4096); the largest order is bounded by the device's LDS
+ * (n <= ~10000).
  */
 #ifndef D4_DSTEP_H
 #define D4_DSTEP_H
