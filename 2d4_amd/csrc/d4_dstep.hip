@@ -143,8 +143,11 @@ __global__ void ds_begin_kernel(DsArgs a)
 // In the realization's LU order (store row/column i = dislocation ord[i]) and, on the first pass, only
 // inside its envelope (blocks top .. last of each column's block: every entry the factorization, the
 // check and the solves read).  only_flagged: just the realizations whose unpivoted factorization was
-// flagged, whole (the pivoted factorization reads every entry).
-__global__ __launch_bounds__(256) void ds_assemble_kernel(DsArgs a, int rescaled, int only_flagged)
+// flagged, whole (the pivoted factorization reads every entry).  M is read only for the pairs inside the
+// storage radius at cfg, the configuration of the pass that wrote it (the same test, d^2 of the lower
+// index minus the higher, as the pass: the pass leaves the rest unwritten); the others are 0 as the pass
+// would have stored them.
+__global__ __launch_bounds__(256) void ds_assemble_kernel(DsArgs a, const double2* cfg, int rescaled, int only_flagged)
 {
 #pragma clang fp contract(off)
     const int r = blockIdx.y, c = blockIdx.x;
@@ -161,13 +164,25 @@ __global__ __launch_bounds__(256) void ds_assemble_kernel(DsArgs a, int rescaled
     double* col = a.store + a.aoff[r] + (size_t)c * np;
     const int oc = (ord && c < n) ? ord[c] : c;
     const double* mc = a.M + a.moff[r] + (size_t)oc * n;
+    const double range_sq = (a.par[r].heaviside ? 1.0 : 49.0) * a.st[r].cut_sq;  // the pass's storage test
+    const double2 pc = cfg[base + (c < n ? oc : 0)];
     for (int row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
         double v;
         if (row >= n || c >= n) {
             v = row == c ? 1.0 : 0.0;
         } else {
             const int orow = ord ? ord[row] : row;
-            v = mc[orow] * ((1 + a.dvec[base + orow]) / 2);
+            double m = 0.0;
+            if (orow == oc) {
+                m = mc[orow];  // s, from the pass's reduction
+            } else {
+                const double2 pr = cfg[base + orow];
+                const bool lower = orow < oc;
+                const double dx = lower ? wrap(pr.x - pc.x) : wrap(pc.x - pr.x);
+                const double dy = lower ? wrap(pr.y - pc.y) : wrap(pc.y - pr.y);
+                if (dist2(dx, dy) < range_sq) m = mc[orow];
+            }
+            v = m * ((1 + a.dvec[base + orow]) / 2);
             if (row == c) v = v + 1;
             if (rescaled) {
                 const double f = a.fac[base + orow];
@@ -393,6 +408,7 @@ struct d4_dstep {
         if (with_m) {
             p.M = const_cast<double*>(a.M);
             p.moff = reinterpret_cast<const int64_t*>(a.moff);
+            p.skip_empty = 1;  // ds_assemble reads no entry outside the storage radius
         }
         return p;
     }
@@ -432,12 +448,13 @@ struct d4_dstep {
         if (e == hipSuccess) e = launch_reduce(q, st);
         return e == hipSuccess ? D4_OK : fail(D4_ERR_HIP, std::string("device step: speed pass: ") + hipGetErrorString(e));
     }
-    int factor(int rescaled, hipStream_t st)
+    // cfg: the configuration of the Jacobian pass whose matrix is assembled
+    int factor(const double2* cfg, int rescaled, hipStream_t st)
     {
         const dim3 g(npad_max, R);
-        hipLaunchKernelGGL(ds_assemble_kernel, g, dim3(256), 0, st, a, rescaled, 0);
+        hipLaunchKernelGGL(ds_assemble_kernel, g, dim3(256), 0, st, a, cfg, rescaled, 0);
         if (lu->factor_unpivoted(a.store, flag_d, st)) return fail(D4_ERR_HIP, "device step: factorization");
-        hipLaunchKernelGGL(ds_assemble_kernel, g, dim3(256), 0, st, a, rescaled, 1);
+        hipLaunchKernelGGL(ds_assemble_kernel, g, dim3(256), 0, st, a, cfg, rescaled, 1);
         if (lu->factor_pivoted(a.store, flag_d, sing_d, st)) return fail(D4_ERR_HIP, "device step: factorization");
         hipLaunchKernelGGL(ds_factored_kernel, dim3((R + 255) / 256), dim3(256), 0, st, a);
         return hipGetLastError() == hipSuccess ? D4_OK : fail(D4_ERR_HIP, "device step: factorization");
@@ -466,20 +483,20 @@ struct d4_dstep {
         hipLaunchKernelGGL(ds_begin_kernel, gr, dim3(256), 0, st, a);
         // stage I
         DS_TRY(jacobian(a.x0, a.rpJ1, a.vA, a.vB, st));
-        DS_TRY(factor(0, st));
+        DS_TRY(factor(a.x0, 0, st));
         DS_TRY(solve(kGStep1, a.x0, a.big, st));
         DS_TRY(speed(a.big, a.rpS1, a.sp2, st));
         DS_TRY(solve(kGStep2, a.big, a.big, st));
         // stage II
         hipLaunchKernelGGL(ds_rescale_kernel, ge, dim3(256), 0, st, a);
-        DS_TRY(factor(1, st));
+        DS_TRY(factor(a.x0, 1, st));
         hipLaunchKernelGGL(ds_take_new_weights_kernel, ge, dim3(256), 0, st, a);
         DS_TRY(solve(kGHalf1, a.x0, a.s1, st));
         DS_TRY(speed(a.s1, a.rpS2, a.sp2, st));
         DS_TRY(solve(kGHalf2, a.s1, a.s1, st));
         // stage III
         DS_TRY(jacobian(a.s1, a.rpJ3, a.vA2, a.vB2, st));
-        DS_TRY(factor(0, st));
+        DS_TRY(factor(a.s1, 0, st));
         DS_TRY(solve(kGStep3a, a.s1, a.s2, st));
         DS_TRY(speed(a.s2, a.rpS3, a.sp2, st));
         DS_TRY(solve(kGStep3b, a.s2, a.s2, st));

@@ -82,6 +82,8 @@ struct PassArgs {
     int tab_n;
     int exact;                // mode 1/2/3: 1 = true derivative instead of the reference's
     int want_dy;              // mode 3: also d/dy
+    int skip_empty;           // mode 4: leave M untouched for the pairs of a 16-step phase with none inside the
+                              // storage radius (for callers that read no entry outside it)
 };
 
 // The pair-list pass's second half: d sigma of each listed pair into its entry.

@@ -117,6 +117,7 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
 
     double q0 = 0.0, q1 = 0.0;
     const bool track_tol = a.tol != nullptr && rp.tol_cap2 > 0.0;
+    unsigned long long inr_phase = 0;  // mode 4: the lanes of this phase's steps with a pair inside the radius
 
     for (int ph = 0; ph < 4; ++ph) {
         if constexpr (kTab) {
@@ -280,9 +281,12 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                 }
             }
             if (kDense) stage[((lane >> 4) * 16 + (lane & 15)) * 17 + ((lane & 15) ^ tt)] = V;
+            if (MODE == kModeJacDenseCut) inr_phase |= __ballot(inr);
         }
         if (kDense) {
             __syncthreads();
+            const bool write = !(MODE == kModeJacDenseCut && a.skip_empty && inr_phase == 0);
+            inr_phase = 0;
             const int g = lane >> 4, q = lane & 15;
             const int ib = td.I0 + 16 * g;
             const int jb = td.J0 + 16 * (g ^ ph);
@@ -291,7 +295,9 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
             // V(ib+k, jb+q) -> column ib+k, row jb+q;  V(ib+q, jb+k) -> column jb+k, row ib+q
             double* p1 = Mb + (int64_t)ib * ld + jb + q;
             double* p2 = Mb + (int64_t)jb * ld + ib + q;
-            if (!diag && td.I0 + kTile <= n && td.J0 + kTile <= n) {
+            if (!write) {
+                // no pair of this phase inside the radius: nothing the caller reads
+            } else if (!diag && td.I0 + kTile <= n && td.J0 + kTile <= n) {
                 // interior tile: every entry valid
 #pragma unroll 4
                 for (int k = 0; k < 16; ++k) {
