@@ -49,6 +49,7 @@
 #include <mutex>
 #include <thread>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../../include/d4_sim.h"
@@ -461,21 +462,52 @@ __global__ __launch_bounds__(64) void lu_diag_kernel(const FacDesc* __restrict__
     for (int c = 0; c < kFB; ++c) blk[(size_t)c * lda + r] = a[c];
 }
 
+// f(std::integral_constant<int, 0>) ... f(std::integral_constant<int, N - 1>), in order: a loop whose
+// index is a constant expression in its body (a DPP lane select must be one)
+template <int... I, class Fn>
+__device__ inline void static_for_seq(std::integer_sequence<int, I...>, Fn&& f)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class Fn>
+__device__ inline void static_for(Fn&& f)
+{
+    static_for_seq(std::make_integer_sequence<int, N>{}, f);
+}
+
+// v of lane 16 g + L, in every lane of the 16-lane row g (DPP row_newbcast)
+template <int L>
+__device__ inline double row_value(double v)
+{
+    return __builtin_amdgcn_mov_dpp(v, 0x150 + L, 0xf, 0xf, true);
+}
+
+// acc = fma(-f', x, acc) with f' the f of lane 16 g + L in the lanes of row g: one v_fmac_f64 whose first
+// operand crosses the row by DPP row_newbcast (no readlane, no SGPR round trip).  A DPP source must not be
+// written by a VALU instruction in the two before it, a hazard the compiler does not see through inline
+// asm: f is always a value loaded from LDS several instructions earlier (the build's assembly is checked
+// for it by tests/test_lu_isa_cpu.py).
+template <int L>
+__device__ inline void fmac_row(double& acc, double f, double x)
+{
+    asm("v_fmac_f64_dpp %0, -%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" : "+v"(acc) : "v"(f), "v"(x), "i"(L));
+}
+
 // One 64 x 64 block per wavefront, four blocks of one matrix per workgroup
 // (each matrix's items padded to a multiple of four), the diagonal block
 // staged in LDS once; after that barrier no wavefront waits on another.  In
 // an L21 block lane lo holds row lo of X in registers (X U11 = B, columns left
-// to right), in a U12 block column lo (L11 X = B, rows top to bottom).  The
-// factor's entries reach every lane by readlane from the lane that holds
-// them (lane c2 keeps column c2 of U11, or row c2 of L11), each read from LDS
-// kAhead steps before its use, so no step waits on LDS.  The order of
-// operations of every entry is that of the substitution written out:
-// x_c2 -= x_c u_{c, c2}, c ascending.
+// to right), in a U12 block column lo (L11 X = B, rows top to bottom).  At
+// step c the factor's row c of U11 (or column c of L11) is read from LDS into
+// four registers, lane 16 g + L holding entry 16 k + L of register k in each
+// of the four rows g, one step ahead; each update is then one v_fmac_f64 that
+// takes its factor entry across the row by DPP.  The order of operations of
+// every entry is that of the substitution written out: x_c2 -= x_c u_{c, c2},
+// c ascending (fma(-u, x, y) and fma(-x, u, y) round alike).
 __global__ __launch_bounds__(256, 1) void lu_panel_kernel(const FacDesc* __restrict__ fd, const int* __restrict__ pre,
                                                           const int* __restrict__ tb, int count,
                                                           double* __restrict__ A, int j0)
 {
-    constexpr int kAhead = 8;
     __shared__ double Dg[kFB * kFLd];
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int g0 = xcd_item() * 4;  // the workgroup's first item; its four items share a matrix
@@ -491,21 +523,30 @@ __global__ __launch_bounds__(256, 1) void lu_panel_kernel(const FacDesc* __restr
     if (item >= 2 * T) return;                     // padding
     const bool rowblk = item < T;                  // L21 block (below) or U12 block (right)
     const int b = (rowblk ? item : item - T) + 1;  // block index past the diagonal
-    double x[kFB], f[kFB];
+    const int L = lo & 15;
+    double x[kFB], F[2][4];  // F[s & 1][k]: step s's factor entries 16 k + L
     if (rowblk) {
         double* blk = A + m.a_off + (size_t)j0 * lda + (j0 + kFB * b);
-        const double rd = 1.0 / Dg[lo * kFLd + lo];  // lane c: 1 / U11(c, c)
+        double R[4];  // lane 16 g + L of R[k]: 1 / U11(16 k + L, 16 k + L)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) R[k] = 1.0 / Dg[(16 * k + L) * kFLd + 16 * k + L];
 #pragma unroll
         for (int c = 0; c < kFB; ++c) x[c] = blk[(size_t)c * lda + lo];
 #pragma unroll
-        for (int k = 0; k < kAhead; ++k) f[k] = Dg[lo * kFLd + k];  // f[r] = U11(r, lo)
+        for (int k = 0; k < 4; ++k) F[0][k] = Dg[(16 * k + L) * kFLd];  // U11(0, 16 k + L)
+        static_for<kFB>([&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            if constexpr (c + 1 < kFB) {
 #pragma unroll
-        for (int c = 0; c < kFB; ++c) {
-            if (c + kAhead < kFB) f[c + kAhead] = Dg[lo * kFLd + c + kAhead];
-            x[c] *= lane_value(rd, c);
-#pragma unroll
-            for (int c2 = c + 1; c2 < kFB; ++c2) x[c2] = fma(-x[c], lane_value(f[c], c2), x[c2]);
-        }
+                for (int k = (c + 2) >> 4; k < 4; ++k) F[(c + 1) & 1][k] = Dg[(16 * k + L) * kFLd + c + 1];
+            }
+            __builtin_amdgcn_sched_barrier(0);  // the next step's loads stay ahead of this step's updates
+            x[c] *= row_value<c & 15>(R[c >> 4]);
+            static_for<kFB>([&](auto cc2) {
+                constexpr int c2 = decltype(cc2)::value;
+                if constexpr (c2 > c) fmac_row<c2 & 15>(x[c2], F[c & 1][c2 >> 4], x[c]);  // -= U11(c, c2) x_c
+            });
+        });
 #pragma unroll
         for (int c = 0; c < kFB; ++c) blk[(size_t)c * lda + lo] = x[c];
     } else {
@@ -513,13 +554,19 @@ __global__ __launch_bounds__(256, 1) void lu_panel_kernel(const FacDesc* __restr
 #pragma unroll
         for (int r = 0; r < kFB; ++r) x[r] = col[r];
 #pragma unroll
-        for (int k = 0; k < kAhead; ++k) f[k] = Dg[k * kFLd + lo];  // f[k] = L11(lo, k)
+        for (int k = 0; k < 4; ++k) F[0][k] = Dg[16 * k + L];  // L11(16 k + L, 0)
+        static_for<kFB - 1>([&](auto rr) {
+            constexpr int r = decltype(rr)::value;
+            if constexpr (r + 1 < kFB - 1) {
 #pragma unroll
-        for (int r = 0; r < kFB - 1; ++r) {
-            if (r + kAhead < kFB) f[r + kAhead] = Dg[(r + kAhead) * kFLd + lo];
-#pragma unroll
-            for (int r2 = r + 1; r2 < kFB; ++r2) x[r2] = fma(-lane_value(f[r], r2), x[r], x[r2]);
-        }
+                for (int k = (r + 2) >> 4; k < 4; ++k) F[(r + 1) & 1][k] = Dg[(r + 1) * kFLd + 16 * k + L];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            static_for<kFB>([&](auto rr2) {
+                constexpr int r2 = decltype(rr2)::value;
+                if constexpr (r2 > r) fmac_row<r2 & 15>(x[r2], F[r & 1][r2 >> 4], x[r]);  // -= L11(r2, r) x_r
+            });
+        });
 #pragma unroll
         for (int r = 0; r < kFB; ++r) col[r] = x[r];
     }
