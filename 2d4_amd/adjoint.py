@@ -332,11 +332,12 @@ def accumulated_strain(x0, y, tau, T, dt, d4=None, checkpoint_every=16, tol=1e-1
     return (_signs(x0.numel(), x0.device) * (x - x0)).sum()
 
 
-def reference_schedule(x, y, tau, t_end, d4=None, opts=None, max_steps=1000000, progress=None, **sim_params):
+def reference_schedule(x, y, tau, t_end, d4=None, opts=None, max_steps=1000000, progress=None, max_accepted=None,
+                       **sim_params):
     """The step sizes the reference's loop accepts from sorted positions (x, y) at constant stress tau until
-    its time passes t_end: Simulation::run with its step doubling, error estimate and step-size control
-    (simulation.cpp:88-276, precision_handler.cpp:34-63), run by the library's host Simulation (d4_sim, its
-    passes on the GPU).  sim_params as 2d4_amd.sim (init_step_size, max_step_size, prec, dipole_prec, solver).
+    its time passes t_end, or until it has accepted max_accepted steps: Simulation::run with its step
+    doubling, error estimate and step-size control (simulation.cpp:88-276, precision_handler.cpp:34-63), run
+    by the library's host Simulation (d4_sim, its passes on the GPU).  sim_params as 2d4_amd.sim (init_step_size, max_step_size, prec, dipole_prec, solver).
     Returns (sizes, final sorted positions of that run); accumulated_strain(..., steps=sizes) replays the
     accepted steps as a differentiable map.  progress(iterations, accepted, sim_time), if given, is called
     after every iteration."""
@@ -373,6 +374,8 @@ def reference_schedule(x, y, tau, t_end, d4=None, opts=None, max_steps=1000000, 
                     raise RuntimeError(f"more than {max_steps} accepted steps before t = {t_end}")
             if progress is not None:
                 progress(iterations, len(sizes), st["sim_time"] + (h if accepted else 0.0))
+            if max_accepted is not None and len(sizes) >= max_accepted:
+                break
         return sizes, sim.positions()
     finally:
         sim.close()

@@ -381,6 +381,80 @@ def adjoint_bench(n, T, dt, every, method="theta"):
             "config": "uniform positions with a minimum separation of 0.5/sqrt(N), sorted"}
 
 
+def c5_adjoint_bench(n=2048, T=256, ncoord=32, seed=2048, h=1e-6):
+    """C5 at N = n uniform positions (x then y from numpy's generator with this seed, b alternating, sorted),
+    tau = 0.2: the strain gradient through T steps of the reference's implicit step with the sizes its own loop
+    accepts from this start (adjoint.reference_schedule: default precision, steps at most 1e-3, GPU dense LU),
+    checkpoints every 16 steps, against central differences with step h on ncoord random coordinates and on
+    tau.  Fixed steps of 1e-3 are not differentiable from this start (--c5-check)."""
+    import torch
+    adj = importlib.import_module("2d4_amd.adjoint")
+    pkg = importlib.import_module("2d4_amd")
+    ctx = pkg.Context(0)
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-0.5, 0.5, n)
+    y = rng.uniform(-0.5, 0.5, n)
+    b = np.where(np.arange(n) % 2 == 1, 1.0, -1.0)
+    o = np.argsort(-(y + b), kind="stable")
+    x, y = x[o].copy(), y[o].copy()
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda:0")  # noqa: E731
+    t0 = time.perf_counter()
+    sizes, _ = adj.reference_schedule(x, y, 0.2, 1.0, ctx, max_step_size=1e-3, solver=1, max_accepted=T)
+    t_sched = time.perf_counter() - t0
+    log(f"[bench] C5 schedule: {len(sizes)} steps, {min(sizes):.3e} ... {max(sizes):.3e}, {t_sched:.1f} s")
+
+    def loss(xa, ya, tau=0.2):
+        with torch.no_grad():
+            return adj.accumulated_strain(dev(xa), dev(ya), tau, 0, 0.0, ctx, checkpoint_every=0, steps=sizes).item()
+
+    t0 = time.perf_counter()
+    L0 = loss(x, y)
+    t_fwd = time.perf_counter() - t0
+    x0, yy = dev(x).requires_grad_(), dev(y).requires_grad_()
+    tt = torch.tensor(0.2, dtype=torch.float64, device="cuda:0", requires_grad=True)
+    t_ck0 = time.perf_counter()
+    L = adj.accumulated_strain(x0, yy, tt, 0, 0.0, ctx, checkpoint_every=16, steps=sizes)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    L.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    gx, gy, gt = x0.grad.cpu().numpy(), yy.grad.cpu().numpy(), tt.grad.item()
+    gmax = max(np.abs(gx).max(), np.abs(gy).max())
+    coords = np.random.default_rng(seed + 1).choice(2 * n, ncoord, replace=False)
+    rows, worst = [], 0.0
+    t0 = time.perf_counter()
+    for kk in coords:
+        col, k = divmod(int(kk), n)
+        arr = x if col == 0 else y
+        p_, m_ = arr.copy(), arr.copy()
+        p_[k] += h
+        m_[k] -= h
+        fd = ((loss(p_, y) - loss(m_, y)) if col == 0 else (loss(x, p_) - loss(x, m_))) / (2 * h)
+        g = float((gx if col == 0 else gy)[k])
+        rel = abs(fd - g) / max(abs(g), 1e-3 * gmax)
+        worst = max(worst, rel)
+        rows.append({"coord": f"{'xy'[col]}[{k}]", "adjoint": g, "fd": fd, "rel": rel})
+        if rel > 1e-5:  # the difference at h/2 and the extrapolation (4 d(h/2) - d(h)) / 3
+            p_[k] -= h / 2
+            m_[k] += h / 2
+            fd2 = ((loss(p_, y) - loss(m_, y)) if col == 0 else (loss(x, p_) - loss(x, m_))) / h
+            rows[-1].update(fd_half_h=fd2, rel_half_h=abs(fd2 - g) / max(abs(g), 1e-3 * gmax),
+                            rel_extrapolated=abs((4 * fd2 - fd) / 3 - g) / max(abs(g), 1e-3 * gmax))
+        log(f"[bench] C5 {rows[-1]}")
+    fdt = (loss(x, y, 0.2 + h) - loss(x, y, 0.2 - h)) / (2 * h)
+    t_fd = time.perf_counter() - t0
+    return {"n": n, "T": len(sizes), "tau": 0.2, "checkpoint_every": 16, "h": h, "coordinates": ncoord,
+            "config": f"uniform positions (numpy default_rng({seed}), x then y), sorted; step sizes of the "
+                      "reference's loop (default precision, at most 1e-3)",
+            "min_step": min(sizes), "max_step": max(sizes), "sim_time": float(sum(sizes)),
+            "schedule_s": t_sched, "forward_s": t_fwd, "ms_per_step_forward": t_fwd * 1e3 / len(sizes),
+            "forward_checkpointed_s": t1 - t_ck0, "backward_s": t2 - t1,
+            "ms_per_step_backward": (t2 - t1) * 1e3 / len(sizes), "fd_s": t_fd, "L": L0,
+            "max_abs_grad": gmax, "dL_dtau": gt, "dL_dtau_fd": fdt, "tau_rel": abs(fdt - gt) / abs(gt),
+            "worst_rel_vs_fd": worst, "checks": rows}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -418,6 +492,9 @@ def main():
                     help="only the adjoint measurement: N,T,dt[,method] (e.g. 2048,256,1e-3; method theta: T steps of "
                          "dt of the reference's step, adaptive: the steps the reference's loop accepts until T dt, "
                          "backward_euler), checkpoints every 16 steps")
+    ap.add_argument("--c5-adjoint", type=str, default="",
+                    help="only C5's strain gradient: N,T,ncoord (e.g. 2048,256,32) through T steps of the reference's "
+                         "own accepted sizes from uniform positions, against central differences on ncoord coordinates")
     ap.add_argument("--c5-check", type=int, default=0,
                     help="only the C5 configuration through the reference's own step for this many steps")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -437,6 +514,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calib_copy_bytes": nel * 8}), flush=True)
+        return
+    if args.c5_adjoint:
+        n_, T_, k_ = (int(v) for v in args.c5_adjoint.split(","))
+        print(json.dumps({"c5_adjoint": c5_adjoint_bench(n_, T_, k_)}), flush=True)
         return
     if args.c5_check:
         print(json.dumps({"c5_reference_step": c5_reference_step_check(args.c5_check)}), flush=True)

@@ -532,6 +532,8 @@ static int lu_factor(int n, double* a, int* piv)
         double d = a[(size_t)k * n + k];
         if (d == 0) { if (!sing) sing = k + 1; continue; }
         for (int r = k + 1; r < n; ++r) a[(size_t)k * n + r] /= d;
+        /* columns are independent: threads change which core updates a column, not its operations */
+#pragma omp parallel for schedule(static) if (n - k > 256)
         for (int c = k + 1; c < n; ++c) {
             double m = a[(size_t)c * n + k];
             if (m == 0) continue;

@@ -39,6 +39,15 @@ def random_config(n, seed):
     return sort_reference(xy)
 
 
+def c5_config(n=2048, seed=2048):
+    """C5's start: x, then y, uniform in the unit cell from numpy's generator with this seed, b alternating
+    (-1 first), in SimulationData's sorted order."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-0.5, 0.5, n)
+    y = rng.uniform(-0.5, 0.5, n)
+    return sort_reference(np.stack([x, y], axis=1))
+
+
 def _walls(dx, E, c):
     """(u, C, S_quirk, weight) of the ten walls (Field.cpp:295-331 layout)."""
     R = 1.0 / E

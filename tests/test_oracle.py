@@ -11,7 +11,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import (TOL, all_pairs, bsign, random_config, scale_dsigma, scale_sigma, wrap)
+from helpers import (TOL, all_pairs, bsign, c5_config, random_config, scale_dsigma, scale_sigma, wrap)
 
 GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -454,3 +454,27 @@ def test_run_invariants(orc, golden):
     xy, st = orc.run(xy0, dt0=1e-6, max_dt=1e-6, prec=1e10, dipole_prec=0.0, step_limit=4)
     disp = wrap(xy[:, 0] - xy0[:, 0])
     assert np.abs(disp).max() < 0.4 and abs(st.strain - (bsign(64) * disp).sum()) < 1e-13
+
+
+
+def test_c5_fixed_steps_from_uniform_positions_are_not_differentiable(orc):
+    """C5 as stated (N = 2048 uniform, tau = 0.2, fixed dt = 1e-3: -P 1e10 -p 0 -M 1e-3 --initial-stepsize 1e-3)
+    through the oracle's run loop (simulation.cpp:88-276): the very first step moves dislocations by up to
+    half the cell and accumulates a strain of thousands of cell lengths, and a 1e-9 change of one coordinate
+    changes the end positions by order one.  A finite-difference check of a strain gradient through 256 such
+    steps has nothing to compare with; the adjoint's C5 test takes the reference's own accepted step sizes."""
+    xy0 = c5_config()
+    kw = dict(dt0=1e-3, max_dt=1e-3, prec=1e10, dipole_prec=0.0, step_limit=0, stress_kind=1, stress_init=0.2,
+              stress_rate=0.0)
+    a, st = orc.run(xy0, **kw)
+    xp = xy0.copy()
+    xp[7, 0] += 1e-9
+    b, stp = orc.run(xp, **kw)
+    step = np.abs(wrap(a[:, 0] - xy0[:, 0])).max()
+    dev = np.abs(wrap(b[:, 0] - a[:, 0])).max()
+    slope = (stp.strain - st.strain) / 1e-9
+    print(f"one step of 1e-3: strain {st.strain:.4e}, largest move {step:.4f}, "
+          f"x_7 + 1e-9 moves the end positions by {dev:.4f}, strain slope {slope:.3e}")
+    assert st.ok_steps == 1
+    assert step > 0.4 and abs(st.strain) > 1e3
+    assert dev > 0.1 and abs(slope) > 1e10
