@@ -226,7 +226,7 @@ D4_HD void pair_field(double dx, double dy, double E, double R, double c, double
     const Hyp h = hyperbolics(E, R);
     const bool pos = dx >= 0.0;
     const double sg = pos ? 1.0 : -1.0;
-    const double ad = pos ? dx : -dx;
+    const double ad = fabs(dx);
     const bool series = !(dist2(dx, dy) > 1e-6);
 
     // the five reciprocal walls, in batch order: +2, -2, +1, -1, 0
@@ -271,61 +271,90 @@ D4_HD void pair_field(double dx, double dy, double E, double R, double c, double
         else wall0_series<false>(dx, dy, sig, dsig);
     }
 
-    // the walls at +-3: rho = R / cosh(6 pi) at +3, E / cosh(6 pi) at -3; q2 = 2c^2 - 1
     constexpr double kIch3 = 1.0 / kCh3, kIch4 = 1.0 / kCh4, kIch5 = 1.0 / kCh5;
-    const double q2 = fma(2.0 * c, c, -1.0);
-    const double rp3 = kIch3 * R, rm3 = kIch3 * E;
-    const double up3 = dx + 3.0, um3 = dx - 3.0;
-    const double ap3 = fma(rp3, q2, c), am3 = fma(rm3, q2, c);
-    sig = fma(up3 * rp3, ap3, sig);
-    sig = fma(um3 * rm3, am3, sig);
-
-    // the walls at -4s (weight 1), +4s (1 - |dx|) and -5s (|dx|): rho c from E c or R c
+    // E f, R f: the factor of rho at the walls -4s, -5s (and -3 for dx >= 0) and at +4s (and +3)
     const double Ef = pos ? E : R, Rf = pos ? R : E;
-    const double ec = Ef * c, rc = Rf * c;
-    const double u2 = dx - 4.0 * sg, u3 = dx + 4.0 * sg, u4 = dx - 5.0 * sg;
-    const double rc2 = kIch4 * ec, rc3 = kIch4 * rc, rc4 = kIch5 * ec;
-    const double f3 = u3 * rc3, f4 = u4 * rc4;
-    sig = fma(u2, rc2, sig);
-    sig = fma(1.0 - ad, f3, sig);
-    sig = fma(ad, f4, sig);
-    sig_out = kTwoPiSq * sig;
-    if (dd) {
-        // s = +1 (reference); EXACT: the sign of the shift
-        const double sp3 = 1.0, sm3 = EXACT ? -1.0 : 1.0;
-        const double s2 = EXACT ? -sg : 1.0, s3 = EXACT ? sg : 1.0, s4 = EXACT ? -sg : 1.0;
-        const double bp3 = fma(c, fma(-kTwoPi * sp3, up3, 1.0), rp3 * q2 * fma(-2.0 * kTwoPi * sp3, up3, 1.0));
-        const double bm3 = fma(c, fma(-kTwoPi * sm3, um3, 1.0), rm3 * q2 * fma(-2.0 * kTwoPi * sm3, um3, 1.0));
-        dsig = fma(rp3, bp3, dsig);
-        dsig = fma(rm3, bm3, dsig);
-        dsig = fma(rc2, fma(-kTwoPi * s2, u2, 1.0), dsig);
-        dsig = fma((1.0 - ad) * rc3, fma(-kTwoPi * s3, u3, 1.0), dsig);
-        dsig = fma(ad * rc4, fma(-kTwoPi * s4, u4, 1.0), dsig);
-        dsig_out = kTwoPiSq * fma(sg, f4 - f3, dsig);
+    if constexpr (EXACT) {
+        // the walls one by one (the true sinh signs of the derivative)
+        // the walls at +-3: rho = R / cosh(6 pi) at +3, E / cosh(6 pi) at -3; q2 = 2c^2 - 1
+        const double q2 = fma(2.0 * c, c, -1.0);
+        const double rp3 = kIch3 * R, rm3 = kIch3 * E;
+        const double up3 = dx + 3.0, um3 = dx - 3.0;
+        const double ap3 = fma(rp3, q2, c), am3 = fma(rm3, q2, c);
+        sig = fma(up3 * rp3, ap3, sig);
+        sig = fma(um3 * rm3, am3, sig);
+        // the walls at -4s (weight 1), +4s (1 - |dx|) and -5s (|dx|): rho c from E c or R c
+        const double ec = Ef * c, rc = Rf * c;
+        const double u2 = dx - 4.0 * sg, u3 = dx + 4.0 * sg, u4 = dx - 5.0 * sg;
+        const double rc2 = kIch4 * ec, rc3 = kIch4 * rc, rc4 = kIch5 * ec;
+        const double f3 = u3 * rc3, f4 = u4 * rc4;
+        sig = fma(u2, rc2, sig);
+        sig = fma(1.0 - ad, f3, sig);
+        sig = fma(ad, f4, sig);
+        sig_out = kTwoPiSq * sig;
+        if (dd) {
+            const double bp3 = fma(c, fma(-kTwoPi, up3, 1.0), rp3 * q2 * fma(-2.0 * kTwoPi, up3, 1.0));
+            const double bm3 = fma(c, fma(kTwoPi, um3, 1.0), rm3 * q2 * fma(2.0 * kTwoPi, um3, 1.0));
+            dsig = fma(rp3, bp3, dsig);
+            dsig = fma(rm3, bm3, dsig);
+            dsig = fma(rc2, fma(kTwoPi * sg, u2, 1.0), dsig);
+            dsig = fma((1.0 - ad) * rc3, fma(-kTwoPi * sg, u3, 1.0), dsig);
+            dsig = fma(ad * rc4, fma(kTwoPi * sg, u4, 1.0), dsig);
+            dsig_out = kTwoPiSq * fma(sg, f4 - f3, dsig);
+        }
+    } else {
+        // The five far walls gathered by their factor of rho (first order) and rho^2 (the +-3 walls'
+        // second order); with a = |dx| and s the sign:
+        //   sigma part  s [c (Ef A1(a) + Rf A2(a)) + q2/ch3^2 (Ef^2 (a - 3) + Rf^2 (a + 3))],
+        //   A1(a) = (a - 3)/ch3 + (a - 4)/ch4 + a (a - 5)/ch5     (-3 for s = +1, -4s weight 1, -5s weight a),
+        //   A2(a) = (a + 3)/ch3 + (1 - a)(a + 4)/ch4              (+3 for s = +1, +4s weight 1 - a);
+        //   d/dx part (the reference's cosh for sinh, and the derivatives of the weights 1 - a and a)
+        //   c (Ef (P1(a) - 2 pi s A1(a)) + Rf (P2(a) - 2 pi s A2(a)))
+        //     + q2/ch3^2 (Ef^2 (1 - 4 pi s (a - 3)) + Rf^2 (1 - 4 pi s (a + 3))),
+        //   P1(a) = 1/ch3 + 1/ch4 - 5/ch5 + 2a/ch5,  P2(a) = 1/ch3 - 3/ch4 - 2a/ch4.
+        // These terms are below 1e-6 of the near walls', so their rounding stays far under the bound.
+        constexpr double a12 = kIch5, a11 = kIch3 + kIch4 - 5.0 * kIch5, a10 = -(3.0 * kIch3 + 4.0 * kIch4);
+        constexpr double a22 = -kIch4, a21 = kIch3 - 3.0 * kIch4, a20 = 3.0 * kIch3 + 4.0 * kIch4;
+        constexpr double k3sq = kIch3 * kIch3;
+        // (fma3 keeps a constant addend in a scalar register instead of copying it into the result first)
+        const double A1 = fma3(fma3(ad, a12, a11), ad, a10);
+        const double A2 = fma3(fma3(ad, a22, a21), ad, a20);
+        const double Ef2 = Ef * Ef, Rf2 = Rf * Rf;
+        const double am3 = ad - 3.0, ap3 = ad + 3.0;
+        const double q2k = fma3((2.0 * k3sq) * c, c, -k3sq);  // (2 c^2 - 1) / ch3^2
+        const double first = fma(Rf, A2, Ef * A1);
+        const double second = fma(Rf2, ap3, Ef2 * am3);
+        sig = fma(sg, fma(q2k, second, c * first), sig);
+        sig_out = kTwoPiSq * sig;
+        if (dd) {
+            const double s2pi = -kTwoPi * sg, s4pi = 2.0 * s2pi;
+            const double P1 = fma3(ad, 2.0 * kIch5, kIch3 + kIch4 - 5.0 * kIch5);
+            const double P2 = fma3(ad, -2.0 * kIch4, kIch3 - 3.0 * kIch4);
+            const double first_d = fma(Rf, fma(s2pi, A2, P2), Ef * fma(s2pi, A1, P1));
+            const double second_d = fma(Rf2, fma(s4pi, ap3, 1.0), Ef2 * fma(s4pi, am3, 1.0));
+            dsig_out = kTwoPiSq * (dsig + fma(q2k, second_d, c * first_d));
+        }
     }
 }
 
 // The reference's transcendental inputs of a pair (simulation.cpp:456-457,
 // Field.cpp:335-336): E = exp2(PI2LN2INV * dx), c = cos(2 pi dy), here with
 // R = 1/E as well.  Own implementations, each within ~1 ulp:
-//   exp2_pair: y = k + f, |f| <= 1/2, 2^f from the Taylor series of e^{f ln 2}
-//     through f^13 (truncation < 5e-18), scaled by 2^k; R = 1/E by rcp();
-//   cos2pi_reduced: half-angle form with the Taylor series of sin through
-//     r^17 on |r| <= pi/4 (truncation < 6e-17 relative).
+//   exp2_pair: y = k + f, |f| <= 1/2, 2^f = 1 + f g(f) with g a degree-10
+//     polynomial near the minimax one for (2^f - 1)/f on [-1/2, 1/2] (Chebyshev
+//     fit, coefficients rounded to double: 2.0e-17 relative), scaled by 2^k;
+//     R = 1/E by rcp();
+//   cos2pi_reduced: half-angle form, sin r = r + r^3 h(r^2) on |r| <= pi/4 with h
+//     of degree 5 fitted the same way (2.1e-17 relative).
 namespace tc {
-// ln2^n / n!
-constexpr double e1 = 0x1.62e42fefa39efp-1, e2 = 0x1.ebfbdff82c58fp-3, e3 = 0x1.c6b08d704a0c0p-5,
-                 e4 = 0x1.3b2ab6fba4e77p-7, e5 = 0x1.5d87fe78a6731p-10, e6 = 0x1.430912f86c787p-13,
-                 e7 = 0x1.ffcbfc588b0c7p-17, e8 = 0x1.62c0223a5c824p-20, e9 = 0x1.b5253d395e7c4p-24,
-                 e10 = 0x1.e4cf5158b8ecap-28, e11 = 0x1.e8cac7351bb25p-32, e12 = 0x1.c3bd650fc2986p-36,
-                 e13 = 0x1.816193166d0f9p-40;
-// (-1)^k / n!
-constexpr double s3 = -0x1.5555555555555p-3, s5 = 0x1.1111111111111p-7, s7 = -0x1.a01a01a01a01ap-13,
-                 s9 = 0x1.71de3a556c734p-19, s11 = -0x1.ae64567f544e4p-26, s13 = 0x1.6124613a86d09p-33,
-                 s15 = -0x1.ae7f3e733b81fp-41, s17 = 0x1.952c77030ad4ap-49;
-constexpr double c2 = -0x1.0000000000000p-1, c4 = 0x1.5555555555555p-5, c6 = -0x1.6c16c16c16c17p-10,
-                 c8 = 0x1.a01a01a01a01ap-16, c10 = -0x1.27e4fb7789f5cp-22, c12 = 0x1.1eed8eff8d898p-29,
-                 c14 = -0x1.93974a8c07c9dp-37, c16 = 0x1.ae7f3e733b81fp-45;
+// g(f) = (2^f - 1)/f, highest degree first
+constexpr double g10 = 0x1.e9d3fe3952179p-32, g9 = 0x1.e6063f7217bc6p-28, g8 = 0x1.b524fae627834p-24,
+                 g7 = 0x1.62bfd47773353p-20, g6 = 0x1.ffcbfc670dcd4p-17, g5 = 0x1.430913096fd9fp-13,
+                 g4 = 0x1.5d87fe78a5276p-10, g3 = 0x1.3b2ab6fba1ddap-7, g2 = 0x1.c6b08d704a0c2p-5,
+                 g1 = 0x1.ebfbdff82c598p-3, g0 = 0x1.62e42fefa39efp-1;
+// h(u) = (sin sqrt(u) - sqrt(u)) / u^(3/2), highest degree first
+constexpr double h5 = 0x1.5e0b19f8b1451p-33, h4 = -0x1.ae600b02b6262p-26, h3 = 0x1.71de37968a100p-19,
+                 h2 = -0x1.a01a019e83aaep-13, h1 = 0x1.1111111110bb2p-7, h0 = -0x1.5555555555555p-3;
 constexpr double pio2_1 = 0x1.921fb54442d18p+0, pio2_2 = 0x1.1a62633145c07p-54, two_over_pi = 0x1.45f306dc9c883p-1;
 }  // namespace tc
 
@@ -333,19 +362,17 @@ D4_HD void exp2_pair(double y, double& E, double& R)
 {
     const double k = rint(y);
     const double f = y - k;
-    double p = fma3(f, tc::e13, tc::e12);
-    p = fma3(f, p, tc::e11);
-    p = fma3(f, p, tc::e10);
-    p = fma3(f, p, tc::e9);
-    p = fma3(f, p, tc::e8);
-    p = fma3(f, p, tc::e7);
-    p = fma3(f, p, tc::e6);
-    p = fma3(f, p, tc::e5);
-    p = fma3(f, p, tc::e4);
-    p = fma3(f, p, tc::e3);
-    p = fma3(f, p, tc::e2);
-    p = fma3(f, p, tc::e1);
-    p = fma3(f, p, 1.0);
+    double q = fma3(f, tc::g10, tc::g9);
+    q = fma3(f, q, tc::g8);
+    q = fma3(f, q, tc::g7);
+    q = fma3(f, q, tc::g6);
+    q = fma3(f, q, tc::g5);
+    q = fma3(f, q, tc::g4);
+    q = fma3(f, q, tc::g3);
+    q = fma3(f, q, tc::g2);
+    q = fma3(f, q, tc::g1);
+    q = fma3(f, q, tc::g0);
+    const double p = fma3(f, q, 1.0);
     E = ldexp(p, (int)k);
     // 1/E correctly rounded, as the reference's identities take it (constants.h:63-79):
     // the last Newton step of rcp() runs from a faithful estimate, which rounds correctly
@@ -363,13 +390,11 @@ D4_HD double cos2pi_reduced(double dy)
     const bool hi = at > 0.5 * tc::pio2_1;
     const double rho = hi ? (tc::pio2_1 - at) + tc::pio2_2 : th;
     const double r2 = rho * rho;
-    double sp = fma3(r2, tc::s17, tc::s15);
-    sp = fma3(r2, sp, tc::s13);
-    sp = fma3(r2, sp, tc::s11);
-    sp = fma3(r2, sp, tc::s9);
-    sp = fma3(r2, sp, tc::s7);
-    sp = fma3(r2, sp, tc::s5);
-    sp = fma3(r2, sp, tc::s3);
+    double sp = fma3(r2, tc::h5, tc::h4);
+    sp = fma3(r2, sp, tc::h3);
+    sp = fma3(r2, sp, tc::h2);
+    sp = fma3(r2, sp, tc::h1);
+    sp = fma3(r2, sp, tc::h0);
     const double sn = fma(rho * r2, sp, rho);
     const double v = fma(-2.0 * sn, sn, 1.0);
     return hi ? -v : v;
@@ -397,7 +422,7 @@ D4_HD double pair_field_dy(double dx, double E, double R, double c, double s2piy
     const Hyp h = hyperbolics(E, R);
     const bool pos = dx >= 0.0;
     const double sg = pos ? 1.0 : -1.0;
-    const double ad = pos ? dx : -dx;
+    const double ad = fabs(dx);
     const double Cp4 = kCh4 * E, Cm4 = kCh4 * R, Cp5 = kCh5 * E, Cm5 = kCh5 * R;
     double acc = 0.0;
     acc = wall_dy(dx - 4.0 * sg, pos ? Cm4 : Cp4, c, 1.0, acc);

@@ -184,8 +184,7 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                         if (MODE == kModeJacDenseTab) {
                             const double d4v[4] = {ch[2 * NP - 2].x, ch[2 * NP - 2].y, ch[2 * NP - 1].x,
                                                    ch[2 * NP - 1].y};
-                            const double bj = j < half ? 1.0 : -1.0;
-                            V = rp.h * bj * bi * bilinear_cell(d4v, axk[k], ayk[k]);
+                            V = flip_if(rp.h * bilinear_cell(d4v, axk[k], ayk[k]), ipos != (j < half));
                             q1 += V;
                             atomicAdd(&jq1[jl], V);
                         }
@@ -225,7 +224,7 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                 } else if (MODE == kModeJacDense) {
                     pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
                     const double f = flip_if(sig, neg);
-                    V = rp.h * bj * bi * dsig;
+                    V = flip_if(rp.h * dsig, neg);  // h b_j b_i d sigma: a product by +-1 is exact
                     q0 += f;
                     atomicAdd(&jq0[jl], f);
                     q1 += V;
@@ -236,7 +235,7 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                     inr = d2 < rp.range_sq;
                     pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig, inr);
                     if (inr) {
-                        V = rp.h * bj * bi * dsig;
+                        V = flip_if(rp.h * dsig, neg);
                         if (!rp.heaviside && d2 > rp.cut_sq) {  // simulation.cpp:469-470
                             const double e = sqrt(d2) - rp.cutoff;
                             V *= exp(-(e * e) * rp.one_per);
@@ -938,7 +937,10 @@ hipError_t launch_rescale(const RescaleArgs& a, hipStream_t s)
 // ------------------------------------------------------ dense final scale
 
 // Each thread owns two consecutive rows (one 16-byte access per column when VEC: n and ld even, M
-// 16-byte aligned) and keeps their factors (1 + d)/2 for the kScaleCols columns of its block.
+// 16-byte aligned) and keeps their factors (1 + d)/2 for the kScaleCols columns of its block.  The
+// interior blocks issue all their columns' loads before the first store (the compiler cannot move a
+// load past a store to the same array by itself), and the matrix streams past the caches both ways:
+// it is read and written once.
 // Unfused, as simulation.cpp:516-522 rounds it: A = M (1 + d_row)/2, then + 1 on the diagonal.
 constexpr int kScaleCols = 8;
 
@@ -951,6 +953,25 @@ __global__ __launch_bounds__(256) void dense_scale_kernel(double* M, int64_t ld,
     const bool two = row + 1 < n;
     const double f0 = (1 + dvec[row]) / 2, f1 = two ? (1 + dvec[row + 1]) / 2 : 0.0;
     const int c0 = blockIdx.y * kScaleCols, c1 = min(c0 + kScaleCols, n);
+    if (VEC && c1 - c0 == kScaleCols) {
+        double2* p = reinterpret_cast<double2*>(M + (int64_t)c0 * ld + row);
+        const int64_t st = ld / 2;
+        double2 v[kScaleCols];
+#pragma unroll
+        for (int k = 0; k < kScaleCols; ++k) {
+            v[k].x = __builtin_nontemporal_load(&p[k * st].x);
+            v[k].y = __builtin_nontemporal_load(&p[k * st].y);
+        }
+#pragma unroll
+        for (int k = 0; k < kScaleCols; ++k) {
+            double a = v[k].x * f0, b = v[k].y * f1;
+            if (row == c0 + k) a = a + 1;
+            if (row + 1 == c0 + k) b = b + 1;
+            __builtin_nontemporal_store(a, &p[k * st].x);
+            __builtin_nontemporal_store(b, &p[k * st].y);
+        }
+        return;
+    }
     for (int c = c0; c < c1; ++c) {
         double* p = M + (int64_t)c * ld + row;
         if (VEC) {
