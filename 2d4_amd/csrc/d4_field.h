@@ -216,13 +216,10 @@ D4_HD void batch_rcp5(const double* x, double* r)
 // cosh-as-sinh).  At +-3 two terms leave ~1e-19 out; at +-4 and +-5
 // (rho <= 5.6e-10) the first term alone leaves ~1e-18 out -- both four orders
 // under the rounding of the near walls, and no reciprocal is needed.
-// With NEED_D, want_d = false skips the derivative (a per-lane choice: where
-// lanes of a wavefront differ, the shared sigma work is done once).
 template <bool NEED_D, bool EXACT>
-D4_HD void pair_field(double dx, double dy, double E, double R, double c, double& sig_out, double& dsig_out,
-                      bool want_d = true)
+D4_HD void pair_field(double dx, double dy, double E, double R, double c, double& sig_out, double& dsig_out)
 {
-    const bool dd = NEED_D && want_d;
+    constexpr bool dd = NEED_D;
     const Hyp h = hyperbolics(E, R);
     const bool pos = dx >= 0.0;
     const double sg = pos ? 1.0 : -1.0;

@@ -120,6 +120,7 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
     unsigned long long inr_phase = 0;  // mode 4: the lanes of this phase's steps with a pair inside the radius
 
     for (int ph = 0; ph < 4; ++ph) {
+        unsigned pend = 0u;  // mode 4: bit tt set when this lane's pair of step tt is inside the radius
         if constexpr (kTab) {
             // tabulated field, four steps at a time with every record load issued before any is used.
             // A record is S doubles, read as S / 4 pairs of 16-byte chunks; lanes 2k and 2k + 1 read
@@ -230,22 +231,15 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                     q1 += V;
                     atomicAdd(&jq1[jl], V);
                 } else if (MODE == kModeJacDenseCut) {
-                    // sigma for every pair, d sigma only inside the storage radius:
-                    // one evaluation whose derivative part runs for the lanes that need it
+                    // sigma for every pair here; d sigma of the pairs inside the storage radius after
+                    // the phase's 16 steps, one pending pair per lane at a time (below), so a step costs
+                    // the derivative only where the lane itself has such a pair
                     inr = d2 < rp.range_sq;
-                    pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig, inr);
-                    if (inr) {
-                        V = flip_if(rp.h * dsig, neg);
-                        if (!rp.heaviside && d2 > rp.cut_sq) {  // simulation.cpp:469-470
-                            const double e = sqrt(d2) - rp.cutoff;
-                            V *= exp(-(e * e) * rp.one_per);
-                        }
-                    }
+                    pair_field<false, EXACT>(dx, dy, E, R, c, sig, dsig);
                     const double f = flip_if(sig, neg);
                     q0 += f;
                     atomicAdd(&jq0[jl], f);
-                    q1 += V;
-                    atomicAdd(&jq1[jl], V);
+                    if (inr) pend |= 1u << tt;
                 } else {  // kVjp
                     pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
                     const double bb = bi * bj;
@@ -281,6 +275,34 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
             }
             if (kDense) stage[((lane >> 4) * 16 + (lane & 15)) * 17 + ((lane & 15) ^ tt)] = V;
             if (MODE == kModeJacDenseCut) inr_phase |= __ballot(inr);
+        }
+        if constexpr (MODE == kModeJacDenseCut) {
+            // the phase's pairs inside the storage radius: each lane takes its own in step order, all
+            // lanes with one left together, until none is left.  V = h b_i b_j d sigma (the Gaussian
+            // taper beyond rc without -h, simulation.cpp:461-475) into the lane's stage slot of that step
+            // (where the step left 0) and into the row and column sums.
+            while (__ballot(pend != 0u)) {
+                if (pend) {
+                    const int tp = __builtin_ctz(pend);
+                    pend &= pend - 1u;
+                    const int jl = lane ^ (ph * 16 + tp);
+                    const int j = td.J0 + jl;
+                    const double dx = wrap(pi.x - sx[jl]);
+                    const double dy = wrap(pi.y - sy[jl]);
+                    const double d2 = dist2(dx, dy);
+                    double E = 0.0, R = 0.0, c = 0.0, sig = 0.0, dsig = 0.0;
+                    field_inputs(dx, dy, E, R, c);
+                    pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
+                    double V = flip_if(rp.h * dsig, ipos != (j < half));
+                    if (!rp.heaviside && d2 > rp.cut_sq) {  // simulation.cpp:469-470
+                        const double e = sqrt(d2) - rp.cutoff;
+                        V *= exp(-(e * e) * rp.one_per);
+                    }
+                    q1 += V;
+                    atomicAdd(&jq1[jl], V);
+                    stage[((lane >> 4) * 16 + (lane & 15)) * 17 + ((lane & 15) ^ tp)] = V;
+                }
+            }
         }
         if (kDense) {
             __syncthreads();

@@ -245,6 +245,34 @@ def test_jacobian_dense_vs_oracle(pkg, ctx, orc, n):
     assert np.allclose(A, Acsc, rtol=1e-13, atol=1e-13 * np.abs(Acsc).max())
 
 
+@pytest.mark.parametrize("case", ["heaviside1024", "taper300", "close8", "nopairs16", "dense64"])
+def test_jacobian_dense_cut_vs_csc_pass(pkg, ctx, orc, case):
+    """The dense pass with the reference's storage cutoff (d4_jacobian_dense_cut_dev: sigma of every pair in
+    the tile steps, d sigma of the pairs inside the radius taken after each 16-step phase): its scaled matrix is
+    the CSC pass's A (pinned to the oracle above) entry for entry, zero outside the stored pattern, and its
+    speeds and weights are the CSC pass's."""
+    import torch
+    xy, k = _jac_case(case)
+    n = xy.shape[0]
+    print(f"[dense cut] {case} n={n}", flush=True)
+    cutoff, cut_sq, one_per = orc.cutoff(n, k["mult"], k["hv"])
+    p = pkg.jac_params(h=k["h"], tau_a=k["ta"], tau_b=k["tb"], cutoff=cutoff, heaviside=k["hv"], wfunc=k["wf"])
+    t = torch.tensor(xy, dtype=torch.float64, device="cuda:0")
+    vA, vB, At, d = ctx.jacobian_dense_t(t, p, cutoff=True, scale=True)  # At[col, row]
+    A = At.cpu().numpy().T
+    G = ctx.jacobian_pass(xy, p)
+    Acsc = np.zeros((n, n))
+    cols = np.repeat(np.arange(n), np.diff(G["Ap"]))
+    Acsc[G["Ai"], cols] = G["Ax"]
+    stored = np.zeros((n, n), dtype=bool)
+    stored[G["Ai"], cols] = True
+    assert np.all(A[~stored] == 0.0)
+    assert np.allclose(A, Acsc, rtol=1e-13, atol=1e-13 * np.abs(Acsc).max())
+    assert np.allclose(vA.cpu().numpy(), G["vA"], rtol=0, atol=1e-13 * np.abs(G["vA"]).max())
+    assert np.allclose(vB.cpu().numpy(), G["vB"], rtol=0, atol=1e-13 * np.abs(G["vB"]).max())
+    assert np.allclose(d.cpu().numpy(), G["dvec"], rtol=1e-12, atol=1e-15)
+
+
 # ------------------------------------------------------------------ full size, device-resident
 
 def test_full_size_speed_and_dense_invariants(pkg, ctx, orc):
