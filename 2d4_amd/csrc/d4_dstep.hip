@@ -147,49 +147,59 @@ __global__ void ds_begin_kernel(DsArgs a)
 // storage radius at cfg, the configuration of the pass that wrote it (the same test, d^2 of the lower
 // index minus the higher, as the pass: the pass leaves the rest unwritten); the others are 0 as the pass
 // would have stored them.
-__global__ __launch_bounds__(256) void ds_assemble_kernel(DsArgs a, const double2* cfg, int rescaled, int only_flagged)
+// A group of columns of one 64-column block (blk: realization, block; the block split over `split`
+// workgroups, group blockIdx.x % split) per workgroup: each of its four wavefronts takes every fourth
+// column, lane l the rows r0 + l + 64 k of the block's row range -- the same rows in every column, so a
+// row's order, position and factors are read once per k for the wavefront's columns.
+__global__ __launch_bounds__(256) void ds_assemble_kernel(DsArgs a, const int2* __restrict__ blk, int split,
+                                                          const double2* cfg, int rescaled, int only_flagged)
 {
 #pragma clang fp contract(off)
-    const int r = blockIdx.y, c = blockIdx.x;
+    const int2 rb = blk[blockIdx.x / split];
+    const int r = rb.x, bc = rb.y, cw = kTile / split, c0 = bc * kTile + (blockIdx.x % split) * cw;
+    if (only_flagged && !a.flag[r]) return;
     const int np = a.npad[r];
-    if (c >= np || (only_flagged && !a.flag[r])) return;
     const int base = a.off[r], n = a.off[r + 1] - base;
     const int* ord = a.ordoff[r] >= 0 ? a.lu_aux + a.ordoff[r] : nullptr;
     int r0 = 0, r1 = np;
     if (!only_flagged && a.envoff[r] >= 0) {
-        const int nb = np / kTile, bc = c / kTile;
+        const int nb = np / kTile;
         r0 = a.lu_aux[a.envoff[r] + nb + bc] * kTile;
         r1 = min(np, (a.lu_aux[a.envoff[r] + bc] + 1) * kTile);
     }
-    double* col = a.store + a.aoff[r] + (size_t)c * np;
-    const int oc = (ord && c < n) ? ord[c] : c;
-    const double* mc = a.M + a.moff[r] + (size_t)oc * n;
+    double* const store = a.store + a.aoff[r];
+    const double* const M = a.M + a.moff[r];
     const double range_sq = (a.par[r].heaviside ? 1.0 : 49.0) * a.st[r].cut_sq;  // the pass's storage test
-    const double2 pc = cfg[base + (c < n ? oc : 0)];
-    for (int row = r0 + threadIdx.x; row < r1; row += blockDim.x) {
-        double v;
-        if (row >= n || c >= n) {
-            v = row == c ? 1.0 : 0.0;
-        } else {
-            const int orow = ord ? ord[row] : row;
-            double m = 0.0;
-            if (orow == oc) {
-                m = mc[orow];  // s, from the pass's reduction
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int row = r0 + lane; row < r1; row += 64) {  // r0, r1 multiples of 64: every lane takes a row
+        const bool real = row < n;
+        const int orow = real ? (ord ? ord[row] : row) : 0;
+        const double2 pr = cfg[base + orow];
+        const double fr = (1 + a.dvec[base + orow]) / 2;
+        const double f = rescaled ? a.fac[base + orow] : 0.0;
+        for (int cc = w; cc < cw; cc += 4) {
+            const int c = c0 + cc;
+            double v;
+            if (!real || c >= n) {
+                v = row == c ? 1.0 : 0.0;
             } else {
-                const double2 pr = cfg[base + orow];
-                const bool lower = orow < oc;
-                const double dx = lower ? wrap(pr.x - pc.x) : wrap(pc.x - pr.x);
-                const double dy = lower ? wrap(pr.y - pc.y) : wrap(pc.y - pr.y);
-                if (dist2(dx, dy) < range_sq) m = mc[orow];
+                const int oc = ord ? ord[c] : c;
+                double m = 0.0;
+                if (orow == oc) {
+                    m = M[(size_t)oc * n + orow];  // s, from the pass's reduction
+                } else {
+                    const double2 pc = cfg[base + oc];
+                    const bool lower = orow < oc;
+                    const double dx = lower ? wrap(pr.x - pc.x) : wrap(pc.x - pr.x);
+                    const double dy = lower ? wrap(pr.y - pc.y) : wrap(pc.y - pr.y);
+                    if (dist2(dx, dy) < range_sq) m = M[(size_t)oc * n + orow];
+                }
+                v = m * fr;
+                if (row == c) v = v + 1;
+                if (rescaled) v = row == c ? 1 + (v - 1) / 2 * f : v / 2 * f;
             }
-            v = m * ((1 + a.dvec[base + orow]) / 2);
-            if (row == c) v = v + 1;
-            if (rescaled) {
-                const double f = a.fac[base + orow];
-                v = row == c ? 1 + (v - 1) / 2 * f : v / 2 * f;
-            }
+            store[(size_t)c * np + row] = v;
         }
-        col[row] = v;
     }
 }
 
@@ -451,10 +461,13 @@ struct d4_dstep {
     // cfg: the configuration of the Jacobian pass whose matrix is assembled
     int factor(const double2* cfg, int rescaled, hipStream_t st)
     {
-        const dim3 g(npad_max, R);
-        hipLaunchKernelGGL(ds_assemble_kernel, g, dim3(256), 0, st, a, cfg, rescaled, 0);
+        // a workgroup per 64-column block of a realization, or per part of one when there are few blocks
+        int split = 1;
+        while (split < 16 && (long long)nblocks * split < 2048) split *= 2;
+        const dim3 g(nblocks * split);
+        hipLaunchKernelGGL(ds_assemble_kernel, g, dim3(256), 0, st, a, blocks, split, cfg, rescaled, 0);
         if (lu->factor_unpivoted(a.store, flag_d, st)) return fail(D4_ERR_HIP, "device step: factorization");
-        hipLaunchKernelGGL(ds_assemble_kernel, g, dim3(256), 0, st, a, cfg, rescaled, 1);
+        hipLaunchKernelGGL(ds_assemble_kernel, g, dim3(256), 0, st, a, blocks, split, cfg, rescaled, 1);
         if (lu->factor_pivoted(a.store, flag_d, sing_d, st)) return fail(D4_ERR_HIP, "device step: factorization");
         hipLaunchKernelGGL(ds_factored_kernel, dim3((R + 255) / 256), dim3(256), 0, st, a);
         return hipGetLastError() == hipSuccess ? D4_OK : fail(D4_ERR_HIP, "device step: factorization");

@@ -163,24 +163,38 @@ constexpr int kFB = 64;                    // block order of the factorization a
 constexpr int kSB = kFB;                   // block of the triangular solves
 constexpr int kSolveThreads = 1024;
 
-// identity beyond n, zero in the leading block: column blockIdx.x of matrix
-// blockIdx.y.  With an envelope, only the rows the factorization, the check
-// and the solves touch in this column: blocks top .. last of its block.
-__global__ void fill_identity_kernel(const MatDesc* __restrict__ desc, double* __restrict__ A,
-                                     const int* __restrict__ aux)
+// Column groups of the per-column kernels below: a 64-column block of a matrix is split over `split`
+// workgroups (a power of two up to 16) when a launch would otherwise hold fewer than ~2048 of them
+inline int column_split(long long blocks)
+{
+    int s = 1;
+    while (s < 16 && blocks * s < 2048) s *= 2;
+    return s;
+}
+
+// identity beyond n, zero in the leading block: the columns of group blockIdx.x % split of block
+// blockIdx.x / split of matrix blockIdx.y, one wavefront per column at a time.  With an envelope, only
+// the rows the factorization, the check and the solves touch in these columns: blocks top .. last of
+// their block.
+__global__ __launch_bounds__(256) void fill_identity_kernel(const MatDesc* __restrict__ desc, double* __restrict__ A,
+                                                            const int* __restrict__ aux, int split)
 {
     const MatDesc m = desc[blockIdx.y];
-    const int c = blockIdx.x;
-    if (c >= m.npad) return;
+    const int b = blockIdx.x / split, cw = kFB / split, c0 = b * kFB + (blockIdx.x % split) * cw;
+    if (b * kFB >= m.npad) return;
     int r0 = 0, r1 = m.npad;
     if (m.env_off >= 0) {
-        const int nb = m.npad / kFB, b = c / kFB;
+        const int nb = m.npad / kFB;
         r0 = aux[m.env_off + nb + b] * kFB;
         r1 = min(m.npad, (aux[m.env_off + b] + 1) * kFB);
     }
-    double* col = A + m.a_off + (size_t)c * m.npad;
-    const double diag = c >= m.n ? 1.0 : 0.0;
-    for (int r = r0 + threadIdx.x; r < r1; r += blockDim.x) col[r] = r == c ? diag : 0.0;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int cc = w; cc < cw; cc += 4) {
+        const int c = c0 + cc;
+        double* col = A + m.a_off + (size_t)c * m.npad;
+        const double diag = c >= m.n ? 1.0 : 0.0;
+        for (int r = r0 + lane; r < r1; r += 64) col[r] = r == c ? diag : 0.0;
+    }
 }
 
 // the CSC entries of column c (one thread) into the dense matrix, rows and
@@ -215,22 +229,30 @@ __global__ void dense_copy_kernel(const MatDesc* __restrict__ desc, double* __re
 // and nonzero and every multiplier |l_rc| <= kMaxMultiplier, i.e. unless
 // threshold partial pivoting with u = 1/kMaxMultiplier would keep the diagonal
 // at every stage
-// (with an envelope the multipliers below its last block are zero, unread)
-__global__ void multiplier_check_kernel(const MatDesc* __restrict__ desc, const double* __restrict__ A,
-                                        int* __restrict__ flag, const int* __restrict__ aux)
+// (with an envelope the multipliers below its last block are zero, unread).
+// The columns of group blockIdx.x % split of block blockIdx.x / split of matrix blockIdx.y, one wavefront
+// per column at a time.
+__global__ __launch_bounds__(256) void multiplier_check_kernel(const MatDesc* __restrict__ desc,
+                                                               const double* __restrict__ A, int* __restrict__ flag,
+                                                               const int* __restrict__ aux, int split)
 {
     const MatDesc m = desc[blockIdx.y];
-    const int c = blockIdx.x;
-    if (c >= m.n) return;
-    const double* col = A + m.a_off + (size_t)c * m.npad;
+    const int b = blockIdx.x / split, cw = kFB / split, c0 = b * kFB + (blockIdx.x % split) * cw;
+    if (c0 >= m.n) return;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int r1 = m.env_off >= 0 ? min(m.n, (aux[m.env_off + b] + 1) * kFB) : m.n;
     bool bad = false;
-    if (threadIdx.x == 0) {
-        const double p = std::fabs(col[c]);
-        bad = !(p > 0.0) || !(p <= 1.7976931348623157e308);
+    for (int cc = w; cc < cw; cc += 4) {
+        const int c = c0 + cc;
+        if (c >= m.n) break;
+        const double* col = A + m.a_off + (size_t)c * m.npad;
+        if (lane == 0) {
+            const double p = std::fabs(col[c]);
+            bad |= !(p > 0.0) || !(p <= 1.7976931348623157e308);
+        }
+        for (int r = c + 1 + lane; r < r1; r += 64) bad |= !(std::fabs(col[r]) <= kMaxMultiplier);
     }
-    const int r1 = m.env_off >= 0 ? min(m.n, (aux[m.env_off + c / kFB] + 1) * kFB) : m.n;
-    for (int r = c + 1 + threadIdx.x; r < r1; r += blockDim.x) bad |= !(std::fabs(col[r]) <= kMaxMultiplier);
-    if (bad) atomicOr(flag + blockIdx.y, 1);
+    if (__any(bad) && lane == 0) atomicOr(flag + blockIdx.y, 1);
 }
 
 // LAPACK pivots (row i swapped with row ipiv[i] - 1, in order) into the
@@ -994,8 +1016,11 @@ public:
         for (int k0 = 0; k0 < K; k0 += kGridY) {
             const int kc = std::min(kGridY, K - k0);
             if (npad_max > 0)
-                hipLaunchKernelGGL(multiplier_check_kernel, dim3(npad_max, kc), dim3(256), 0, st, desc_.p + k0, A_.p,
-                                   flag + k0, aux_.p);
+            {
+                const int sp = column_split((long long)(npad_max / kFB) * kc);
+                hipLaunchKernelGGL(multiplier_check_kernel, dim3(npad_max / kFB * sp, kc), dim3(256), 0, st,
+                                   desc_.p + k0, A_.p, flag + k0, aux_.p, sp);
+            }
         }
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(info_h_.p, info, sizeof(int) * 2 * K, hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -1260,7 +1285,9 @@ private:
         }
         for (int k0 = 0; k0 < K; k0 += kGridY) {
             const int kc = std::min(kGridY, K - k0);
-            hipLaunchKernelGGL(fill_identity_kernel, dim3(npad_max, kc), dim3(256), 0, st, desc + k0, A_.p, aux_.p);
+            const int sp = column_split((long long)(npad_max / kFB) * kc);
+            hipLaunchKernelGGL(fill_identity_kernel, dim3(npad_max / kFB * sp, kc), dim3(256), 0, st, desc + k0, A_.p,
+                               aux_.p, sp);
             hipLaunchKernelGGL(scatter_kernel, dim3((npad_max + 255) / 256, kc), dim3(256), 0, st, desc + k0, A_.p,
                                aux_.p);
         }
@@ -1373,6 +1400,7 @@ public:
         if (poff > INT_MAX || btot > INT_MAX || aux.size() > (size_t)INT_MAX) return -1;
         store_ = a;
         btot_ = btot;
+        check_split_ = column_split((long long)(npad_max_ / kFB) * std::min(K, kGridY));
         // the factorization plan: matrices largest first, each within its envelope
         std::vector<int> order(K);
         for (int k = 0; k < K; ++k) order[k] = k;
@@ -1415,8 +1443,8 @@ public:
         if (d4::launch_fill_int(flag, 0, K_, st) != hipSuccess) return -2;
         if (launch_lu_steps(steps_, fd_d_, pre_d_, store, st)) return -2;
         for (int k0 = 0; k0 < K_; k0 += kGridY)
-            hipLaunchKernelGGL(multiplier_check_kernel, dim3(npad_max_, std::min(kGridY, K_ - k0)), dim3(256), 0, st,
-                               md_d_ + k0, store, flag + k0, aux_d_);
+            hipLaunchKernelGGL(multiplier_check_kernel, dim3(npad_max_ / kFB * check_split_, std::min(kGridY, K_ - k0)),
+                               dim3(256), 0, st, md_d_ + k0, store, flag + k0, aux_d_, check_split_);
         return hipGetLastError() != hipSuccess ? -2 : 0;
     }
 
@@ -1442,6 +1470,7 @@ public:
 private:
     static constexpr int kGridY = 65535;
     int K_ = 0, npad_max_ = 0, lds_order_ = 0;
+    int check_split_ = 1;  // column groups per block of the multiplier check
     long long store_ = 0, btot_ = 0;
     std::vector<long long> a_off_;
     std::vector<int> ord_off_, env_off_;
