@@ -118,6 +118,7 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
     double q0 = 0.0, q1 = 0.0;
     const bool track_tol = a.tol != nullptr && rp.tol_cap2 > 0.0;
     unsigned long long inr_phase = 0;  // mode 4: the lanes of this phase's steps with a pair inside the radius
+    const bool all_in = MODE == kModeJacDenseCut && rp.range_sq > 0.5;  // mode 4: every pair inside it
 
     for (int ph = 0; ph < 4; ++ph) {
         unsigned pend = 0u;  // mode 4: bit tt set when this lane's pair of step tt is inside the radius
@@ -233,13 +234,26 @@ __device__ __forceinline__ void pair_tile(const PassArgs& a, const TileDesc td)
                 } else if (MODE == kModeJacDenseCut) {
                     // sigma for every pair here; d sigma of the pairs inside the storage radius after
                     // the phase's 16 steps, one pending pair per lane at a time (below), so a step costs
-                    // the derivative only where the lane itself has such a pair
+                    // the derivative only where the lane itself has such a pair -- or here, with sigma,
+                    // when the radius holds every pair of the cell (d^2 <= 1/2 for wrapped dx, dy)
                     inr = d2 < rp.range_sq;
-                    pair_field<false, EXACT>(dx, dy, E, R, c, sig, dsig);
+                    const bool now = all_in;
+                    if (now) pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
+                    else pair_field<false, EXACT>(dx, dy, E, R, c, sig, dsig);
                     const double f = flip_if(sig, neg);
                     q0 += f;
                     atomicAdd(&jq0[jl], f);
-                    if (inr) pend |= 1u << tt;
+                    if (now) {
+                        V = flip_if(rp.h * dsig, neg);
+                        if (!rp.heaviside && d2 > rp.cut_sq) {  // simulation.cpp:469-470
+                            const double e = sqrt(d2) - rp.cutoff;
+                            V *= exp(-(e * e) * rp.one_per);
+                        }
+                        q1 += V;
+                        atomicAdd(&jq1[jl], V);
+                    } else if (inr) {
+                        pend |= 1u << tt;
+                    }
                 } else {  // kVjp
                     pair_field<true, EXACT>(dx, dy, E, R, c, sig, dsig);
                     const double bb = bi * bj;
