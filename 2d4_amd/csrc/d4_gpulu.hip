@@ -599,6 +599,59 @@ __global__ __launch_bounds__(256, 1) void lu_panel_kernel(const FacDesc* __restr
 // B[l / 16][l % 16], D[4 e + l / 16][l % 16] for e = 0..3), so that C and the
 // L21 operand are read down columns (coalesced) and only U12 goes through LDS.
 // Wave w owns the 32 x 32 quadrant rows 32 (w >> 1), columns 32 (w & 1) of C.
+// QuadUpdate is that quadrant's part for the pivoted factorization below;
+// lu_update_kernel writes the same steps out with its own addressing.
+struct QuadUpdate {
+    f64x4 acc[2][2];  // [tc][tr]: C^T tile (c = cw + 16 tc + 4 e + lk, r = rw + 16 tr + lr)
+    int lr, lk, rw, cw;
+    __device__ QuadUpdate(int w, int l) : lr(l & 15), lk(l >> 4), rw(32 * (w >> 1)), cw(32 * (w & 1)) {}
+    // the quadrant's entries of the tile at C (column-major, leading dimension lda)
+    __device__ void load(const double* __restrict__ C, size_t lda)
+    {
+#pragma unroll
+        for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+            for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[tc][tr][e] = C[(cw + 16 * tc + 4 * e + lk) * lda + rw + 16 * tr + lr];
+    }
+    // acc -= L21 U12: L21[r][k] at L[k * lda + r] (the tile's rows), U12[k][c] at U[k * kFLd + c] (LDS); the
+    // L21 operands of eight k-steps loaded together (a 1024-thread workgroup has 128 VGPRs a lane)
+    __device__ void apply(const double* __restrict__ L, size_t lda, const double* U)
+    {
+        const double* Lq = L + rw + lr;
+#pragma unroll
+        for (int h = 0; h < kFB / 4; h += 8) {
+            double bl[8][2];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const size_t k = 4 * (h + s) + lk;
+                bl[s][0] = Lq[k * lda];
+                bl[s][1] = Lq[k * lda + 16];
+            }
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int k = 4 * (h + s) + lk;
+                const double a0 = U[k * kFLd + cw + lr], a1 = U[k * kFLd + cw + 16 + lr];  // U12^T[c][k]
+                const double b0 = -bl[s][0], b1 = -bl[s][1];                                 // -L21^T[k][r]
+                acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+                acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+                acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+            }
+        }
+    }
+    __device__ void store(double* __restrict__ C, size_t lda) const
+    {
+#pragma unroll
+        for (int tc = 0; tc < 2; ++tc)
+#pragma unroll
+            for (int tr = 0; tr < 2; ++tr)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) C[(cw + 16 * tc + 4 * e + lk) * lda + rw + 16 * tr + lr] = acc[tc][tr][e];
+    }
+};
+
 __global__ __launch_bounds__(256) void lu_update_kernel(const FacDesc* __restrict__ fd, const int* __restrict__ pre,
                                                         const int* __restrict__ tb, int count,
                                                         double* __restrict__ A, int j0)
@@ -725,24 +778,35 @@ int launch_lu_steps(const std::vector<LuStep>& steps, const FacDesc* fdesc, cons
     return hipGetLastError() != hipSuccess ? -1 : 0;
 }
 
-// Partial pivoting LU (LAPACK getf2's order: the first row of largest magnitude in the column, the rows
-// swapped across every column, multipliers by division) of each matrix whose flag is set, one workgroup
-// per matrix, in place in its store; perm (npad entries per matrix, at its solve descriptor's perm_off)
-// receives its solve's row order, the identity for the matrices whose flag is clear, and singular[k]
-// whether a zero pivot was met.  The rare fallback of a factorization whose multipliers exceed the
-// threshold: a latency chain of n steps, but only launches, so it can sit in a captured graph.
+// Partial pivoting LU of each matrix whose flag is set, one workgroup (16 wavefronts) per matrix, in place
+// in its store, blocked like LAPACK's getrf by kFB columns: the panel (columns j0 .. j0 + kFB - 1, rows j0 ..
+// n - 1) factored column by column as getf2 does (the first row of largest magnitude in the column, the
+// panel's rows swapped, multipliers by division, the panel's later columns updated), the panel's row
+// interchanges applied to the other columns in order, U12 = L11^-1 A12 (one wavefront per column of A12, the
+// unit lower L11 in LDS), and A22 -= L21 U12 on the matrix cores, four 64 x 64 tiles at a time (QuadUpdate,
+// U12 tiles through LDS).  perm (npad entries per matrix, at its solve descriptor's perm_off) holds the
+// pivots during the factorization and then the solve's row order: the identity for the matrices whose flag
+// is clear.  singular[k]: whether a zero pivot was met (its column is then left as it is, as getf2 leaves
+// it).  The rare fallback of a factorization whose multipliers exceed the threshold: launches only, so it
+// can sit in a captured graph, with one compute unit per flagged matrix.
+// Rows and columns of the padding (n .. npad - 1) hold the identity and no interchange reaches them, so
+// the trailing update runs over whole tiles up to npad: their L21 rows and U12 columns are zero.
 constexpr int kPivThreads = 1024;
+constexpr int kPivTiles = kPivThreads / 256;  // trailing-update tiles in flight, four wavefronts each
 
 __global__ __launch_bounds__(kPivThreads) void lu_pivoted_kernel(const MatDesc* __restrict__ desc,
                                                                  const SolveDesc* __restrict__ sdesc,
                                                                  double* __restrict__ A, const int* __restrict__ flag,
                                                                  int* __restrict__ perm, int* __restrict__ singular)
 {
-    extern __shared__ int piv[];  // npad swaps, then the permutation
+    // U12 tiles of the update; L11 of the row-block solve (the first); the permutation at the end (as ints)
+    __shared__ double tb[kPivTiles * kFB * kFLd];
     __shared__ double rv[kPivThreads / 64];
     __shared__ int ri[kPivThreads / 64];
+    __shared__ int pk;
     const MatDesc m = desc[blockIdx.x];
     const int n = m.n, npad = m.npad, tid = threadIdx.x;
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     int* pm = perm + sdesc[blockIdx.x].perm_off;
     if (!flag[blockIdx.x]) {
         for (int i = tid; i < npad; i += kPivThreads) pm[i] = i;
@@ -752,57 +816,117 @@ __global__ __launch_bounds__(kPivThreads) void lu_pivoted_kernel(const MatDesc* 
     double* a = A + m.a_off;
     const size_t ld = (size_t)npad;
     int sing = 0;
-    for (int k = 0; k < n; ++k) {
-        // the pivot: largest |a_rk|, r >= k, the first one on ties
-        double bv = -1.0;
-        int bi = n;
-        for (int r = k + tid; r < n; r += kPivThreads) {
-            const double v = std::fabs(a[(size_t)k * ld + r]);
-            if (v > bv) { bv = v; bi = r; }
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            const double ov = __shfl_down(bv, o);
-            const int oi = __shfl_down(bi, o);
-            if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-        }
-        if ((tid & 63) == 0) { rv[tid >> 6] = bv; ri[tid >> 6] = bi; }
-        __syncthreads();
-        if (tid == 0) {
-            for (int w = 1; w < kPivThreads / 64; ++w)
-                if (rv[w] > rv[0] || (rv[w] == rv[0] && ri[w] < ri[0])) { rv[0] = rv[w]; ri[0] = ri[w]; }
-            piv[k] = ri[0] < n ? ri[0] : k;
-        }
-        __syncthreads();
-        const int p = piv[k];
-        if (p != k)
-            for (int c = tid; c < n; c += kPivThreads) {
-                const double t = a[(size_t)c * ld + k];
-                a[(size_t)c * ld + k] = a[(size_t)c * ld + p];
-                a[(size_t)c * ld + p] = t;
+    for (int j0 = 0; j0 < n; j0 += kFB) {
+        const int je = min(j0 + kFB, n);
+        // the panel, column by column
+        for (int k = j0; k < je; ++k) {
+            double* ck = a + (size_t)k * ld;
+            // the pivot: largest |a_rk|, r >= k, the first one on ties
+            double bv = -1.0;
+            int bi = n;
+            for (int r = k + tid; r < n; r += kPivThreads) {
+                const double v = std::fabs(ck[r]);
+                if (v > bv) { bv = v; bi = r; }
             }
-        __syncthreads();
-        const double d = a[(size_t)k * ld + k];
-        if (d == 0.0) {
-            sing = 1;
+            for (int o = 32; o > 0; o >>= 1) {
+                const double ov = __shfl_down(bv, o);
+                const int oi = __shfl_down(bi, o);
+                if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+            }
+            if (lane == 0) { rv[wv] = bv; ri[wv] = bi; }
             __syncthreads();
-            continue;
+            if (tid == 0) {
+                for (int w = 1; w < kPivThreads / 64; ++w)
+                    if (rv[w] > rv[0] || (rv[w] == rv[0] && ri[w] < ri[0])) { rv[0] = rv[w]; ri[0] = ri[w]; }
+                pk = ri[0] < n ? ri[0] : k;
+                pm[k] = pk;
+            }
+            __syncthreads();
+            const int p = pk;
+            if (p != k && tid < je - j0) {  // the panel's two rows
+                double* c = a + (size_t)(j0 + tid) * ld;
+                const double t = c[k];
+                c[k] = c[p];
+                c[p] = t;
+            }
+            __syncthreads();
+            const double d = ck[k];
+            if (d == 0.0) {
+                sing = 1;
+                continue;  // uniform: every thread read the same pivot
+            }
+            for (int r = k + 1 + tid; r < n; r += kPivThreads) ck[r] /= d;
+            __syncthreads();
+            // the panel's later columns, rows below k: a_rc -= a_rk a_kc
+            for (int r = k + 1 + tid; r < n; r += kPivThreads) {
+                const double l = ck[r];
+                for (int c = k + 1; c < je; ++c) {
+                    double* cc = a + (size_t)c * ld;
+                    cc[r] -= l * cc[k];
+                }
+            }
+            __syncthreads();
         }
-        for (int r = k + 1 + tid; r < n; r += kPivThreads) a[(size_t)k * ld + r] /= d;
-        __syncthreads();
-        for (int c = k + 1; c < n; ++c) {
-            const double akc = a[(size_t)c * ld + k];
-            if (akc == 0.0) continue;
-            for (int r = k + 1 + tid; r < n; r += kPivThreads) a[(size_t)c * ld + r] -= a[(size_t)k * ld + r] * akc;
+        // the panel's interchanges on the columns outside it, in pivot order
+        for (int c = tid; c < n; c += kPivThreads) {
+            if (c >= j0 && c < je) continue;
+            double* cc = a + (size_t)c * ld;
+            for (int k = j0; k < je; ++k) {
+                const int p = pm[k];
+                if (p != k) {
+                    const double t = cc[k];
+                    cc[k] = cc[p];
+                    cc[p] = t;
+                }
+            }
+        }
+        if (je >= n) break;  // the last panel: no columns right of it
+        // U12 = L11^-1 A12: L11 (unit lower, rows and columns j0 .. j0 + kFB - 1) into LDS as L[k][i]
+        double* L11 = tb;
+        for (int e = tid; e < kFB * kFB; e += kPivThreads) {
+            const int i = e % kFB, k = e / kFB;
+            L11[k * kFLd + i] = a[(size_t)(j0 + k) * ld + j0 + i];
         }
         __syncthreads();
+        for (int c = je + wv; c < n; c += kPivThreads / 64) {
+            double* cc = a + (size_t)c * ld + j0;
+            double x = cc[lane];
+            for (int k = 0; k < kFB - 1; ++k) {
+                const double xk = lane_value(x, k);
+                if (lane > k) x = fma(-L11[k * kFLd + lane], xk, x);
+            }
+            cc[lane] = x;
+        }
+        __syncthreads();
+        // A22 -= L21 U12 over the tiles of rows and columns je .. npad - 1 (je = j0 + kFB here)
+        const int T = (npad - je) / kFB;
+        const int tg = wv >> 2;
+        double* U = tb + tg * kFB * kFLd;
+        for (int t0 = 0; t0 < T * T; t0 += kPivTiles) {
+            const int t = t0 + tg;
+            const bool act = t < T * T;
+            const size_t r0 = (size_t)je + kFB * (act ? t / T : 0), c0 = (size_t)je + kFB * (act ? t % T : 0);
+            QuadUpdate u(wv & 3, lane);
+            if (act) {
+                for (int c = wv & 3; c < kFB; c += 4) U[lane * kFLd + c] = a[(c0 + c) * ld + j0 + lane];  // U12[k][c]
+                u.load(a + c0 * ld + r0, ld);
+            }
+            __syncthreads();
+            if (act) {
+                u.apply(a + (size_t)j0 * ld + r0, ld, U);
+                u.store(a + c0 * ld + r0, ld);
+            }
+            __syncthreads();
+        }
     }
-    // the swap sequence into the permutation (rows beyond n stay in place)
-    int* q = piv + npad;
+    // the pivots into the permutation (rows beyond n stay in place)
+    int* q = reinterpret_cast<int*>(tb);
+    __syncthreads();
     for (int i = tid; i < npad; i += kPivThreads) q[i] = i;
     __syncthreads();
     if (tid == 0)
         for (int i = 0; i < n; ++i) {
-            const int j = piv[i];
+            const int j = pm[i];
             if (j != i) {
                 const int t = q[i];
                 q[i] = q[j];
@@ -1398,6 +1522,8 @@ public:
             npad_max_ = std::max(npad_max_, npad);
         }
         if (poff > INT_MAX || btot > INT_MAX || aux.size() > (size_t)INT_MAX) return -1;
+        // the pivoted factorization composes each permutation in its LDS tile buffer
+        if (npad_max_ > kPivTiles * kFB * kFLd * (int)(sizeof(double) / sizeof(int))) return -1;
         store_ = a;
         btot_ = btot;
         check_split_ = column_split((long long)(npad_max_ / kFB) * std::min(K, kGridY));
@@ -1450,7 +1576,7 @@ public:
 
     int factor_pivoted(double* store, const int* flag, int* singular, hipStream_t st) override
     {
-        hipLaunchKernelGGL(lu_pivoted_kernel, dim3(K_), dim3(kPivThreads), sizeof(int) * 2 * npad_max_, st, md_d_,
+        hipLaunchKernelGGL(lu_pivoted_kernel, dim3(K_), dim3(kPivThreads), 0, st, md_d_,
                            sd_d_, store, flag, perm_d_, singular);
         return hipGetLastError() != hipSuccess ? -2 : 0;
     }
