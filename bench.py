@@ -455,6 +455,40 @@ def c5_adjoint_bench(n=2048, T=256, ncoord=32, seed=2048, h=1e-6):
             "worst_rel_vs_fd": worst, "checks": rows}
 
 
+def c4_device_run(iterations=200, R=256, seed=4242):
+    """The C4 workload (the bench ensemble: R realizations, N_r even in [256, 4096], heaviside cutoff 0.05,
+    tau = 0.05, seed 4242) through the device-resident step for `iterations` lockstep iterations, graph capture
+    and the first iteration included in the timed region; progress every 20 iterations."""
+    dstep = importlib.import_module("2d4_amd.dstep")
+    pkg = importlib.import_module("2d4_amd")
+    ctx = pkg.Context(0)
+    rng = np.random.default_rng(seed)
+    sizes = 2 * rng.integers(128, 2049, R)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    T = int(off[-1])
+    x = rng.uniform(-0.5, 0.5, T)
+    y = rng.uniform(-0.5, 0.5, T)
+    b = np.concatenate([np.where(np.arange(m) % 2 == 0, -1.0, 1.0) for m in sizes])
+    params = [dict(cutoff_mult=0.05 * float(np.sqrt(m)), heaviside=True, stress=("fixed", 0.05, 0.0)) for m in sizes]
+    t0 = time.perf_counter()
+    s = dstep.DeviceSimulation(ctx, off, x, y, b, params)
+    done = 0
+    while done < iterations:
+        k = min(20, iterations - done)
+        s.run(k)
+        done += k
+        log(f"[bench] C4 device step: {done}/{iterations} iterations, {time.perf_counter() - t0:.1f} s")
+    t = time.perf_counter() - t0
+    st = s.stats()
+    s.close()
+    fac = sum(q["factorizations"] for q in st)
+    unp = sum(q["unpivoted_factorizations"] for q in st)
+    return {"workload": f"C4: R={R}, N_r even in [256, 4096] (sum {T}), heaviside cutoff 0.05, tau=0.05, seed {seed}",
+            "iterations": iterations, "seconds": t, "realization_steps_per_s": R * iterations / t,
+            "accepted": int(sum(q["successful_steps"] for q in st)), "failed": int(sum(q["failed_steps"] for q in st)),
+            "factorizations": int(fac), "pivoted_factorizations": int(fac - unp)}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -492,6 +526,8 @@ def main():
                     help="only the adjoint measurement: N,T,dt[,method] (e.g. 2048,256,1e-3; method theta: T steps of "
                          "dt of the reference's step, adaptive: the steps the reference's loop accepts until T dt, "
                          "backward_euler), checkpoints every 16 steps")
+    ap.add_argument("--c4-dstep", type=int, default=0,
+                    help="only the C4 workload through the device-resident step for this many lockstep iterations")
     ap.add_argument("--c5-adjoint", type=str, default="",
                     help="only C5's strain gradient: N,T,ncoord (e.g. 2048,256,32) through T steps of the reference's "
                          "own accepted sizes from uniform positions, against central differences on ncoord coordinates")
@@ -514,6 +550,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"calib_copy_bytes": nel * 8}), flush=True)
+        return
+    if args.c4_dstep:
+        print(json.dumps({"c4_device_step": c4_device_run(args.c4_dstep)}), flush=True)
         return
     if args.c5_adjoint:
         n_, T_, k_ = (int(v) for v in args.c5_adjoint.split(","))
