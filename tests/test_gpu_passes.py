@@ -88,6 +88,57 @@ def test_speed_pass_deterministic(ctx):
     assert np.array_equal(a, b)
 
 
+def test_single_dislocation(pkg, ctx, orc):
+    """n = 1: no pair; its b is -1 (the first n/2 = 0 have b = +1, simulation_data.cpp:265-270), so its speed is
+    -tau, its tolerance is untouched, and the Jacobian is the 1 x 1 identity with d = 0."""
+    xy = np.array([[0.1, -0.2]])
+    tol = np.array([0.5])
+    v = ctx.speed_pass(xy, 0.3, tol, 1.0)
+    assert np.array_equal(v, orc.speed_pass(xy, 0.3)) and v[0] == -0.3 and tol[0] == 0.5
+    p = pkg.jac_params(h=1e-3, tau_a=0.1, tau_b=0.2, n=1)
+    G = ctx.jacobian_pass(xy, p)
+    assert G["nnz"] == 1 and list(G["Ap"]) == [0, 1] and list(G["Ai"]) == [0] and G["Ax"][0] == 1.0
+    assert G["vA"][0] == -0.1 and G["vB"][0] == -0.2 and G["dvec"][0] == 0.0
+
+
+def test_coincident_pair_is_nan_as_in_the_reference(ctx, orc):
+    """Two dislocations at the same point: the reference's near-field series divides 0 by 0 (Field.cpp:176-188),
+    so both their speeds are NaN; the other rows stay finite and within the bound, and the tolerance of the two
+    is d^2 = 0, as the oracle gives."""
+    xy = random_config(40, 21)
+    xy[7] = xy[30]
+    n = xy.shape[0]
+    cap2 = 1e-2
+    t_gpu, t_orc = np.full(n, cap2), np.full(n, cap2)
+    v = ctx.speed_pass(xy, 0.05, t_gpu, cap2)
+    v0 = orc.speed_pass(xy, 0.05, t_orc, cap2)
+    assert np.array_equal(np.isnan(v), np.isnan(v0)) and set(np.flatnonzero(np.isnan(v))) == {7, 30}
+    assert np.array_equal(t_gpu, t_orc) and t_gpu[7] == 0.0 and t_gpu[30] == 0.0
+    ok = ~np.isnan(v0)
+    assert np.all(np.isfinite(v[ok]))
+    with np.errstate(invalid="ignore"):
+        allow = speed_allowance(orc, xy)
+    assert np.all(np.abs(v[ok] - v0[ok]) <= allow[ok] + 1e-15)
+
+
+def test_speed_pass_at_n65536(ctx):
+    """The largest speed pass of the suite (65 536 dislocations, 2.1e9 pairs in 524 800 tiles): the pair sums
+    cancel (sum_i v_i = tau sum_i b_i = 0), a permutation-free property that holds at any size, and the pass is
+    bitwise repeatable."""
+    import torch
+    n = 65536
+    xy = random_config(n, 22)
+    t = torch.from_numpy(xy).to("cuda:0")
+    print(f"[max] speed pass n={n}", flush=True)
+    v = ctx.speed_pass_t(t, tau=0.1)
+    w = ctx.speed_pass_t(t, tau=0.1)
+    torch.cuda.synchronize()
+    assert torch.equal(v, w) and bool(torch.isfinite(v).all())
+    assert abs(v.sum().item()) <= 1e-12 * v.abs().sum().item()
+    b = torch.from_numpy(bsign(n)).to("cuda:0")
+    assert abs((v - 0.1 * b).sum().item()) <= 1e-12 * v.abs().sum().item()
+
+
 # ------------------------------------------------------------------ Jacobian (CSC)
 
 def _jac_case(name):
