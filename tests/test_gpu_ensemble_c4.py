@@ -116,3 +116,34 @@ def test_c4_device_step_matches_host_driven_ensemble(ctx):
         worst = max(worst, float(np.abs(wrap(pos_d[sl, 0] - pos_h[sl, 0])).max()))
     print(f"[c4] device step vs host-driven ensemble: max |dx| {worst:.3e}", flush=True)
     assert worst <= 1e-9
+
+
+def test_c4_device_step_200_iterations(ctx):
+    """C4 as stated: 200 lockstep iterations of all 256 realizations through the device-resident step (one
+    captured graph per iteration). Every realization takes 200 loop iterations (accepted + failed), keeps y,
+    keeps x normalised, and its strain is sum_i b_i (x_end - x_0) modulo whole cells; the factorizations are
+    three per iteration, the rare ones with row interchanges included."""
+    dstep = import_module("2d4_amd.dstep")
+    sizes, off, x, y, b, params = _c4()
+    s = dstep.DeviceSimulation(ctx, off, x, y, b, [{k: v for k, v in q.items() if k != "solver"} for q in params])
+    for k in range(10):
+        s.run(20)
+        print(f"[c4 device] {20 * (k + 1)} iterations", flush=True)
+    pos = s.positions()
+    st = s.stats()
+    s.close()
+    for r, n in enumerate(sizes):
+        sl = slice(off[r], off[r + 1])
+        o = dconf.sort_order(y[sl], b[sl])
+        start = np.stack([x[sl][o], y[sl][o]], axis=1)
+        p = pos[sl]
+        np.testing.assert_array_equal(p[:, 1], start[:, 1])
+        assert np.all((p[:, 0] >= -0.5) & (p[:, 0] < 0.5))
+        bs = np.where(np.arange(n) < n // 2, 1.0, -1.0)
+        gap = st[r]["strain"] - (bs * wrap(p[:, 0] - start[:, 0])).sum()
+        assert abs(gap - round(gap)) < 1e-9, (r, gap)
+        assert st[r]["successful_steps"] + st[r]["failed_steps"] == 200
+        assert st[r]["factorizations"] == 600
+    pivoted = sum(q["factorizations"] - q["unpivoted_factorizations"] for q in st)
+    accepted = sum(q["successful_steps"] for q in st)
+    print(f"[c4 device] {accepted} accepted steps, {pivoted} factorizations with row interchanges", flush=True)

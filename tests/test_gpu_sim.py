@@ -156,6 +156,29 @@ def test_c2_heaviside_n1024_100_steps(ctx, orc):
     _compare(ctx, orc, x, y, b, dict(C2, step_limit=99), tol=1e-4)
 
 
+def test_c2_heaviside_n1024_1000_steps_invariants(ctx):
+    """C2 as stated, to its 1000 accepted steps (-C 999): the run completes with every step accounted for, y
+    is untouched, x stays normalised, and the accumulated strain is sum_i b_i (x_end - x_0) modulo whole cells
+    (simulation.cpp:163-175, 713-720) -- properties that hold whatever the trajectory, which beyond ~25 steps
+    no two implementations that round differently share (test_c2_heaviside_n1024_100_steps)."""
+    x, y, b = _random(1024, 2024)
+    start = _sorted_xy(x, y, b)
+    s = simmod.Simulation.from_arrays(ctx, x, y, b, **dict(C2, step_limit=999))
+    st = s.run()
+    p = s.positions()
+    s.close()
+    print(f"C2: {st['successful_steps']} accepted + {st['failed_steps']} failed steps, t = {st['sim_time']:.4e}, "
+          f"strain {st['strain']:.6e}", flush=True)
+    assert st["successful_steps"] == 1000
+    np.testing.assert_array_equal(p[:, 1], start[:, 1])
+    assert np.all((p[:, 0] >= -0.5) & (p[:, 0] < 0.5))
+    bs = np.where(np.arange(1024) < 512, 1.0, -1.0)
+    d = p[:, 0] - start[:, 0]
+    gap = st["strain"] - (bs * (d - np.rint(d))).sum()
+    assert abs(gap - round(gap)) < 1e-9
+    assert np.isfinite(st["sim_time"]) and st["sim_time"] > 0
+
+
 def test_taper_dipole_cyclic_n256(ctx, orc):
     x, y, b = _random(256, 5)
     kw = dict(cutoff_mult=0.9, wfunc="p", stress=("cyclic", 0.05, 2.0, 0.01), prec=1e-6, dipole_prec=0.1,
