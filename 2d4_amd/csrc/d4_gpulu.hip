@@ -162,6 +162,7 @@ constexpr double kMaxMultiplier = 1000.0;  // threshold partial pivoting with u 
 constexpr int kFB = 64;                    // block order of the factorization and of the envelope
 constexpr int kSB = kFB;                   // block of the triangular solves
 constexpr int kSolveThreads = 1024;
+constexpr int kSolveLds = 2 * kSB * kSB + 2 * kSB;  // doubles of LDS before x: two staged blocks, two diagonals
 
 // Column groups of the per-column kernels below: a 64-column block of a matrix is split over `split`
 // workgroups (a power of two up to 16) when a launch would otherwise hold fewer than ~2048 of them
@@ -297,19 +298,24 @@ __device__ inline double lane_value(double v, int lane)  // v of `lane` (uniform
 // x := U^{-1} L^{-1} P b for the factors in A (getrf's layout: unit lower L
 // below the diagonal, U on and above), one workgroup per matrix.  Blocked by
 // kSB rows: the diagonal block is staged in LDS and solved by wave 0 (lane l
-// owns row l, the solved value of lane c broadcast at step c), then every
-// thread updates rows outside the block with the block's columns
-// (x_r -= sum_c a_rc x_c, accumulated in ascending c).  The order of
-// operations is fixed by the matrix alone.  x lives in LDS (kLds) or in
-// global scratch; the arithmetic is the same either way.  A reordered matrix
-// gathers b through ord and scatters x back through it; with an envelope the
-// updates stop at its bounds, past which the factors' columns are zero (a
-// zero sum leaves x_r as it is, so the bits are those of the full sweep); a
-// matrix whose full flag is set is swept whole.
-// Latency: the next diagonal block is staged (by waves 1..15, into the other
-// of two LDS buffers) while wave 0 solves the current one; wave 0 reads its
-// entries of the block kAhead steps before their use; a row update issues
-// its column loads 32 at a time.
+// owns row l, the solved value of lane c broadcast at step c), then waves
+// 1.. update the rows outside the block with the block's columns,
+// x_r -= sum_c a_rc x_c.  A row is shared by P threads (P = 4, 2, 1 for
+// npad <= 1024, <= 2048, larger): thread k of the row accumulates the columns
+// c = k, k + P, ... in ascending order, and the P partial sums are added as
+// (s0 + s1) + (s2 + s3) in every thread of the row.  The order of operations
+// is fixed by the matrix alone.  x lives in LDS (kLds) or in global scratch;
+// the arithmetic is the same either way.  A reordered matrix gathers b
+// through ord and scatters x back through it; with an envelope the updates
+// stop at its bounds, past which the factors' columns are zero (a zero sum
+// leaves x_r as it is, so the bits are those of the full sweep); a matrix
+// whose full flag is set is swept whole.
+// Latency: every update thread loads its first row's factor entries of the
+// block (up to 32) before the block's diagonal solve, which they do not
+// depend on, so that the loads are in flight during it; the next diagonal
+// block is staged (by waves 1..15, into the other of two LDS buffers) at the
+// same time; wave 0 reads its entries of the block kAhead steps before their
+// use.
 template <bool kLds>
 __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc* __restrict__ desc,
                                                                   const double* __restrict__ A,
@@ -318,29 +324,44 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
                                                                   const int* __restrict__ aux,
                                                                   const int* __restrict__ full)
 {
-    constexpr int kAhead = 8, kBatch = 32;
+    constexpr int kAhead = 8, kPre = 16;
+    constexpr int kStage = (kSB * kSB + kSolveThreads - 65) / (kSolveThreads - 64);  // entries per staging thread
     extern __shared__ double lds[];
     const SolveDesc m = desc[blockIdx.x];
     const int n = m.n, tid = threadIdx.x;
     const size_t lda = (size_t)m.npad;
     const double* Am = A + m.a_off;
     double* bm = b + m.b_off;
-    double* Dbuf = lds;  // two kSB x kSB diagonal blocks, column-major
-    double* x = kLds ? lds + 2 * kSB * kSB : scratch + m.b_off;
+    double* Dbuf = lds;                   // two kSB x kSB diagonal blocks, column-major
+    double* Ddiag = lds + 2 * kSB * kSB;  // the diagonals of U's blocks staged there
+    double* x = kLds ? lds + kSolveLds : scratch + m.b_off;
     const int* ord = m.ord_off >= 0 ? aux + m.ord_off : nullptr;
     // full[k] set: matrix k was refactored with pivoting, whose factors are not held to its envelope
     const int* env = (m.env_off >= 0 && !(full && full[blockIdx.x])) ? aux + m.env_off : nullptr;
     const int nb = m.npad / kSB;
     const int nblk = (n + kSB - 1) / kSB;
-    // diagonal block number s of the two sweeps (0..2 nblk - 1) into buffer s & 1,
-    // by the threads t0, t0 + stride, ...
+    // diagonal block number s of the two sweeps (0..2 nblk - 1) into buffer s & 1, by the threads t0,
+    // t0 + stride, ...: the strictly lower part of L's block (first sweep) or the strictly upper part of U's
+    // block and its diagonal (second sweep), zeros elsewhere
     auto stage = [&](int s, int t0, int stride) {
-        const int bi = s < nblk ? s : 2 * nblk - 1 - s;
+        const bool up = s >= nblk;
+        const int bi = up ? 2 * nblk - 1 - s : s;
         const int jb = bi * kSB, B = min(kSB, n - jb);
         double* D = Dbuf + (s & 1) * kSB * kSB;
-        for (int e = t0; e < kSB * kSB; e += stride) {
-            const int r = e % kSB, c = e / kSB;
-            D[e] = (r < B && c < B) ? Am[(size_t)(jb + c) * lda + jb + r] : 0.0;
+        // all of a thread's loads are issued before its first store (at most kStage entries a thread)
+        double v[kStage];
+#pragma unroll
+        for (int i = 0; i < kStage; ++i) {
+            const int e = t0 + i * stride, r = e % kSB, c = e / kSB;
+            v[i] = (e < kSB * kSB && r < B && c < B) ? Am[(size_t)(jb + c) * lda + jb + r] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < kStage; ++i) {
+            const int e = t0 + i * stride, r = e % kSB, c = e / kSB;
+            if (e < kSB * kSB) {
+                D[e] = (up ? r < c : r > c) ? v[i] : 0.0;
+                if (up && r == c) Ddiag[(s & 1) * kSB + r] = (r < B) ? v[i] : 1.0;
+            }
         }
     };
     for (int i = tid; i < n; i += kSolveThreads) {
@@ -351,60 +372,90 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
     if (nblk > 0) stage(0, tid, kSolveThreads);
     __syncthreads();
 
+    // threads per row in the row updates, the columns each of them takes, and the rows of one round
+    const int P = m.npad <= 1024 ? 4 : m.npad <= 2048 ? 2 : 1;
+    const int W = kSB / P, rows = (kSolveThreads - 64) / P;
+    const int ut = tid - 64;
+    const int k = ut >= 0 ? ut % P : 0;
+    const int q = ut >= 0 ? ut / P : (1 << 29);  // wave 0 takes no rows
     for (int s = 0; s < 2 * nblk; ++s) {
         const int pass = s < nblk ? 0 : 1;  // 0: L y = P b top down, 1: U x = y bottom up
         const int bi = pass == 0 ? s : 2 * nblk - 1 - s;
         const int jb = bi * kSB, B = min(kSB, n - jb);
         const double* D = Dbuf + (s & 1) * kSB * kSB;
+        const int r0 = pass == 0 ? jb + B : (env ? env[2 * nb + bi] * kSB : 0);
+        const int r1 = pass == 0 ? (env ? min(n, (env[bi] + 1) * kSB) : n) : jb;
+        const bool whole = B == kSB;
+        const int rf = r0 + q;
+        const size_t step = (size_t)P * lda;  // from one of a thread's columns to the next
+        double av[kPre];  // the first kPre of the first row's entries
+        if (whole && rf < r1) {
+            const double* a = Am + (size_t)(jb + k) * lda + rf;
+#pragma unroll
+            for (int c = 0; c < kPre; ++c) av[c] = a[c * step];
+        }
         if (tid < 64) {
+            // lane l owns row l; the staged block is zero outside the triangle a step updates, so every
+            // lane takes every step with no mask (a zero factor leaves v as it is)
             const int l = tid;
             double v = l < B ? x[jb + l] : 0.0;
             double d[kSB];  // d[c] = D(l, c)
             if (pass == 0) {
 #pragma unroll
-                for (int k = 0; k < kAhead; ++k) d[k] = D[k * kSB + l];
+                for (int kk = 0; kk < kAhead; ++kk) d[kk] = D[kk * kSB + l];
 #pragma unroll
                 for (int c = 0; c < kSB - 1; ++c) {
                     if (c + kAhead < kSB - 1) d[c + kAhead] = D[(c + kAhead) * kSB + l];
-                    const double yc = lane_value(v, c);
-                    if (l > c) v = fma(-d[c], yc, v);
+                    v = fma(-d[c], lane_value(v, c), v);
                 }
             } else {
+                // x_c = v_c / u_cc, correctly rounded, from the reciprocal: q = v r, then q + (v - q u) r
+                // (exact residual by fma) is the quotient IEEE division gives, with no division on the
+                // step-to-step chain
+                const double ul = Ddiag[(s & 1) * kSB + l], rl = 1.0 / ul;
 #pragma unroll
-                for (int k = 0; k < kAhead; ++k) d[kSB - 1 - k] = D[(kSB - 1 - k) * kSB + l];
+                for (int kk = 0; kk < kAhead; ++kk) d[kSB - 1 - kk] = D[(kSB - 1 - kk) * kSB + l];
 #pragma unroll
                 for (int c = kSB - 1; c >= 0; --c) {
                     if (c - kAhead >= 0) d[c - kAhead] = D[(c - kAhead) * kSB + l];
                     if (c < B) {
-                        if (l == c) v = v / d[c];
-                        const double xc = lane_value(v, c);
-                        if (l < c) v = fma(-d[c], xc, v);
+                        const double vc = lane_value(v, c), uc = lane_value(ul, c), rc = lane_value(rl, c);
+                        const double q0 = vc * rc;
+                        v = fma(-d[c], fma(fma(-q0, uc, vc), rc, q0), v);
                     }
                 }
+                v = v / ul;  // lane l's own quotient, as step l formed it (no later step changes v_l)
             }
             if (l < B) x[jb + l] = v;
         } else if (s + 1 < 2 * nblk) {
             stage(s + 1, tid - 64, kSolveThreads - 64);
         }
         __syncthreads();
-        const int r0 = pass == 0 ? jb + B : (env ? env[2 * nb + bi] * kSB : 0);
-        const int r1 = pass == 0 ? (env ? min(n, (env[bi] + 1) * kSB) : n) : jb;
-        for (int r = r0 + tid; r < r1; r += kSolveThreads) {
-            const double* a = Am + (size_t)jb * lda + r;
+        // rows r0 + q, r0 + q + rows, ...: the same rows for the P threads of a row
+        for (int r = rf; r < r1; r += rows) {
+            const double* a = Am + (size_t)(jb + k) * lda + r;
+            const double* xk = x + jb + k;
             double acc = 0.0;
-            if (B == kSB) {
+            if (whole) {
+                if (r != rf) {
 #pragma unroll
-                for (int h = 0; h < kSB; h += kBatch) {
-                    double av[kBatch];
+                    for (int c = 0; c < kPre; ++c) av[c] = a[c * step];
+                }
 #pragma unroll
-                    for (int c = 0; c < kBatch; ++c) av[c] = a[(size_t)(h + c) * lda];
+                for (int c = 0; c < kPre; ++c) acc = fma(av[c], xk[P * c], acc);
+                for (int h = kPre; h < W; h += kPre) {
+                    double bv[kPre];
 #pragma unroll
-                    for (int c = 0; c < kBatch; ++c) acc = fma(av[c], x[jb + h + c], acc);
+                    for (int c = 0; c < kPre; ++c) bv[c] = a[(h + c) * step];
+#pragma unroll
+                    for (int c = 0; c < kPre; ++c) acc = fma(bv[c], xk[P * (h + c)], acc);
                 }
             } else {
-                for (int c = 0; c < B; ++c) acc = fma(a[c * lda], x[jb + c], acc);
+                for (int c = k; c < B; c += P) acc = fma(a[(size_t)(c - k) * lda], x[jb + c], acc);
             }
-            x[r] -= acc;
+            if (P >= 2) acc += __shfl_xor(acc, 1);
+            if (P >= 4) acc += __shfl_xor(acc, 2);
+            if (k == 0) x[r] -= acc;
         }
         __syncthreads();
     }
@@ -415,7 +466,7 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
 // Blocked right-looking LU without row interchanges for a ragged batch (every
 // matrix of order npad, a multiple of kFB).  Step p (j0 = kFB p) covers the
 // matrices with npad > j0, a prefix of the batch sorted by npad, largest first:
-//   lu_diag_kernel    factors the diagonal block A11 = L11 U11 (one workgroup per matrix, LDS);
+//   lu_diag_kernel    factors the diagonal block A11 = L11 U11 (one workgroup per matrix);
 //   lu_panel_kernel   L21 = A21 U11^-1 and U12 = L11^-1 A12, one wavefront per kFB x kFB block;
 //   lu_update_kernel  A22 -= L21 U12, one workgroup per kFB x kFB tile, on the matrix cores.
 // Per step and matrix, tb[i] = T: the blocks p + 1 .. p + T below and right
@@ -539,21 +590,29 @@ __global__ __launch_bounds__(256, 1) void lu_panel_kernel(const FacDesc* __restr
     const int T = tb[i];  // blocks below and right of the diagonal block that can be nonzero
     const double* dg = A + m.a_off + (size_t)j0 * lda + j0;
     const int t = threadIdx.x, lo = t & 63, q = t >> 6;
-    for (int c = q; c < kFB; c += 4) Dg[c * kFLd + lo] = dg[(size_t)c * lda + lo];
-    __syncthreads();
     const int item = g0 - pre[i] + wv;
-    if (item >= 2 * T) return;                     // padding
+    const bool live = item < 2 * T;                // else padding
     const bool rowblk = item < T;                  // L21 block (below) or U12 block (right)
     const int b = (rowblk ? item : item - T) + 1;  // block index past the diagonal
-    const int L = lo & 15;
+    double* blk = A + m.a_off + (size_t)j0 * lda + (j0 + kFB * b);
+    double* col = A + m.a_off + (size_t)(j0 + kFB * b + lo) * lda + j0;
+    // the block's own entries first: their loads are in flight while the diagonal block is staged
     double x[kFB], F[2][4];  // F[s & 1][k]: step s's factor entries 16 k + L
+    if (live && rowblk) {
+#pragma unroll
+        for (int c = 0; c < kFB; ++c) x[c] = blk[(size_t)c * lda + lo];
+    } else if (live) {
+#pragma unroll
+        for (int r = 0; r < kFB; ++r) x[r] = col[r];
+    }
+    for (int c = q; c < kFB; c += 4) Dg[c * kFLd + lo] = dg[(size_t)c * lda + lo];
+    __syncthreads();
+    if (!live) return;
+    const int L = lo & 15;
     if (rowblk) {
-        double* blk = A + m.a_off + (size_t)j0 * lda + (j0 + kFB * b);
         double R[4];  // lane 16 g + L of R[k]: 1 / U11(16 k + L, 16 k + L)
 #pragma unroll
         for (int k = 0; k < 4; ++k) R[k] = 1.0 / Dg[(16 * k + L) * kFLd + 16 * k + L];
-#pragma unroll
-        for (int c = 0; c < kFB; ++c) x[c] = blk[(size_t)c * lda + lo];
 #pragma unroll
         for (int k = 0; k < 4; ++k) F[0][k] = Dg[(16 * k + L) * kFLd];  // U11(0, 16 k + L)
         static_for<kFB>([&](auto cc) {
@@ -572,9 +631,6 @@ __global__ __launch_bounds__(256, 1) void lu_panel_kernel(const FacDesc* __restr
 #pragma unroll
         for (int c = 0; c < kFB; ++c) blk[(size_t)c * lda + lo] = x[c];
     } else {
-        double* col = A + m.a_off + (size_t)(j0 + kFB * b + lo) * lda + j0;
-#pragma unroll
-        for (int r = 0; r < kFB; ++r) x[r] = col[r];
 #pragma unroll
         for (int k = 0; k < 4; ++k) F[0][k] = Dg[16 * k + L];  // L11(16 k + L, 0)
         static_for<kFB - 1>([&](auto rr) {
@@ -945,7 +1001,7 @@ public:
         int lds = 0;
         if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lds = 65536;
         lds_bytes_ = lds;
-        lds_order_ = std::max(0, (lds / 8 - 2 * kSB * kSB) / 256 * 256);
+        lds_order_ = std::max(0, (lds / 8 - kSolveLds) / 256 * 256);
     }
 
     ~BatchLU() override
@@ -1390,11 +1446,11 @@ private:
         if (nsolve_ == 0) return 0;
         if (npad_max_ <= lds_order_)
             hipLaunchKernelGGL(lu_solve_kernel<true>, dim3(nsolve_), dim3(kSolveThreads),
-                               sizeof(double) * (2 * kSB * kSB + npad_max_), st, sdesc_.p, A_.p, ipiv_.p, b_.p,
+                               sizeof(double) * (kSolveLds + npad_max_), st, sdesc_.p, A_.p, ipiv_.p, b_.p,
                                b_.p + total, aux_.p, nullptr);
         else
             hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(nsolve_), dim3(kSolveThreads),
-                               sizeof(double) * 2 * kSB * kSB, st, sdesc_.p, A_.p, ipiv_.p, b_.p, b_.p + total,
+                               sizeof(double) * kSolveLds, st, sdesc_.p, A_.p, ipiv_.p, b_.p, b_.p + total,
                                aux_.p, nullptr);
         return hipGetLastError() != hipSuccess ? -1 : 0;
     }
@@ -1539,7 +1595,7 @@ public:
         }
         std::vector<int> pre;
         if (!plan_lu_steps(fd, last, steps_, pre)) return -1;
-        lds_order_ = std::max(0, (lds / 8 - 2 * kSB * kSB) / 256 * 256);
+        lds_order_ = std::max(0, (lds / 8 - kSolveLds) / 256 * 256);
         const size_t scratch = npad_max_ <= lds_order_ ? 1 : (size_t)btot;
         if (hipMalloc(&fd_d_, sizeof(FacDesc) * fd.size()) != hipSuccess ||
             hipMalloc(&pre_d_, sizeof(int) * std::max<size_t>(pre.size(), 1)) != hipSuccess ||
@@ -1585,10 +1641,10 @@ public:
     {
         if (npad_max_ <= lds_order_)
             hipLaunchKernelGGL(lu_solve_kernel<true>, dim3(K_), dim3(kSolveThreads),
-                               sizeof(double) * (2 * kSB * kSB + npad_max_), st, sd_d_, store, perm_d_, b, scratch_d_,
+                               sizeof(double) * (kSolveLds + npad_max_), st, sd_d_, store, perm_d_, b, scratch_d_,
                                aux_d_, flag_);
         else
-            hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(K_), dim3(kSolveThreads), sizeof(double) * 2 * kSB * kSB,
+            hipLaunchKernelGGL(lu_solve_kernel<false>, dim3(K_), dim3(kSolveThreads), sizeof(double) * kSolveLds,
                                st, sd_d_, store, perm_d_, b, scratch_d_, aux_d_, flag_);
         return hipGetLastError() != hipSuccess ? -2 : 0;
     }
