@@ -28,6 +28,15 @@ void hm_field_dy(long n, const double* dx, const double* dy, double* out)
     }
 }
 double hm_wrap(double v) { return d4::wrap(v); }
+// v / u as the triangular-solve kernel forms it off its step chain: q = v r with r = 1/u (correctly
+// rounded), then q + (v - q u) r with the residual exact by fma
+void hm_div_by_reciprocal(long n, const double* v, const double* u, double* out)
+{
+    for (long k = 0; k < n; ++k) {
+        const double r = 1.0 / u[k], q = v[k] * r;
+        out[k] = fma(fma(-q, u[k], v[k]), r, q);
+    }
+}
 // the reference's d sigma / dx and its derivatives in dx and dy (d4_dual.h)
 void hm_dsigma_dual(long n, const double* dx, const double* dy, double* v, double* ddx, double* ddy)
 {

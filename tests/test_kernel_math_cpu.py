@@ -22,6 +22,7 @@ def hm():
     L.hm_field_dy.argtypes = [C.c_long, vp, vp, vp]
     L.hm_wrap.restype = C.c_double
     L.hm_wrap.argtypes = [C.c_double]
+    L.hm_div_by_reciprocal.argtypes = [C.c_long, vp, vp, vp]
     return L
 
 
@@ -119,3 +120,23 @@ def test_kernel_own_inputs_within_bound_b(hm, orc):
     ad, d0 = pair_allowance(orc, dx, dy, e, c, deriv=True)
     assert np.all(np.abs(s - s0) <= a)
     assert np.all(np.abs(d - d0) <= ad)
+
+
+def test_division_by_correctly_rounded_reciprocal_with_one_correction_is_ieee_division(hm):
+    """The triangular solves' U step, q = v r (r = 1/u), then q + (v - q u) r: the same quotient as v / u,
+    bit for bit, over wide exponent ranges, divisors near 1 (the step matrices' diagonal) and mantissas of
+    every pattern."""
+    rng = np.random.default_rng(77)
+    n = 2_000_000
+
+    def spread(m, emin, emax):
+        mant = rng.uniform(1.0, 2.0, m) * rng.choice([-1.0, 1.0], m)
+        return np.ldexp(mant, rng.integers(emin, emax + 1, m))
+
+    v = np.concatenate([spread(n // 2, -60, 60), rng.uniform(-4, 4, n // 2)])
+    u = np.concatenate([spread(n // 4, -30, 30), 1.0 + spread(n // 4, -50, -1),
+                        rng.uniform(0.5, 2.0, n // 4), spread(n // 4, -3, 3)])
+    rng.shuffle(u)
+    out = np.empty(n)
+    hm.hm_div_by_reciprocal(n, _p(v), _p(u), _p(out))
+    assert np.array_equal(out, v / u)
