@@ -1100,24 +1100,38 @@ public:
         // each matrix's symmetric order and block envelope, from its pattern
         // (a dense device matrix keeps its order and is factored whole)
         plan_.resize(K);
+        cur_.assign(K, nullptr);
+        for (int k = 0; k < K; ++k) cur_[k] = m[k].key >= 0 ? &keyed_[m[k].key] : &plan_[k];
         long long aux_total = 0;
         if (!dense) {
             par(K, [&](int k) {
-                Plan& pl = plan_[k];
+                Plan& pl = *cur_[k];
                 const int n = m[k].n;
                 // the order depends on the pattern alone: kept while slot k's pattern repeats
                 if (n > 0 && pl.n == n && std::equal(m[k].Ap, m[k].Ap + n + 1, pl.pat_p.begin()) &&
-                    std::equal(m[k].Ai, m[k].Ai + m[k].Ap[n], pl.pat_i.begin()))
+                    pl.pat_i.size() == (size_t)m[k].Ap[n] && std::equal(m[k].Ai, m[k].Ai + m[k].Ap[n], pl.pat_i.begin()))
                     return;
+                // a keyed matrix whose pattern changed keeps its order while that order's envelope of the new
+                // pattern stays within 1/8 of the envelope it was chosen with
+                const bool keep = m[k].key >= 0 && n > 0 && pl.n == n && pl.base_tiles > 0;
                 pl.n = n;
                 pl.pat_p.assign(m[k].Ap, m[k].Ap + (n > 0 ? n + 1 : 0));
                 pl.pat_i.assign(m[k].Ai, m[k].Ai + (n > 0 ? m[k].Ap[n] : 0));
+                if (keep) {
+                    BlockEnvelope e = block_envelope(n, desc_h_[k].npad, kFB, m[k].Ap, m[k].Ai,
+                                                     pl.re ? pl.inv.data() : nullptr);
+                    if (8 * e.tiles <= 9 * pl.base_tiles) {
+                        pl.env = std::move(e);
+                        return;
+                    }
+                }
                 pl.re = n > 0 && band_order(n, desc_h_[k].npad, kFB, m[k].Ap, m[k].Ai, pl.ord, pl.inv, pl.env);
+                pl.base_tiles = pl.env.tiles;
             });
             for (int k = 0; k < K; ++k) {
                 MatDesc& d = desc_h_[k];
                 if (d.n == 0) continue;
-                if (plan_[k].re) {
+                if (cur_[k]->re) {
                     d.ord_off = (int)aux_total;
                     aux_total += 2LL * d.n;
                 }
@@ -1128,7 +1142,7 @@ public:
             if (!aux_.ensure((size_t)aux_total) || !aux_h_.ensure((size_t)aux_total)) return -2;
             par(K, [&](int k) {
                 const MatDesc& d = desc_h_[k];
-                const Plan& pl = plan_[k];
+                const Plan& pl = *cur_[k];
                 if (d.ord_off >= 0) {
                     std::copy(pl.ord.begin(), pl.ord.end(), aux_h_.p + d.ord_off);
                     std::copy(pl.inv.begin(), pl.inv.end(), aux_h_.p + d.ord_off + d.n);
@@ -1340,8 +1354,11 @@ private:
         BlockEnvelope env;
         int n = -1;  // the pattern the order was taken from
         std::vector<int> pat_p, pat_i;
+        long long base_tiles = 0;  // the envelope's tiles when the order was chosen
     };
-    std::vector<Plan> plan_;  // per matrix slot of the last factor() (CSC input)
+    std::vector<Plan> plan_;      // per matrix slot of the last factor() (CSC input)
+    std::map<int, Plan> keyed_;   // per caller key (Matrix::key >= 0)
+    std::vector<Plan*> cur_;      // the plan of each matrix of the last factor()
     DevBuf<int> aux_;         // orders and envelopes, at MatDesc::ord_off / env_off
     PinBuf<int> aux_h_;
     DevBuf<FacDesc> fdesc_;
@@ -1370,7 +1387,7 @@ private:
         for (int k : order)
             if (desc_h_[k].npad > 0) {
                 fd.push_back({desc_h_[k].a_off, desc_h_[k].npad, 0});
-                last.push_back(desc_h_[k].env_off >= 0 ? plan_[k].env.last.data() : nullptr);
+                last.push_back(desc_h_[k].env_off >= 0 ? cur_[k]->env.last.data() : nullptr);
             }
         if (fd.empty()) return 0;
         std::vector<LuStep> steps;
@@ -1495,6 +1512,7 @@ public:
         singular_ = unpivoted_ = false;
         if (n < 0 || (n > 0 && !Ap)) return -1;
         BatchDenseLU::Matrix m{n, Ap, Ai, Ax};
+        m.key = 0;  // one simulation's matrices: its order follows its own sequence of patterns
         unsigned char s = 0, u = 0;
         const int rc = lu_.factor(1, &m, &s, &u);
         if (rc < 0) return rc;
