@@ -170,6 +170,17 @@ __global__ __launch_bounds__(256) void ds_assemble_kernel(DsArgs a, const int2* 
     double* const store = a.store + a.aoff[r];
     const double* const M = a.M + a.moff[r];
     const double range_sq = (a.par[r].heaviside ? 1.0 : 49.0) * a.st[r].cut_sq;  // the pass's storage test
+    // the workgroup's columns: their original indices and positions, read once (the row loop below meets
+    // each column once per 64 rows)
+    __shared__ int col_o[kTile];
+    __shared__ double2 col_p[kTile];
+    for (int cc = threadIdx.x; cc < cw; cc += blockDim.x) {
+        const int c = c0 + cc;
+        const int oc = c < n ? (ord ? ord[c] : c) : -1;
+        col_o[cc] = oc;
+        col_p[cc] = oc >= 0 ? cfg[base + oc] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     for (int row = r0 + lane; row < r1; row += 64) {  // r0, r1 multiples of 64: every lane takes a row
         const bool real = row < n;
@@ -183,12 +194,12 @@ __global__ __launch_bounds__(256) void ds_assemble_kernel(DsArgs a, const int2* 
             if (!real || c >= n) {
                 v = row == c ? 1.0 : 0.0;
             } else {
-                const int oc = ord ? ord[c] : c;
+                const int oc = col_o[cc];
                 double m = 0.0;
                 if (orow == oc) {
                     m = M[(size_t)oc * n + orow];  // s, from the pass's reduction
                 } else {
-                    const double2 pc = cfg[base + oc];
+                    const double2 pc = col_p[cc];
                     const bool lower = orow < oc;
                     const double dx = lower ? wrap(pr.x - pc.x) : wrap(pc.x - pr.x);
                     const double dy = lower ? wrap(pr.y - pc.y) : wrap(pc.y - pr.y);

@@ -590,29 +590,21 @@ __global__ __launch_bounds__(256, 1) void lu_panel_kernel(const FacDesc* __restr
     const int T = tb[i];  // blocks below and right of the diagonal block that can be nonzero
     const double* dg = A + m.a_off + (size_t)j0 * lda + j0;
     const int t = threadIdx.x, lo = t & 63, q = t >> 6;
-    const int item = g0 - pre[i] + wv;
-    const bool live = item < 2 * T;                // else padding
-    const bool rowblk = item < T;                  // L21 block (below) or U12 block (right)
-    const int b = (rowblk ? item : item - T) + 1;  // block index past the diagonal
-    double* blk = A + m.a_off + (size_t)j0 * lda + (j0 + kFB * b);
-    double* col = A + m.a_off + (size_t)(j0 + kFB * b + lo) * lda + j0;
-    // the block's own entries first: their loads are in flight while the diagonal block is staged
-    double x[kFB], F[2][4];  // F[s & 1][k]: step s's factor entries 16 k + L
-    if (live && rowblk) {
-#pragma unroll
-        for (int c = 0; c < kFB; ++c) x[c] = blk[(size_t)c * lda + lo];
-    } else if (live) {
-#pragma unroll
-        for (int r = 0; r < kFB; ++r) x[r] = col[r];
-    }
     for (int c = q; c < kFB; c += 4) Dg[c * kFLd + lo] = dg[(size_t)c * lda + lo];
     __syncthreads();
-    if (!live) return;
+    const int item = g0 - pre[i] + wv;
+    if (item >= 2 * T) return;                     // padding
+    const bool rowblk = item < T;                  // L21 block (below) or U12 block (right)
+    const int b = (rowblk ? item : item - T) + 1;  // block index past the diagonal
     const int L = lo & 15;
+    double x[kFB], F[2][4];  // F[s & 1][k]: step s's factor entries 16 k + L
     if (rowblk) {
+        double* blk = A + m.a_off + (size_t)j0 * lda + (j0 + kFB * b);
         double R[4];  // lane 16 g + L of R[k]: 1 / U11(16 k + L, 16 k + L)
 #pragma unroll
         for (int k = 0; k < 4; ++k) R[k] = 1.0 / Dg[(16 * k + L) * kFLd + 16 * k + L];
+#pragma unroll
+        for (int c = 0; c < kFB; ++c) x[c] = blk[(size_t)c * lda + lo];
 #pragma unroll
         for (int k = 0; k < 4; ++k) F[0][k] = Dg[(16 * k + L) * kFLd];  // U11(0, 16 k + L)
         static_for<kFB>([&](auto cc) {
@@ -631,6 +623,9 @@ __global__ __launch_bounds__(256, 1) void lu_panel_kernel(const FacDesc* __restr
 #pragma unroll
         for (int c = 0; c < kFB; ++c) blk[(size_t)c * lda + lo] = x[c];
     } else {
+        double* col = A + m.a_off + (size_t)(j0 + kFB * b + lo) * lda + j0;
+#pragma unroll
+        for (int r = 0; r < kFB; ++r) x[r] = col[r];
 #pragma unroll
         for (int k = 0; k < 4; ++k) F[0][k] = Dg[16 * k + L];  // L11(16 k + L, 0)
         static_for<kFB - 1>([&](auto rr) {
