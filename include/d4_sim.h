@@ -39,7 +39,8 @@ typedef struct {
                                 0 host sparse LU (left-looking, minimum-degree order, threshold
                                   pivoting u = 0.1 preferring the diagonal);
                                 1 dense LU on the GPU: the matrix in reverse Cuthill-McKee order
-                                  where that halves the work, padded to a multiple of 64, factored
+                                  where that halves the work (kept while the pattern's envelope in it
+                                  grows by at most 1/8), padded to a multiple of 64, factored
                                   first by a blocked LU without row interchanges inside its block
                                   envelope; kept when every multiplier is at most 1000 (threshold
                                   u = 0.001, the diagonal preferred), otherwise factored again with
