@@ -46,6 +46,7 @@
 #include <climits>
 #include <cmath>
 #include <map>
+#include <set>
 #include <mutex>
 #include <thread>
 #include <type_traits>
@@ -410,8 +411,8 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
                 }
             } else {
                 // x_c = v_c / u_cc, correctly rounded, from the reciprocal: q = v r, then q + (v - q u) r
-                // (exact residual by fma) is the quotient IEEE division gives, with no division on the
-                // step-to-step chain
+                // (exact residual by fma) is the quotient IEEE division gives (away from the overflow and
+                // underflow thresholds), with no division on the step-to-step chain
                 const double ul = Ddiag[(s & 1) * kSB + l], rl = 1.0 / ul;
 #pragma unroll
                 for (int kk = 0; kk < kAhead; ++kk) d[kSB - 1 - kk] = D[(kSB - 1 - kk) * kSB + l];
@@ -1096,7 +1097,11 @@ public:
         // (a dense device matrix keeps its order and is factored whole)
         plan_.resize(K);
         cur_.assign(K, nullptr);
-        for (int k = 0; k < K; ++k) cur_[k] = m[k].key >= 0 ? &keyed_[m[k].key] : &plan_[k];
+        {
+            std::set<int> seen;  // a key named twice in one call: those matrices keep slot plans
+            for (int k = 0; k < K; ++k)
+                cur_[k] = m[k].key >= 0 && seen.insert(m[k].key).second ? &keyed_[m[k].key] : &plan_[k];
+        }
         long long aux_total = 0;
         if (!dense) {
             par(K, [&](int k) {
@@ -1108,7 +1113,7 @@ public:
                     return;
                 // a keyed matrix whose pattern changed keeps its order while that order's envelope of the new
                 // pattern stays within 1/8 of the envelope it was chosen with
-                const bool keep = m[k].key >= 0 && n > 0 && pl.n == n && pl.base_tiles > 0;
+                const bool keep = cur_[k] != &plan_[k] && n > 0 && pl.n == n && pl.base_tiles > 0;
                 pl.n = n;
                 pl.pat_p.assign(m[k].Ap, m[k].Ap + (n > 0 ? n + 1 : 0));
                 pl.pat_i.assign(m[k].Ai, m[k].Ai + (n > 0 ? m[k].Ap[n] : 0));

@@ -183,8 +183,8 @@ public:
         // >= 0: the caller's name for this matrix across calls (a realization, one simulation's solver).
         // Its order is then kept when its pattern changes while the pattern's envelope in that order stays
         // within 1/8 of the envelope the order was chosen with (a matrix's order then depends on its own
-        // sequence of patterns, not on the other matrices of a call); < 0: an order is kept only while the
-        // pattern of the matrix slot repeats exactly
+        // sequence of patterns, not on the other matrices of a call); < 0 (or a key named twice in one
+        // call): an order is kept only while the pattern of the matrix slot repeats exactly
         int key = -1;
     };
     // runs fn(k) for k in [0, count), possibly on several threads
