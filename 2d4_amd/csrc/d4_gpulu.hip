@@ -299,21 +299,23 @@ __device__ inline double lane_value(double v, int lane)  // v of `lane` (uniform
 // x := U^{-1} L^{-1} P b for the factors in A (getrf's layout: unit lower L
 // below the diagonal, U on and above), one workgroup per matrix.  Blocked by
 // kSB rows: the diagonal block is staged in LDS and solved by wave 0 (lane l
-// owns row l, the solved value of lane c broadcast at step c), then waves
-// 1.. update the rows outside the block with the block's columns,
-// x_r -= sum_c a_rc x_c.  A row is shared by P threads (P = 4, 2, 1 for
-// npad <= 1024, <= 2048, larger): thread k of the row accumulates the columns
-// c = k, k + P, ... in ascending order, and the P partial sums are added as
-// (s0 + s1) + (s2 + s3) in every thread of the row.  The order of operations
+// owns row l, the solved value of lane c broadcast at step c), then the
+// other rows are updated with the block's columns,
+// x_r -= sum_c a_rc x_c.  For npad <= 1024 a row is shared by four threads:
+// thread k of the row accumulates the columns c = k, k + 4, ... in ascending
+// order, and the four partial sums are added as (s0 + s1) + (s2 + s3) in
+// every thread of the row; larger matrices sum a row in one thread, c
+// ascending.  The order of operations
 // is fixed by the matrix alone.  x lives in LDS (kLds) or in global scratch;
 // the arithmetic is the same either way.  A reordered matrix gathers b
 // through ord and scatters x back through it; with an envelope the updates
 // stop at its bounds, past which the factors' columns are zero (a zero sum
 // leaves x_r as it is, so the bits are those of the full sweep); a matrix
 // whose full flag is set is swept whole.
-// Latency: every update thread loads its first row's factor entries of the
-// block (up to 32) before the block's diagonal solve, which they do not
-// depend on, so that the loads are in flight during it; the next diagonal
+// Latency: with four threads a row, every thread loads its first row's 16
+// factor entries of the block before the block's diagonal solve, which they
+// do not depend on, so that the loads are in flight during it (one thread a
+// row loads the row's 64 entries 32 at a time after it); the next diagonal
 // block is staged (by waves 1..15, into the other of two LDS buffers) at the
 // same time; wave 0 reads its entries of the block kAhead steps before their
 // use.
@@ -373,12 +375,14 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
     if (nblk > 0) stage(0, tid, kSolveThreads);
     __syncthreads();
 
-    // threads per row in the row updates, the columns each of them takes, and the rows of one round
-    const int P = m.npad <= 1024 ? 4 : m.npad <= 2048 ? 2 : 1;
-    const int W = kSB / P, rows = (kSolveThreads - 64) / P;
+    // npad <= 1024: each row update shared by four threads of waves 1..15 (wave 0 solves the diagonal
+    // blocks), thread k of a row taking the columns k, k + 4, ..., its 16 entries of the row loaded before
+    // the diagonal solve; larger: one thread of the whole workgroup per row, its 64 entries 32 at a time
+    const bool split = m.npad <= 1024;
     const int ut = tid - 64;
-    const int k = ut >= 0 ? ut % P : 0;
-    const int q = ut >= 0 ? ut / P : (1 << 29);  // wave 0 takes no rows
+    const int k = split && ut >= 0 ? ut & 3 : 0;
+    const int q = split ? (ut >= 0 ? ut >> 2 : (1 << 29)) : tid;
+    const int rows = split ? (kSolveThreads - 64) / 4 : kSolveThreads;
     for (int s = 0; s < 2 * nblk; ++s) {
         const int pass = s < nblk ? 0 : 1;  // 0: L y = P b top down, 1: U x = y bottom up
         const int bi = pass == 0 ? s : 2 * nblk - 1 - s;
@@ -388,12 +392,11 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
         const int r1 = pass == 0 ? (env ? min(n, (env[bi] + 1) * kSB) : n) : jb;
         const bool whole = B == kSB;
         const int rf = r0 + q;
-        const size_t step = (size_t)P * lda;  // from one of a thread's columns to the next
-        double av[kPre];  // the first kPre of the first row's entries
-        if (whole && rf < r1) {
+        double av[kPre];  // split: the first row's 16 entries
+        if (split && whole && rf < r1) {
             const double* a = Am + (size_t)(jb + k) * lda + rf;
 #pragma unroll
-            for (int c = 0; c < kPre; ++c) av[c] = a[c * step];
+            for (int c = 0; c < kPre; ++c) av[c] = a[(size_t)(4 * c) * lda];
         }
         if (tid < 64) {
             // lane l owns row l; the staged block is zero outside the triangle a step updates, so every
@@ -432,31 +435,43 @@ __global__ __launch_bounds__(kSolveThreads) void lu_solve_kernel(const SolveDesc
             stage(s + 1, tid - 64, kSolveThreads - 64);
         }
         __syncthreads();
-        // rows r0 + q, r0 + q + rows, ...: the same rows for the P threads of a row
-        for (int r = rf; r < r1; r += rows) {
-            const double* a = Am + (size_t)(jb + k) * lda + r;
-            const double* xk = x + jb + k;
-            double acc = 0.0;
-            if (whole) {
-                if (r != rf) {
+        if (split) {
+            // rows r0 + q, r0 + q + rows, ...: the same rows for the four threads of a row
+            for (int r = rf; r < r1; r += rows) {
+                const double* a = Am + (size_t)(jb + k) * lda + r;
+                double acc = 0.0;
+                if (whole) {
+                    if (r != rf) {
 #pragma unroll
-                    for (int c = 0; c < kPre; ++c) av[c] = a[c * step];
+                        for (int c = 0; c < kPre; ++c) av[c] = a[(size_t)(4 * c) * lda];
+                    }
+#pragma unroll
+                    for (int c = 0; c < kPre; ++c) acc = fma(av[c], x[jb + k + 4 * c], acc);
+                } else {
+                    for (int c = k; c < B; c += 4) acc = fma(a[(size_t)(c - k) * lda], x[jb + c], acc);
                 }
-#pragma unroll
-                for (int c = 0; c < kPre; ++c) acc = fma(av[c], xk[P * c], acc);
-                for (int h = kPre; h < W; h += kPre) {
-                    double bv[kPre];
-#pragma unroll
-                    for (int c = 0; c < kPre; ++c) bv[c] = a[(h + c) * step];
-#pragma unroll
-                    for (int c = 0; c < kPre; ++c) acc = fma(bv[c], xk[P * (h + c)], acc);
-                }
-            } else {
-                for (int c = k; c < B; c += P) acc = fma(a[(size_t)(c - k) * lda], x[jb + c], acc);
+                acc += __shfl_xor(acc, 1);
+                acc += __shfl_xor(acc, 2);
+                if (k == 0) x[r] -= acc;
             }
-            if (P >= 2) acc += __shfl_xor(acc, 1);
-            if (P >= 4) acc += __shfl_xor(acc, 2);
-            if (k == 0) x[r] -= acc;
+        } else {
+            for (int r = rf; r < r1; r += rows) {
+                const double* a = Am + (size_t)jb * lda + r;
+                double acc = 0.0;
+                if (whole) {
+#pragma unroll
+                    for (int h = 0; h < kSB; h += 2 * kPre) {
+                        double bv[2 * kPre];
+#pragma unroll
+                        for (int c = 0; c < 2 * kPre; ++c) bv[c] = a[(size_t)(h + c) * lda];
+#pragma unroll
+                        for (int c = 0; c < 2 * kPre; ++c) acc = fma(bv[c], x[jb + h + c], acc);
+                    }
+                } else {
+                    for (int c = 0; c < B; ++c) acc = fma(a[(size_t)c * lda], x[jb + c], acc);
+                }
+                x[r] -= acc;
+            }
         }
         __syncthreads();
     }
