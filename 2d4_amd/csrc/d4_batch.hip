@@ -640,6 +640,7 @@ int run_linear(d4_ens* e, const std::vector<int>& act, PassJob::Kind kind)
                 const sdddstCore::SimulationData& d = *e->data[gpu[k]];
                 m[k] = {(int)d.dc, d.Ap.data(), d.Ai.data(), d.Ax.data()};
                 m[k].key = gpu[k];  // the realization: its order follows its own sequence of patterns
+                m[k].symmetric = true;  // a step matrix's pattern (V is stored iff V^T is)
                 if (on_dev) {
                     const int q = pos[gpu[k]];
                     m[k].dAp = static_cast<const int*>(e->bp->Ap_d.p) + e->csc_off[q];

@@ -1134,13 +1134,14 @@ public:
                 pl.pat_i.assign(m[k].Ai, m[k].Ai + (n > 0 ? m[k].Ap[n] : 0));
                 if (keep) {
                     BlockEnvelope e = block_envelope(n, desc_h_[k].npad, kFB, m[k].Ap, m[k].Ai,
-                                                     pl.re ? pl.inv.data() : nullptr);
+                                                     pl.re ? pl.inv.data() : nullptr, m[k].symmetric);
                     if (8 * e.tiles <= 9 * pl.base_tiles) {
                         pl.env = std::move(e);
                         return;
                     }
                 }
-                pl.re = n > 0 && band_order(n, desc_h_[k].npad, kFB, m[k].Ap, m[k].Ai, pl.ord, pl.inv, pl.env);
+                pl.re = n > 0 && band_order(n, desc_h_[k].npad, kFB, m[k].Ap, m[k].Ai, pl.ord, pl.inv, pl.env,
+                                            m[k].symmetric);
                 pl.base_tiles = pl.env.tiles;
             });
             for (int k = 0; k < K; ++k) {
@@ -1528,6 +1529,7 @@ public:
         if (n < 0 || (n > 0 && !Ap)) return -1;
         BatchDenseLU::Matrix m{n, Ap, Ai, Ax};
         m.key = 0;  // one simulation's matrices: its order follows its own sequence of patterns
+        m.symmetric = true;  // the simulation's step matrices (V is stored iff V^T is)
         unsigned char s = 0, u = 0;
         const int rc = lu_.factor(1, &m, &s, &u);
         if (rc < 0) return rc;

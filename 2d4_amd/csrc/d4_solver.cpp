@@ -178,10 +178,24 @@ static BlockEnvelope envelope_from_rows(const std::vector<int>& f, int npad, int
     return e;
 }
 
-BlockEnvelope block_envelope(int n, int npad, int bs, const int* Ap, const int* Ai, const int* inv)
+BlockEnvelope block_envelope(int n, int npad, int bs, const int* Ap, const int* Ai, const int* inv, bool symmetric)
 {
     std::vector<int> f(npad);
     for (int i = 0; i < npad; ++i) f[i] = i;
+    if (symmetric) {
+        // row i's entries are column i's (the pattern is symmetric): f(new c) is the lowest new index among
+        // column c's rows, one gather pass and no scattered updates
+        for (int c = 0; c < n; ++c) {
+            const int nc = inv ? inv[c] : c;
+            int lo = nc;
+            if (inv)
+                for (int k = Ap[c]; k < Ap[c + 1]; ++k) lo = std::min(lo, inv[Ai[k]]);
+            else
+                for (int k = Ap[c]; k < Ap[c + 1]; ++k) lo = std::min(lo, Ai[k]);
+            f[nc] = lo;
+        }
+        return envelope_from_rows(f, npad, bs);
+    }
     for (int c = 0; c < n; ++c) {
         const int nc = inv ? inv[c] : c;
         for (int k = Ap[c]; k < Ap[c + 1]; ++k) {
@@ -204,16 +218,16 @@ BlockEnvelope key_band_envelope(int n, int npad, int bs, const double* key, doub
 }
 
 bool band_order(int n, int npad, int bs, const int* Ap, const int* Ai, std::vector<int>& ord, std::vector<int>& inv,
-                BlockEnvelope& env)
+                BlockEnvelope& env, bool symmetric)
 {
-    env = block_envelope(n, npad, bs, Ap, Ai, nullptr);
+    env = block_envelope(n, npad, bs, Ap, Ai, nullptr, symmetric);
     ord.clear();
     inv.clear();
     if (n < 2 * bs || env.tiles == 0) return false;
     std::vector<int> o = rcm_order(n, Ap, Ai);
     std::vector<int> iv(n);
     for (int i = 0; i < n; ++i) iv[o[i]] = i;
-    BlockEnvelope e = block_envelope(n, npad, bs, Ap, Ai, iv.data());
+    BlockEnvelope e = block_envelope(n, npad, bs, Ap, Ai, iv.data(), symmetric);
     if (2 * e.tiles > env.tiles) return false;
     ord.swap(o);
     inv.swap(iv);

@@ -129,7 +129,10 @@ struct BlockEnvelope {
     std::vector<int> first, last, top;
     long long tiles = 0;
 };
-BlockEnvelope block_envelope(int n, int npad, int bs, const int* Ap, const int* Ai, const int* inv);
+// symmetric: the caller knows the pattern is symmetric (entry (i, j) iff (j, i)), which lets one gather pass
+// per column replace the scattered per-entry updates; the result is the same
+BlockEnvelope block_envelope(int n, int npad, int bs, const int* Ap, const int* Ai, const int* inv,
+                             bool symmetric = false);
 
 // The block envelope (as block_envelope) of every matrix whose rows and columns, sorted by a key
 // (key[0] <= key[1] <= ...), couple only where the keys differ by less than reach: a bound that holds for
@@ -145,7 +148,7 @@ BlockEnvelope key_band_envelope(int n, int npad, int bs, const double* key, doub
 // and inv (inv[old] = new) when it reorders; env is the envelope of the order
 // chosen.
 bool band_order(int n, int npad, int bs, const int* Ap, const int* Ai, std::vector<int>& ord, std::vector<int>& inv,
-                BlockEnvelope& env);
+                BlockEnvelope& env, bool symmetric = false);
 
 // Dense LU on the GPU: the CSC matrix is scattered into a device matrix of
 // order gpu_lu_padded(n) (identity beyond n) and factored there with
@@ -153,8 +156,9 @@ bool band_order(int n, int npad, int bs, const int* Ap, const int* Ai, std::vect
 // symmetric strategy: a blocked LU without interchanges, kept when every
 // multiplier is at most 1000 in magnitude, else rocSOLVER getrf), then solved there by a triangular-solve
 // kernel; one stream and rocBLAS handle per calling thread.  It is BatchDenseLU with one matrix, so a
-// matrix gets the same bits from either.  Defined in d4_gpulu.hip; null if the
-// device cannot be used.
+// matrix gets the same bits from either.  It is the simulation's solver: its matrices are one simulation's
+// sequence of step matrices, whose patterns are symmetric (Matrix::key and Matrix::symmetric set).  Defined
+// in d4_gpulu.hip; null if the device cannot be used.
 LinearSolver* new_gpu_dense_lu(int device);
 
 // order of the padded device matrix for an n x n system: n rounded up to a
@@ -186,6 +190,9 @@ public:
         // sequence of patterns, not on the other matrices of a call); < 0 (or a key named twice in one
         // call): an order is kept only while the pattern of the matrix slot repeats exactly
         int key = -1;
+        // the pattern is symmetric (entry (i, j) iff (j, i)), as the step matrices' are: the envelope is
+        // then found without scattered updates
+        bool symmetric = false;
     };
     // runs fn(k) for k in [0, count), possibly on several threads
     using ParallelFor = void (*)(void* user, int count, void (*fn)(void*, int), void* arg);

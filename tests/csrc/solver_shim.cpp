@@ -30,6 +30,19 @@ extern "C" int ts_band_order(int n, int npad, int bs, const int* Ap, const int* 
     return re ? 1 : 0;
 }
 
+// the block envelope of a pattern in an order (inv[old] = new, or null), by the general or the symmetric pass
+extern "C" void ts_block_envelope(int n, int npad, int bs, const int* Ap, const int* Ai, const int* inv, int symmetric,
+                                  int* first, int* last, int* top, long long* tiles)
+{
+    const d4host::BlockEnvelope e = d4host::block_envelope(n, npad, bs, Ap, Ai, inv, symmetric != 0);
+    for (int b = 0; b < npad / bs; ++b) {
+        first[b] = e.first[b];
+        last[b] = e.last[b];
+        top[b] = e.top[b];
+    }
+    *tiles = e.tiles;
+}
+
 // the envelope fixed by sorted keys that couple only within reach (host code)
 extern "C" void ts_key_band_envelope(int n, int npad, int bs, const double* key, double reach, int* first, int* last,
                                      int* top, long long* tiles)

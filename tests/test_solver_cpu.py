@@ -294,3 +294,32 @@ def test_column_order_kept_while_the_pattern_drifts_by_at_most_one_percent(ts, m
                             C.byref(nord), C.byref(nrep))
     assert rc == 0 and nord.value == orders and nrep.value == 0
     assert np.max(np.abs(x - np.linalg.solve(A2, b))) <= 1e-12
+
+
+@pytest.mark.parametrize("n", [70, 300, 1000])
+def test_symmetric_envelope_pass_equals_the_general_one(ts, n):
+    """block_envelope's pass for symmetric patterns (one gather per column, what the GPU LU uses for the step
+    matrices) gives the general pass's envelope, in the natural order and in random orders; on an
+    unsymmetric pattern the general pass still sees entries the symmetric one would miss."""
+    ts.ts_block_envelope.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 3 + [C.c_int] + \
+        [C.c_void_p] * 3 + [C.POINTER(C.c_longlong)]
+    rng = np.random.default_rng(n)
+    bs = 64
+    npad = (n + bs - 1) // bs * bs
+    nb = npad // bs
+    p = lambda a: C.c_void_p(a.ctypes.data) if a is not None else None  # noqa: E731
+
+    def env(A, inv, sym):
+        Ap, Ai, _ = _csc(A)
+        out = [np.empty(nb, np.int32) for _ in range(3)]
+        tiles = C.c_longlong(0)
+        ts.ts_block_envelope(n, npad, bs, p(Ap), p(Ai), p(inv), sym, p(out[0]), p(out[1]), p(out[2]),
+                             C.byref(tiles))
+        return [o.tolist() for o in out] + [tiles.value]
+
+    A = (_cutoff_pattern(n, rng) | np.eye(n, dtype=bool)).astype(float)
+    for inv in (None, rng.permutation(n).astype(np.int32), rng.permutation(n).astype(np.int32)):
+        assert env(A, inv, 1) == env(A, inv, 0)
+    U = np.eye(n)
+    U[n - 1, 0] = 1.0  # one entry far below the diagonal, without its mirror
+    assert env(U, None, 0) != env(U, None, 1)
