@@ -77,7 +77,7 @@ def cpu_baseline(n_single, n_omp, min_seconds):
     return out
 
 
-def ensemble_bench(pkg, ctx, steps, R, seed=4242, plain=True, solver=1):
+def ensemble_bench(pkg, ctx, steps, R, seed=4242, plain=True, solver=1, graphs=True):
     """Batched ensemble vs the plain one-realization-at-a-time GPU path on the
     same realizations, same number of lockstep iterations, same solver.
 
@@ -104,6 +104,9 @@ def ensemble_bench(pkg, ctx, steps, R, seed=4242, plain=True, solver=1):
     e.step(1)
     cfg, tune_ms = e.autotune()
     log(f"[bench] ensemble autotuned {cfg} ({tune_ms:.2f} ms per speed+Jacobian pass)")
+    if not graphs:  # every launch its own dispatch (hardware counters are not collected inside a graph)
+        e.set_config(use_graph=0)
+        cfg = e.config()
     t_before = e.timing()
     t_batched = 0.0
     done = 0
@@ -513,6 +516,8 @@ def main():
     ap.add_argument("--ens-solver", type=int, default=1, choices=(0, 1),
                     help="with --ens-only: 1 = GPU dense LU (default), 0 = host sparse LU for every realization")
     ap.add_argument("--no-ens", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--ens-no-graph", action="store_true",
+                    help="with --ens-only: launch every pass without hipGraph replays (for hardware counters)")
     ap.add_argument("--no-step", action="store_true", help="skip the N=1024 implicit-step measurement")
     ap.add_argument("--dstep-only", action="store_true",
                     help="only the device-resident step measurement (C2 at 1 and 64 realizations, the bench ensemble)")
@@ -580,7 +585,8 @@ def main():
         return
     if args.ens_only:
         ectx = pkg.Context(0)
-        print(json.dumps({"ensemble": ensemble_bench(pkg, ectx, args.ens_steps, args.ens_r, plain=False, solver=args.ens_solver)}), flush=True)
+        print(json.dumps({"ensemble": ensemble_bench(pkg, ectx, args.ens_steps, args.ens_r, plain=False, solver=args.ens_solver,
+                                                     graphs=not args.ens_no_graph)}), flush=True)
         return
     n = args.n
     ctx = pkg.Context(0)
